@@ -1,0 +1,1611 @@
+/*
+ * oracle.c -- CPU restatement of Patmol's two-electron hot path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; the product in
+ * patmol_amd/ never links, loads or calls it.
+ *
+ * Function -> reference it restates -> how it is pinned (tests/test_oracle_*.py):
+ *   or_basis_new        shell.c:47-77, basis.c:243-321   SF6 golden E (K9); (ss|ss) closed form
+ *   or_boys             mdr.c:6-34                        mpmath closed form, F_m(0), erf form,
+ *                                                         downward identity; reference formMDR (_ref)
+ *   or_rys              rys.c:2217-2243 (own method)      mpmath Golub-Welsch moments; Hermite
+ *                                                         large-T limit; reference Rys_findRoots (_ref)
+ *   or_hermite_e/_r     mdd.c:3-43, mdr.c:37-2163         reference formMDD/formMDR (_ref); E^00_0=1
+ *   or_eri_block        twoints.c:23-186, rys.c:2269-2335 numpy McMurchie-Davidson brute force;
+ *                                                         (ss|ss) closed form; 8-fold symmetry,
+ *                                                         translation, Schwarz; ERIs built from
+ *                                                         reference Rys primitives (_ref)
+ *   or_two_electron_ints twoints.c:188-418                integer recount of the canonical filter;
+ *                                                         SF6 golden E
+ *   or_*_form_gmatrix   scfaux.c:428-714                  einsum over the full 4-index tensor
+ *   or_one_electron_ints oneints.c:12-234                 reference one_electron_ints (_ref); SF6
+ *   or_dpm_ints         dpmints.c:18-199                  reference dpm_ints (_ref); s-type closed form
+ *   or_rhf_scf          rhfscf.c:21-174,432-471           SF6 golden: E, 43 iterations, E_nuc
+ *   or_uhf_scf          uhfscf.c:22-168,485-527           closed-shell UHF == RHF (parity unpinned
+ *                                                         for open shells: no reference answer)
+ *   or_quartet_tasks    (no reference counterpart: the reference has no Schwarz list, K4)
+ *                                                         integer brute force over all pairs
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define SRTERM 34.9868366552497250 /* 2 pi^(5/2) */
+
+/* ------------------------------------------------------------------ */
+/* basis                                                               */
+/* ------------------------------------------------------------------ */
+
+/* Contraction normalisation, restating shell.c:47-77. */
+static void normalize_shell(OShell *sh)
+{
+    const double twofact = 2.8284271247461903;   /* 2 sqrt 2 */
+    const double piterm = 5.568327996831707;     /* pi^(3/2) */
+    double lpow = 1.5 + sh->lsh, sum = 0.0;
+    for (int i = 0; i < sh->npr; ++i)
+        for (int j = 0; j < sh->npr; ++j)
+            sum += sh->co[i] * sh->co[j] *
+                   pow(sqrt(sh->al[i] * sh->al[j]) / (sh->al[i] + sh->al[j]), lpow);
+    sum = 1.0 / sqrt(sum * twofact);
+    for (int i = 0; i < sh->npr; ++i)
+        sh->co[i] *= sum * sqrt(pow(2.0 * sh->al[i], lpow) / piterm);
+}
+
+OBasis *or_basis_new(int ncen, const double *q, const double *xyz,
+                     int nshell, const int *npr, const int *lsh, const int *cen,
+                     const double *al, const double *co, int normalize)
+{
+    static const double dfact[] = {1., 1., 3., 15., 105., 945., 10395.};
+    OBasis *b = (OBasis *)calloc(1, sizeof(OBasis));
+    b->ncen = ncen;
+    b->nshell = nshell;
+    b->skipcen = ncen;
+    b->cen = (OCenter *)calloc(ncen, sizeof(OCenter));
+    for (int c = 0; c < ncen; ++c) {
+        b->cen[c].q = q[c];
+        for (int x = 0; x < 3; ++x) b->cen[c].r[x] = xyz[3 * c + x];
+    }
+    b->sh = (OShell *)calloc(nshell, sizeof(OShell));
+    int off = 0, p = 0;
+    b->maxl = 0;
+    for (int s = 0; s < nshell; ++s) {
+        OShell *sh = &b->sh[s];
+        sh->npr = npr[s];
+        sh->lsh = lsh[s];
+        sh->cen = cen[s];
+        sh->off = off;
+        sh->nst = (lsh[s] + 1) * (lsh[s] + 2) / 2;
+        off += sh->nst;
+        sh->al = (double *)malloc(sizeof(double) * npr[s]);
+        sh->co = (double *)malloc(sizeof(double) * npr[s]);
+        for (int k = 0; k < npr[s]; ++k) {
+            sh->al[k] = al[p + k];
+            sh->co[k] = co[p + k];
+        }
+        p += npr[s];
+        if (normalize) normalize_shell(sh);
+        if (sh->lsh > b->maxl) b->maxl = sh->lsh;
+    }
+    b->norb = off;
+    b->maxlst = (b->maxl + 1) * (b->maxl + 2) / 2;
+    /* Cartesian order and per-component factor, basis.c:294-319 */
+    for (int l = 0; l <= OR_MAXL; ++l) {
+        int lst = 0;
+        for (int lx = l; lx >= 0; --lx)
+            for (int ly = l - lx; ly >= 0; --ly) {
+                b->lxyz[l][lst][0] = lx;
+                b->lxyz[l][lst][1] = ly;
+                b->lxyz[l][lst][2] = l - lx - ly;
+                b->nf[l][lst] = 1.0 / sqrt(dfact[lx] * dfact[ly] * dfact[l - lx - ly]);
+                ++lst;
+            }
+    }
+    return b;
+}
+
+void or_basis_free(OBasis *b)
+{
+    if (!b) return;
+    for (int s = 0; s < b->nshell; ++s) {
+        free(b->sh[s].al);
+        free(b->sh[s].co);
+    }
+    free(b->sh);
+    free(b->cen);
+    free(b);
+}
+
+int or_basis_norb(const OBasis *b) { return b->norb; }
+
+void or_basis_coefs(const OBasis *b, double *co_out)
+{
+    int p = 0;
+    for (int s = 0; s < b->nshell; ++s)
+        for (int k = 0; k < b->sh[s].npr; ++k) co_out[p++] = b->sh[s].co[k];
+}
+
+void or_basis_set_center(OBasis *b, int icen, const double *r)
+{
+    memcpy(b->cen[icen].r, r, 3 * sizeof(double));
+}
+
+void or_basis_set_skipcen(OBasis *b, int skipcen) { b->skipcen = skipcen; }
+
+/* basis.c:174-195 */
+double or_nuclear_repulsion(const OBasis *b)
+{
+    double sum = 0.0;
+    for (int i = 0; i < b->ncen; ++i)
+        for (int j = i + 1; j < b->ncen; ++j) {
+            double x = b->cen[i].r[0] - b->cen[j].r[0];
+            double y = b->cen[i].r[1] - b->cen[j].r[1];
+            double z = b->cen[i].r[2] - b->cen[j].r[2];
+            sum += b->cen[i].q * b->cen[j].q / sqrt(x * x + y * y + z * z);
+        }
+    return sum;
+}
+
+/* ------------------------------------------------------------------ */
+/* Boys function, restating mdr.c:6-34: series for F_m then downward    */
+/* recursion; asymptotic form above T = 36 + 2m.                         */
+/* ------------------------------------------------------------------ */
+void or_boys(int m, double t, double *fm)
+{
+    const double tcrit = 36. + 2 * m;
+    const double sqrtpi = 0.88622692545275801365; /* sqrt(pi)/2 */
+    if (t > tcrit) {
+        fm[0] = sqrtpi / sqrt(t);
+        for (int i = 1; i <= m; i++) fm[i] = fm[i - 1] * (i - 0.5) / t;
+        return;
+    }
+    double mphalf = m + 0.5, term = 0.5 / mphalf, sum = term;
+    for (int i = 1; i <= 500; i++) {
+        term *= t / (mphalf + i);
+        sum += term;
+        if (term < 1e-14) break;
+    }
+    double expt = exp(-t);
+    fm[m] = sum * expt;
+    for (int i = m; i--;) fm[i] = (fm[i + 1] * 2.0 * t + expt) / (2.0 * i + 1.0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Rys quadrature nodes and weights.                                     */
+/* The reference evaluates piecewise fits (rys.c:328-2214) and a long-   */
+/* double moment method (rys.c:245-323).  This restatement uses its own  */
+/* method: the discretised Stieltjes procedure (Gautschi, "Orthogonal    */
+/* Polynomials: Computation and Approximation", 2004, sec. 2.2.3) on a   */
+/* composite Gauss-Legendre discretisation of e^{-T t^2} dt on [0,b],    */
+/* then the Golub-Welsch eigenproblem of the Jacobi matrix.              */
+/* Returns nodes as x = t^2 (the reference returns u = t^2/(1-t^2) and   */
+/* converts back at twoints.c:122) and weights summing to F_0(T).        */
+/* ------------------------------------------------------------------ */
+#define GL_M 20
+#define GL_P 4
+static double gl_x[GL_M], gl_w[GL_M];
+static int gl_ready = 0;
+
+static void gauss_legendre01(void)
+{
+    int m = GL_M;
+    for (int i = 0; i < (m + 1) / 2; ++i) {
+        double z = cos(M_PI * (i + 0.75) / (m + 0.5)), pp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= m; ++j) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = m * (z * p1 - p2) / (z * z - 1.0);
+            double z1 = z;
+            z = z1 - p1 / pp;
+            if (fabs(z - z1) < 1e-16) break;
+        }
+        gl_x[i] = 0.5 * (1.0 - z);
+        gl_x[m - 1 - i] = 0.5 * (1.0 + z);
+        gl_w[i] = gl_w[m - 1 - i] = 1.0 / ((1.0 - z * z) * pp * pp);
+    }
+    gl_ready = 1;
+}
+
+/* Implicit QL on a symmetric tridiagonal matrix (d diag, e sub-diagonal in
+ * e[1..n-1]); z tracks the first row of the eigenvector matrix.  Textbook
+ * algorithm (Golub & Van Loan, sec. 8.3). */
+static int tridiag_ql(int n, double *d, double *e, double *z)
+{
+    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                volatile double dd = fabs(d[m]) + fabs(d[m + 1]);
+                volatile double ed = fabs(e[m]) + dd;
+                if (ed == dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 60) return -1;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i], bb = c * e[i];
+                    r = hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * bb;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - bb;
+                    f = z[i + 1];
+                    z[i + 1] = s * z[i] + c * f;
+                    z[i] = c * z[i] - s * f;
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    return 0;
+}
+
+/* Optional root provider (tests only): lets a test swap in another root
+ * generator, e.g. the reference's fits, to separate root error from the rest
+ * of the pipeline.  Callers that set one must run single-threaded if the
+ * provider is not reentrant. */
+static int or_verbose = 0;
+void or_set_verbose(int v) { or_verbose = v; }
+typedef void (*or_root_fn)(int, double, double *, double *);
+static or_root_fn root_provider = 0;
+void or_set_root_provider(void *fn) { root_provider = (or_root_fn)fn; }
+
+int or_rys(int n, double T, double *xout, double *wout)
+{
+    if (root_provider) {
+        root_provider(n, T, xout, wout);
+        return 0;
+    }
+    if (!gl_ready) gauss_legendre01();
+    if (n < 1 || n > OR_MAXROOTS) return -1;
+    enum { M = GL_M * GL_P };
+    double X[M], W[M], Q[OR_MAXROOTS][M], r[M];
+    double b = 1.0;
+    if (T > 0.0) {
+        double s = (50.0 + 6.0 * n) / T;
+        if (s < 1.0) b = sqrt(s);
+    }
+    double h = b / GL_P, mu0 = 0.0;
+    for (int pnl = 0; pnl < GL_P; ++pnl)
+        for (int k = 0; k < GL_M; ++k) {
+            double t = h * (pnl + gl_x[k]);
+            X[pnl * GL_M + k] = t * t;
+            W[pnl * GL_M + k] = h * gl_w[k] * exp(-T * t * t);
+            mu0 += W[pnl * GL_M + k];
+        }
+    double a[OR_MAXROOTS], bb[OR_MAXROOTS + 1], z[OR_MAXROOTS];
+    for (int j = 0; j < M; ++j) Q[0][j] = 1.0 / sqrt(mu0);
+    bb[0] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double s = 0.0;
+        for (int j = 0; j < M; ++j) s += W[j] * X[j] * Q[k][j] * Q[k][j];
+        a[k] = s;
+        if (k == n - 1) break;
+        for (int j = 0; j < M; ++j) r[j] = (X[j] - a[k]) * Q[k][j] - (k ? bb[k] * Q[k - 1][j] : 0.0);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int q = 0; q <= k; ++q) {
+                double c = 0.0;
+                for (int j = 0; j < M; ++j) c += W[j] * r[j] * Q[q][j];
+                for (int j = 0; j < M; ++j) r[j] -= c * Q[q][j];
+            }
+        double nr = 0.0;
+        for (int j = 0; j < M; ++j) nr += W[j] * r[j] * r[j];
+        bb[k + 1] = sqrt(nr);
+        for (int j = 0; j < M; ++j) Q[k + 1][j] = r[j] / bb[k + 1];
+    }
+    double d[OR_MAXROOTS], e[OR_MAXROOTS];
+    for (int k = 0; k < n; ++k) {
+        d[k] = a[k];
+        e[k] = (k ? bb[k] : 0.0);
+        z[k] = (k == 0);
+    }
+    if (tridiag_ql(n, d, e, z)) return -2;
+    /* sort ascending */
+    for (int i = 0; i < n; ++i) {
+        int m = i;
+        for (int j = i + 1; j < n; ++j)
+            if (d[j] < d[m]) m = j;
+        double t = d[i]; d[i] = d[m]; d[m] = t;
+        t = z[i]; z[i] = z[m]; z[m] = t;
+    }
+    for (int i = 0; i < n; ++i) {
+        xout[i] = d[i];
+        wout[i] = mu0 * z[i] * z[i];
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rys VRR and HRR, restating RysRecur (rys.c:2269-2292) and RysShift   */
+/* (rys.c:2295-2335).                                                   */
+/* ------------------------------------------------------------------ */
+#define GMAX 9
+typedef struct {
+    double B00, B1, B1p, C, Cp;
+} RysCoef;
+
+static void rys_vrr(double G[GMAX][GMAX], int l12, int l34, const RysCoef *k)
+{
+    G[0][0] = 1.0;
+    G[0][1] = k->Cp;
+    G[1][0] = k->C;
+    G[1][1] = k->B00 + k->C * k->Cp;
+    if (l12 < 2 && l34 < 2) return;
+    for (int j = 1; j < l34; j++) {
+        G[0][j + 1] = j * k->B1p * G[0][j - 1] + k->Cp * G[0][j];
+        G[1][j + 1] = j * k->B1p * G[1][j - 1] + k->B00 * G[0][j] + k->Cp * G[1][j];
+    }
+    for (int i = 2; i <= l12; i++) {
+        G[i][0] = (i - 1) * k->B1 * G[i - 2][0] + k->C * G[i - 1][0];
+        G[i][1] = i * k->B00 * G[i - 1][0] + k->Cp * G[i][0];
+        for (int j = 1; j < l34; j++)
+            G[i][j + 1] = j * k->B1p * G[i][j - 1] + i * k->B00 * G[i - 1][j] + k->Cp * G[i][j];
+    }
+}
+
+static const double binom[GMAX][GMAX] = {
+    {1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}, {1, 5, 10, 10, 5, 1},
+    {1, 6, 15, 20, 15, 6, 1}, {1, 7, 21, 35, 35, 21, 7, 1}, {1, 8, 28, 56, 70, 56, 28, 8, 1}};
+
+static double rys_hrr(double abx, double cdx, double G[GMAX][GMAX], int l12, int l2, int l34, int l4)
+{
+    double sum = 0.0, x12t = 1.0;
+    for (int i = 0; i <= l2; i++) {
+        double x34t = binom[l2][i] * x12t;
+        for (int j = 0; j <= l4; j++) {
+            sum += binom[l4][j] * x34t * G[l12 - i][l34 - j];
+            x34t *= cdx;
+        }
+        x12t *= abx;
+    }
+    return sum;
+}
+
+/* Contracted shell quartet, restating calc_pteints (twoints.c:23-186).
+ * Shells s1..s4 at centres A..D, with l1>=l2 and l3>=l4 as the caller
+ * arranges (twoints.c:249-266,297-314).  out[((a*n2+b)*n3+c)*n4+d], each
+ * multiplied by the component normalisation product (twoints.c:356). */
+static void quartet_raw(const OBasis *bs, const OShell *s1, const OShell *s2, const OShell *s3,
+                        const OShell *s4, int cA, int cB, int cC, int cD, double *out)
+{
+    const double *a = bs->cen[cA].r, *b = bs->cen[cB].r, *c = bs->cen[cC].r, *d = bs->cen[cD].r;
+    const int l1 = s1->lsh, l2 = s2->lsh, l3 = s3->lsh, l4 = s4->lsh;
+    const int n1 = s1->nst, n2 = s2->nst, n3 = s3->nst, n4 = s4->nst;
+    const int l12 = l1 + l2, l34 = l3 + l4, nroots = (l12 + l34) / 2 + 1;
+    const int ntot = n1 * n2 * n3 * n4;
+    const double ab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    const double cd[3] = {c[0] - d[0], c[1] - d[1], c[2] - d[2]};
+    const double absq = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+    const double cdsq = cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2];
+    for (int i = 0; i < ntot; ++i) out[i] = 0.0;
+    double p[3], q[3], pa[3], qc[3], s12 = 1.0, s34 = 1.0;
+    if (cA == cB) {
+        memcpy(p, a, sizeof p);
+        memset(pa, 0, sizeof pa);
+    }
+    if (cC == cD) {
+        memcpy(q, c, sizeof q);
+        memset(qc, 0, sizeof qc);
+    }
+    double G[OR_MAXROOTS][3][GMAX][GMAX], rt[OR_MAXROOTS], wt[OR_MAXROOTS];
+    double *acc = (double *)calloc(ntot, sizeof(double));
+    for (int i = 0; i < s1->npr; ++i) {
+        double axp = s1->al[i], c1 = s1->co[i], f12 = 1.0;
+        int jend = s2->npr;
+        if (s1 == s2) {
+            f12 = 2.0;
+            jend = i + 1;
+        }
+        for (int j = 0; j < jend; ++j) {
+            if (i == j) f12 = 1.0;
+            double c12 = c1 * f12 * s2->co[j], bxp = s2->al[j];
+            double pxp = axp + bxp, abi = 1.0 / pxp;
+            if (cA != cB) {
+                s12 = exp(-axp * bxp * absq * abi);
+                for (int x = 0; x < 3; ++x) {
+                    p[x] = (axp * a[x] + bxp * b[x]) * abi;
+                    pa[x] = p[x] - a[x];
+                }
+            }
+            for (int k = 0; k < s3->npr; ++k) {
+                double cxp = s3->al[k], c3 = s3->co[k], f34 = 1.0;
+                int lend = s4->npr;
+                if (s3 == s4) {
+                    f34 = 2.0;
+                    lend = k + 1;
+                }
+                for (int l = 0; l < lend; ++l) {
+                    if (k == l) f34 = 1.0;
+                    double c34 = c3 * f34 * s4->co[l], dxp = s4->al[l];
+                    double qxp = cxp + dxp, cdi = 1.0 / qxp;
+                    if (cC != cD) {
+                        s34 = exp(-cxp * dxp * cdsq * cdi);
+                        for (int x = 0; x < 3; ++x) {
+                            q[x] = (cxp * c[x] + dxp * d[x]) * cdi;
+                            qc[x] = q[x] - c[x];
+                        }
+                    }
+                    double txp = pxp + qxp;
+                    double sr = SRTERM * s12 * s34 * abi * cdi / sqrt(txp);
+                    if (sr < 1e-15) continue; /* twoints.c:110-111 */
+                    sr *= c12 * c34;
+                    double pq[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+                    double pq2 = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
+                    double t = pq2 * pxp * qxp / txp;
+                    or_rys(nroots, t, rt, wt);
+                    for (int r = 0; r < nroots; ++r) {
+                        double fff = rt[r] / txp; /* rt is t^2 = u/(1+u), twoints.c:122-123 */
+                        RysCoef kc;
+                        kc.B00 = 0.5 * fff;
+                        kc.B1 = (0.5 - kc.B00 * qxp) / pxp;
+                        kc.B1p = (0.5 - kc.B00 * pxp) / qxp;
+                        for (int x = 0; x < 3; ++x) {
+                            kc.C = pa[x] - qxp * pq[x] * fff;
+                            kc.Cp = qc[x] + pxp * pq[x] * fff;
+                            rys_vrr(G[r][x], l12, l34, &kc);
+                        }
+                    }
+                    int idx = 0;
+                    for (int ia = 0; ia < n1; ++ia) {
+                        const int *A = bs->lxyz[l1][ia];
+                        for (int ib = 0; ib < n2; ++ib) {
+                            const int *B = bs->lxyz[l2][ib];
+                            for (int ic = 0; ic < n3; ++ic) {
+                                const int *C = bs->lxyz[l3][ic];
+                                for (int id = 0; id < n4; ++id, ++idx) {
+                                    const int *D = bs->lxyz[l4][id];
+                                    double sum = 0.0;
+                                    for (int r = 0; r < nroots; ++r) {
+                                        sum += rys_hrr(ab[0], cd[0], G[r][0], A[0] + B[0], B[0], C[0] + D[0], D[0]) *
+                                               rys_hrr(ab[1], cd[1], G[r][1], A[1] + B[1], B[1], C[1] + D[1], D[1]) *
+                                               rys_hrr(ab[2], cd[2], G[r][2], A[2] + B[2], B[2], C[2] + D[2], D[2]) *
+                                               wt[r];
+                                    }
+                                    acc[idx] += sum * sr;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    int idx = 0;
+    for (int ia = 0; ia < n1; ++ia)
+        for (int ib = 0; ib < n2; ++ib)
+            for (int ic = 0; ic < n3; ++ic)
+                for (int id = 0; id < n4; ++id, ++idx)
+                    out[idx] = acc[idx] * bs->nf[l1][ia] * bs->nf[l2][ib] * bs->nf[l3][ic] * bs->nf[l4][id];
+    free(acc);
+}
+
+/* Full block (IJ|KL) in the caller's shell order, swapping internally so the
+ * higher angular momentum leads each pair (twoints.c:249-266,297-314). */
+void or_eri_block(const OBasis *bs, int I, int J, int K, int L, double *out)
+{
+    const OShell *sI = &bs->sh[I], *sJ = &bs->sh[J], *sK = &bs->sh[K], *sL = &bs->sh[L];
+    int sw12 = sI->lsh < sJ->lsh, sw34 = sK->lsh < sL->lsh;
+    const OShell *s1 = sw12 ? sJ : sI, *s2 = sw12 ? sI : sJ;
+    const OShell *s3 = sw34 ? sL : sK, *s4 = sw34 ? sK : sL;
+    int n1 = s1->nst, n2 = s2->nst, n3 = s3->nst, n4 = s4->nst;
+    double *raw = (double *)malloc(sizeof(double) * n1 * n2 * n3 * n4);
+    quartet_raw(bs, s1, s2, s3, s4, s1->cen, s2->cen, s3->cen, s4->cen, raw);
+    int nI = sI->nst, nJ = sJ->nst, nK = sK->nst, nL = sL->nst;
+    for (int i = 0; i < nI; ++i)
+        for (int j = 0; j < nJ; ++j)
+            for (int k = 0; k < nK; ++k)
+                for (int l = 0; l < nL; ++l) {
+                    int a = sw12 ? j : i, b = sw12 ? i : j, c = sw34 ? l : k, d = sw34 ? k : l;
+                    out[((i * nJ + j) * nK + k) * nL + l] = raw[((a * n2 + b) * n3 + c) * n4 + d];
+                }
+    free(raw);
+}
+
+void or_free(void *p) { free(p); }
+
+/* ------------------------------------------------------------------ */
+/* Canonical enumeration, restating two_electron_ints (twoints.c:188-418): */
+/* shell loops ish>=start, jsh<=ish, ksh<=ish, lsh<=ksh; component filter  */
+/* jr<=ir, kr<=ir, lr<=kr, (ir==kr => lr<=jr) (316-337); write filter       */
+/* |v| > 1e-14 (374-379).  Records keep the reference's order; threads    */
+/* split the ish loop and are concatenated in ish order.                   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    OERINT *v;
+    int64_t n, cap;
+} RecVec;
+
+static void rec_push(RecVec *r, double val, int i, int j, int k, int l)
+{
+    if (r->n == r->cap) {
+        r->cap = r->cap ? 2 * r->cap : 4096;
+        r->v = (OERINT *)realloc(r->v, sizeof(OERINT) * r->cap);
+    }
+    OERINT *e = &r->v[r->n++];
+    e->val = val;
+    e->i = (unsigned short)i;
+    e->j = (unsigned short)j;
+    e->k = (unsigned short)k;
+    e->l = (unsigned short)l;
+}
+
+static void canonical_shell_row(const OBasis *bs, int ish, RecVec *rv, int64_t *counts, double *blk)
+{
+    const OShell *sh = bs->sh;
+    for (int jsh = 0; jsh <= ish; ++jsh)
+        for (int ksh = 0; ksh <= ish; ++ksh)
+            for (int lsh = 0; lsh <= ksh; ++lsh) {
+                const OShell *s1 = &sh[ish], *s2 = &sh[jsh], *s3 = &sh[ksh], *s4 = &sh[lsh];
+                int knt = 0;
+                for (int ils = 0; ils < s1->nst && !knt; ++ils) {
+                    int ir = s1->off + ils;
+                    for (int jls = 0; jls < s2->nst && !knt; ++jls) {
+                        int jr = s2->off + jls;
+                        if (jr > ir) break;
+                        for (int kls = 0; kls < s3->nst && !knt; ++kls) {
+                            int kr = s3->off + kls;
+                            if (kr > ir) break;
+                            for (int lls = 0; lls < s4->nst; ++lls) {
+                                int lr = s4->off + lls;
+                                if (lr > kr) break;
+                                if (ir == kr && lr > jr) break;
+                                knt = 1;
+                                break;
+                            }
+                        }
+                    }
+                }
+                if (counts) counts[0] += 1; /* loop tuples */
+                if (!knt) continue;
+                if (counts) counts[1] += 1; /* shell quartets with >=1 canonical component */
+                if (!rv && !counts) continue;
+                if (rv) or_eri_block(bs, ish, jsh, ksh, lsh, blk);
+                int nJ = s2->nst, nK = s3->nst, nL = s4->nst;
+                for (int ils = 0; ils < s1->nst; ++ils) {
+                    int ir = s1->off + ils;
+                    for (int jls = 0; jls < s2->nst; ++jls) {
+                        int jr = s2->off + jls;
+                        if (jr > ir) break;
+                        for (int kls = 0; kls < s3->nst; ++kls) {
+                            int kr = s3->off + kls;
+                            if (kr > ir) break;
+                            for (int lls = 0; lls < s4->nst; ++lls) {
+                                int lr = s4->off + lls;
+                                if (lr > kr) break;
+                                if (ir == kr && lr > jr) break;
+                                if (counts) counts[2] += 1; /* canonical function integrals */
+                                if (rv) {
+                                    double v = blk[((ils * nJ + jls) * nK + kls) * nL + lls];
+                                    if (fabs(v) > 1e-14) rec_push(rv, v, ir, jr, kr, lr);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+}
+
+int64_t or_two_electron_ints(const OBasis *bs, int start, OERINT **out)
+{
+    int ns = bs->nshell;
+    RecVec *rows = (RecVec *)calloc(ns, sizeof(RecVec));
+    int ml = bs->maxlst;
+    if (!gl_ready) gauss_legendre01();
+#pragma omp parallel if (!root_provider)
+    {
+        double *blk = (double *)malloc(sizeof(double) * ml * ml * ml * ml);
+#pragma omp for schedule(dynamic, 1)
+        for (int r = ns - 1; r >= start; --r) canonical_shell_row(bs, r, &rows[r], NULL, blk);
+        free(blk);
+    }
+    int64_t n = 0;
+    for (int r = start; r < ns; ++r) n += rows[r].n;
+    OERINT *all = (OERINT *)malloc(sizeof(OERINT) * (n ? n : 1));
+    int64_t p = 0;
+    for (int r = start; r < ns; ++r) {
+        if (rows[r].n) memcpy(all + p, rows[r].v, sizeof(OERINT) * rows[r].n);
+        p += rows[r].n;
+        free(rows[r].v);
+    }
+    free(rows);
+    *out = all;
+    return n;
+}
+
+/* counts[0] = loop tuples, [1] = shell quartets with a canonical component,
+ * [2] = canonical function integrals (before the write filter). */
+void or_eri_count(const OBasis *bs, int start, int64_t *counts)
+{
+    counts[0] = counts[1] = counts[2] = 0;
+    for (int r = start; r < bs->nshell; ++r) canonical_shell_row(bs, r, NULL, counts, NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* G build: 8-fold packed scatter, restating rhf_form_gmatrix           */
+/* (scfaux.c:428-552) and uhf_form_gmatrix (scfaux.c:554-714).          */
+/* ------------------------------------------------------------------ */
+static inline void rhf_scatter(const OERINT *e, const double *Pmat, double *Gmat)
+{
+    double val = e->val;
+    int i = e->i, j = e->j, k = e->k, l = e->l;
+    int ii = (i * (i + 1)) / 2, ij = ii + j, ik = ii + k, il = ii + l;
+    int kl = (k * (k + 1)) / 2 + l;
+    int jk = (j > k) ? (j * (j + 1)) / 2 + k : (k * (k + 1)) / 2 + j;
+    int jl = (j > l) ? (j * (j + 1)) / 2 + l : (l * (l + 1)) / 2 + j;
+    double da = val * 2.0 * Pmat[ij];
+    double db = val * 2.0 * Pmat[kl];
+    double sjl = val * Pmat[ik], sjk = val * Pmat[il], sik = val * Pmat[jl], sil = val * Pmat[jk];
+    if (k != l) {
+        db = db + db;
+        Gmat[ik] -= sik;
+        if (i != j && j >= k) Gmat[jk] -= sjk;
+    }
+    Gmat[il] -= sil;
+    Gmat[ij] += db;
+    if (i != j && j >= l) Gmat[jl] -= sjl;
+    if (ij != kl) {
+        if (i != j) da = da + da;
+        if (j <= k) {
+            Gmat[jk] -= sjk;
+            if (i != j && i <= k) Gmat[ik] -= sik;
+            if (k != l && j <= l) Gmat[jl] -= sjl;
+        }
+        Gmat[kl] += da;
+    }
+}
+
+void or_rhf_form_gmatrix(const OERINT *ints, int64_t n, const double *P, double *G)
+{
+    for (int64_t x = 0; x < n; ++x) rhf_scatter(&ints[x], P, G);
+}
+
+void or_rhf_form_gmatrix_mt(const OERINT *ints, int64_t n, int norb, const double *P, double *G,
+                            int nthreads)
+{
+    if (nthreads <= 1) {
+        or_rhf_form_gmatrix(ints, n, P, G);
+        return;
+    }
+    int64_t no2 = (int64_t)norb * (norb + 1) / 2;
+    double *part = (double *)calloc((size_t)nthreads * no2, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+        int t = 0, nt = 1;
+#endif
+        int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+        double *g = part + (size_t)t * no2;
+        for (int64_t x = lo; x < hi; ++x) rhf_scatter(&ints[x], P, g);
+    }
+    for (int t = 0; t < nthreads; ++t)
+        for (int64_t i = 0; i < no2; ++i) G[i] += part[(size_t)t * no2 + i];
+    free(part);
+}
+
+void or_uhf_form_gmatrix(const OERINT *ints, int64_t n, const double *PA, const double *PB,
+                         double *GA, double *GB)
+{
+    for (int64_t x = 0; x < n; ++x) {
+        const OERINT *e = &ints[x];
+        double val = e->val;
+        int i = e->i, j = e->j, k = e->k, l = e->l;
+        int ii = (i * (i + 1)) / 2, ij = ii + j, ik = ii + k, il = ii + l;
+        int kl = (k * (k + 1)) / 2 + l;
+        int jk = (j > k) ? (j * (j + 1)) / 2 + k : (k * (k + 1)) / 2 + j;
+        int jl = (j > l) ? (j * (j + 1)) / 2 + l : (l * (l + 1)) / 2 + j;
+        double da = val * (PA[ij] + PB[ij]);
+        double db = val * (PA[kl] + PB[kl]);
+        double sjlA = val * PA[ik], sjkA = val * PA[il], sikA = val * PA[jl], silA = val * PA[jk];
+        double sjlB = val * PB[ik], sjkB = val * PB[il], sikB = val * PB[jl], silB = val * PB[jk];
+        if (k != l) {
+            db = db + db;
+            GA[ik] -= sikA;
+            GB[ik] -= sikB;
+            if (i != j && j >= k) {
+                GA[jk] -= sjkA;
+                GB[jk] -= sjkB;
+            }
+        }
+        GA[il] -= silA;
+        GA[ij] += db;
+        GB[il] -= silB;
+        GB[ij] += db;
+        if (i != j && j >= l) {
+            GA[jl] -= sjlA;
+            GB[jl] -= sjlB;
+        }
+        if (ij != kl) {
+            if (i != j) da = da + da;
+            if (j <= k) {
+                GA[jk] -= sjkA;
+                GB[jk] -= sjkB;
+                if (i != j && i <= k) {
+                    GA[ik] -= sikA;
+                    GB[ik] -= sikB;
+                }
+                if (k != l && j <= l) {
+                    GA[jl] -= sjlA;
+                    GB[jl] -= sjlB;
+                }
+            }
+            GA[kl] += da;
+            GB[kl] += da;
+        }
+    }
+}
+
+/* Direct G build: the same enumeration and scatter, ERIs recomputed and
+ * consumed on the fly (no store).  Threads take ish rows round-robin, the
+ * CPU analogue of the reference's rank split (twoints.c:278-286), each
+ * into a private G that is summed in thread order.  max_quartets > 0 stops
+ * after that many shell quartets (a bounded timing sample). */
+void or_rhf_direct_gmatrix(const OBasis *bs, int start, const double *P, double *G, int nthreads,
+                           int64_t max_quartets, int64_t *done_quartets)
+{
+    int ns = bs->nshell, ml = bs->maxlst;
+    int64_t no2 = (int64_t)bs->norb * (bs->norb + 1) / 2;
+    if (nthreads < 1) nthreads = 1;
+    if (!gl_ready) gauss_legendre01();
+    double *part = (double *)calloc((size_t)nthreads * no2, sizeof(double));
+    int64_t done = 0;
+    int stop = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : done)
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num();
+#else
+        int t = 0;
+#endif
+        double *g = part + (size_t)t * no2;
+        double *blk = (double *)malloc(sizeof(double) * ml * ml * ml * ml);
+#pragma omp for schedule(dynamic, 1)
+        for (int ish = ns - 1; ish >= start; --ish) {
+            if (stop) continue;
+            const OShell *sh = bs->sh;
+            for (int jsh = 0; jsh <= ish; ++jsh)
+                for (int ksh = 0; ksh <= ish; ++ksh)
+                    for (int lsh = 0; lsh <= ksh; ++lsh) {
+                        const OShell *s1 = &sh[ish], *s2 = &sh[jsh], *s3 = &sh[ksh], *s4 = &sh[lsh];
+                        int nJ = s2->nst, nK = s3->nst, nL = s4->nst, computed = 0;
+                        for (int ils = 0; ils < s1->nst; ++ils) {
+                            int ir = s1->off + ils;
+                            for (int jls = 0; jls < s2->nst; ++jls) {
+                                int jr = s2->off + jls;
+                                if (jr > ir) break;
+                                for (int kls = 0; kls < s3->nst; ++kls) {
+                                    int kr = s3->off + kls;
+                                    if (kr > ir) break;
+                                    for (int lls = 0; lls < s4->nst; ++lls) {
+                                        int lr = s4->off + lls;
+                                        if (lr > kr) break;
+                                        if (ir == kr && lr > jr) break;
+                                        if (!computed) {
+                                            or_eri_block(bs, ish, jsh, ksh, lsh, blk);
+                                            computed = 1;
+                                            ++done;
+                                        }
+                                        double v = blk[((ils * nJ + jls) * nK + kls) * nL + lls];
+                                        if (fabs(v) > 1e-14) {
+                                            OERINT e;
+                                            e.val = v;
+                                            e.i = ir; e.j = jr; e.k = kr; e.l = lr;
+                                            rhf_scatter(&e, P, g);
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+            if (max_quartets > 0 && done > max_quartets / nthreads) stop = 1;
+        }
+        free(blk);
+    }
+    for (int t = 0; t < nthreads; ++t)
+        for (int64_t i = 0; i < no2; ++i) G[i] += part[(size_t)t * no2 + i];
+    free(part);
+    if (done_quartets) *done_quartets = done;
+}
+
+/* Rows of G = 2J[P] - K[P] from full (non-canonical) shell blocks:
+ * G[p,q] = sum_rs P[r,s] (2 (pq|rs) - (pr|qs)) for every function p of the
+ * listed shells and all q.  Pfull is the full symmetric N x N half-density.
+ * Grows is (sum of listed shells' nst) x N. */
+void or_g_rows(const OBasis *bs, const double *Pf, int nrows, const int *rows, double *Grows,
+               int nthreads)
+{
+    int ns = bs->nshell, N = bs->norb, ml = bs->maxlst;
+    if (nthreads < 1) nthreads = 1;
+    if (!gl_ready) gauss_legendre01();
+    int *roff = (int *)malloc(sizeof(int) * (nrows + 1));
+    roff[0] = 0;
+    for (int r = 0; r < nrows; ++r) roff[r + 1] = roff[r] + bs->sh[rows[r]].nst;
+    memset(Grows, 0, sizeof(double) * roff[nrows] * N);
+    for (int r = 0; r < nrows; ++r) {
+        int Ps = rows[r];
+        const OShell *sP = &bs->sh[Ps];
+        double *Gr = Grows + (size_t)roff[r] * N;
+#pragma omp parallel num_threads(nthreads)
+        {
+            double *blk = (double *)malloc(sizeof(double) * ml * ml * ml * ml);
+            double *gl = (double *)calloc((size_t)sP->nst * N, sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+            for (int Qs = 0; Qs < ns; ++Qs) {
+                const OShell *sQ = &bs->sh[Qs];
+                for (int Rs = 0; Rs < ns; ++Rs) {
+                    const OShell *sR = &bs->sh[Rs];
+                    for (int Ss = 0; Ss < ns; ++Ss) {
+                        const OShell *sS = &bs->sh[Ss];
+                        /* Coulomb (PQ|RS) */
+                        or_eri_block(bs, Ps, Qs, Rs, Ss, blk);
+                        int nQ = sQ->nst, nR = sR->nst, nS = sS->nst;
+                        for (int a = 0; a < sP->nst; ++a)
+                            for (int b = 0; b < nQ; ++b) {
+                                double s = 0.0;
+                                for (int c = 0; c < nR; ++c)
+                                    for (int d = 0; d < nS; ++d)
+                                        s += blk[((a * nQ + b) * nR + c) * nS + d] *
+                                             Pf[(size_t)(sR->off + c) * N + sS->off + d];
+                                gl[(size_t)a * N + sQ->off + b] += 2.0 * s;
+                            }
+                        /* Exchange (PR|QS): here the loop variables play (P,Q,R,S) ->
+                         * block (P Q | R S) contributes -(pq|rs) P[q?]; use
+                         * G[p, r] -= sum_{q,s} (pq|rs) P[q,s]. */
+                        for (int a = 0; a < sP->nst; ++a)
+                            for (int c = 0; c < nR; ++c) {
+                                double s = 0.0;
+                                for (int b = 0; b < nQ; ++b)
+                                    for (int d = 0; d < nS; ++d)
+                                        s += blk[((a * nQ + b) * nR + c) * nS + d] *
+                                             Pf[(size_t)(sQ->off + b) * N + sS->off + d];
+                                gl[(size_t)a * N + sR->off + c] -= s;
+                            }
+                    }
+                }
+            }
+#pragma omp critical
+            for (size_t x = 0; x < (size_t)sP->nst * N; ++x) Gr[x] += gl[x];
+            free(gl);
+            free(blk);
+        }
+    }
+    free(roff);
+}
+
+/* ------------------------------------------------------------------ */
+/* McMurchie-Davidson E and R, restating formMDD (mdd.c:3-43) and the   */
+/* R_{tuv} that formMDR (mdr.c:37-2163) tabulates in closed form; here   */
+/* by the textbook recursion R^{n}_{t+1,u,v} = t R^{n+1}_{t-1,u,v} +     */
+/* X R^{n+1}_{t,u,v} (Helgaker, Jorgensen, Olsen eq. 9.9.18-20).         */
+/* ------------------------------------------------------------------ */
+#define EMAX 8
+/* d[(i*(EMAX) + j)*(2*EMAX+1) + n] : A power i, B power j, Hermite n */
+#define EIDX(i, j, n) (((i) * EMAX + (j)) * (2 * EMAX + 1) + (n))
+void or_hermite_e(double *d, int l1, int l2, double abi, double ax, double bx)
+{
+    int ltot = l1 + l2;
+    for (int i = 0; i <= l1; i++)
+        for (int j = 0; j <= l2; j++)
+            for (int n = 0; n <= ltot + 1; n++) d[EIDX(i, j, n)] = 0.0;
+    d[EIDX(0, 0, 0)] = 1.0;
+    for (int j = 1; j <= l2; j++)
+        for (int n = 0; n <= j; n++)
+            d[EIDX(0, j, n)] = (n ? abi * d[EIDX(0, j - 1, n - 1)] : 0.0) + bx * d[EIDX(0, j - 1, n)] +
+                               (n + 1) * d[EIDX(0, j - 1, n + 1)];
+    for (int i = 1; i <= l1; i++)
+        for (int j = 0; j <= l2; j++)
+            for (int n = 0; n <= i + j; n++)
+                d[EIDX(i, j, n)] = (n ? abi * d[EIDX(i - 1, j, n - 1)] : 0.0) + ax * d[EIDX(i - 1, j, n)] +
+                                   (n + 1) * d[EIDX(i - 1, j, n + 1)];
+}
+
+#define RMAX 13
+#define RIDX(t, u, v) (((t) * RMAX + (u)) * RMAX + (v))
+/* r[RIDX(t,u,v)] = sr * R^0_{tuv}(w, pq) for t+u+v <= ltot */
+void or_hermite_r(double *r, int ltot, double sr, double T, double w, const double *pq)
+{
+    double fm[RMAX + 1];
+    or_boys(ltot, T, fm);
+    /* R^n_{000} */
+    double (*Rn)[RMAX][RMAX][RMAX] = (double (*)[RMAX][RMAX][RMAX])malloc(sizeof(double) * (ltot + 1) * RMAX * RMAX * RMAX);
+    double term = -2.0 * w, st = 1.0;
+    for (int n = 0; n <= ltot; ++n) {
+        Rn[n][0][0][0] = sr * st * fm[n];
+        st *= term;
+    }
+    for (int n = ltot - 1; n >= 0; --n) {
+        int L = ltot - n;
+        for (int t = 0; t <= L; ++t)
+            for (int u = 0; u <= L - t; ++u)
+                for (int v = 0; v <= L - t - u; ++v) {
+                    if (t + u + v == 0) continue;
+                    double val;
+                    if (t > 0)
+                        val = pq[0] * Rn[n + 1][t - 1][u][v] + (t > 1 ? (t - 1) * Rn[n + 1][t - 2][u][v] : 0.0);
+                    else if (u > 0)
+                        val = pq[1] * Rn[n + 1][t][u - 1][v] + (u > 1 ? (u - 1) * Rn[n + 1][t][u - 2][v] : 0.0);
+                    else
+                        val = pq[2] * Rn[n + 1][t][u][v - 1] + (v > 1 ? (v - 1) * Rn[n + 1][t][u][v - 2] : 0.0);
+                    Rn[n][t][u][v] = val;
+                }
+    }
+    for (int t = 0; t <= ltot; ++t)
+        for (int u = 0; u <= ltot - t; ++u)
+            for (int v = 0; v <= ltot - t - u; ++v) r[RIDX(t, u, v)] = Rn[0][t][u][v];
+    free(Rn);
+}
+
+/* One-electron S, T, H = T + V packed, restating oneints.c:12-234. */
+void or_one_electron_ints(const OBasis *bs, double *S, double *Tm, double *H)
+{
+    const double piterm = 5.5683279968317079;
+    static double dx[EMAX * EMAX * (2 * EMAX + 1)], dy[EMAX * EMAX * (2 * EMAX + 1)],
+        dz[EMAX * EMAX * (2 * EMAX + 1)];
+    double *rsum = (double *)malloc(sizeof(double) * RMAX * RMAX * RMAX);
+    double *rterm = (double *)malloc(sizeof(double) * RMAX * RMAX * RMAX);
+    for (int ish = 0; ish < bs->nshell; ++ish) {
+        const OShell *s1 = &bs->sh[ish];
+        const double *a = bs->cen[s1->cen].r;
+        int lv1 = s1->lsh;
+        for (int jsh = 0; jsh <= ish; ++jsh) {
+            const OShell *s2 = &bs->sh[jsh];
+            const double *b = bs->cen[s2->cen].r;
+            int lv2 = s2->lsh, lvt = lv1 + lv2;
+            double absq = 0;
+            for (int x = 0; x < 3; ++x) absq += (a[x] - b[x]) * (a[x] - b[x]);
+            int nst1 = s1->nst, nst2 = s2->nst;
+            double sv[OR_MAXLST * OR_MAXLST], tv[OR_MAXLST * OR_MAXLST], vv[OR_MAXLST * OR_MAXLST];
+            memset(sv, 0, sizeof sv);
+            memset(tv, 0, sizeof tv);
+            memset(vv, 0, sizeof vv);
+            for (int ip = 0; ip < s1->npr; ip++) {
+                double axp = s1->al[ip], c1 = s1->co[ip];
+                for (int jp = 0; jp < s2->npr; jp++) {
+                    double c12 = c1 * s2->co[jp], bxp = s2->al[jp];
+                    double pxp = axp + bxp, abi = 1.0 / pxp;
+                    double s12 = piterm * exp(-axp * bxp * abi * absq) / (pxp * sqrt(pxp));
+                    double p[3];
+                    for (int x = 0; x < 3; ++x) p[x] = (axp * a[x] + bxp * b[x]) * abi;
+                    abi *= 0.5;
+                    or_hermite_e(dx, lv1 + 1, lv2 + 1, abi, p[0] - a[0], p[0] - b[0]);
+                    or_hermite_e(dy, lv1 + 1, lv2 + 1, abi, p[1] - a[1], p[1] - b[1]);
+                    or_hermite_e(dz, lv1 + 1, lv2 + 1, abi, p[2] - a[2], p[2] - b[2]);
+                    double sr = 2.0 * s12 * sqrt(pxp / M_PI);
+                    for (int t = 0; t < RMAX * RMAX * RMAX; ++t) rsum[t] = 0.0;
+                    for (int ic = 0; ic < bs->ncen; ic++) {
+                        if (ic == bs->skipcen) continue;
+                        double qc = bs->cen[ic].q, pc[3];
+                        for (int x = 0; x < 3; ++x) pc[x] = p[x] - bs->cen[ic].r[x];
+                        double pc2 = pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2];
+                        or_hermite_r(rterm, lvt, sr, pxp * pc2, pxp, pc);
+                        for (int t = 0; t <= lvt; ++t)
+                            for (int u = 0; u <= lvt - t; ++u)
+                                for (int v = 0; v <= lvt - t - u; ++v)
+                                    rsum[RIDX(t, u, v)] -= qc * rterm[RIDX(t, u, v)];
+                    }
+                    for (int ia = 0; ia < nst1; ++ia)
+                        for (int ib = 0; ib < nst2; ++ib) {
+                            int kc = ia * nst2 + ib;
+                            double nfct = c12 * bs->nf[lv1][ia] * bs->nf[lv2][ib];
+                            int l1 = bs->lxyz[lv1][ia][0], m1 = bs->lxyz[lv1][ia][1], n1 = bs->lxyz[lv1][ia][2];
+                            int l2 = bs->lxyz[lv2][ib][0], m2 = bs->lxyz[lv2][ib][1], n2 = bs->lxyz[lv2][ib][2];
+                            double ex = dx[EIDX(l1, l2, 0)], ey = dy[EIDX(m1, m2, 0)], ez = dz[EIDX(n1, n2, 0)];
+                            sv[kc] += nfct * s12 * ex * ey * ez;
+                            double tx = 2.0 * axp * bxp * dx[EIDX(l1 + 1, l2 + 1, 0)];
+                            if (l1) tx -= l1 * bxp * dx[EIDX(l1 - 1, l2 + 1, 0)];
+                            if (l2) {
+                                tx -= l2 * axp * dx[EIDX(l1 + 1, l2 - 1, 0)];
+                                if (l1) tx += 0.5 * l1 * l2 * dx[EIDX(l1 - 1, l2 - 1, 0)];
+                            }
+                            double ty = 2.0 * axp * bxp * dy[EIDX(m1 + 1, m2 + 1, 0)];
+                            if (m1) ty -= m1 * bxp * dy[EIDX(m1 - 1, m2 + 1, 0)];
+                            if (m2) {
+                                ty -= m2 * axp * dy[EIDX(m1 + 1, m2 - 1, 0)];
+                                if (m1) ty += 0.5 * m1 * m2 * dy[EIDX(m1 - 1, m2 - 1, 0)];
+                            }
+                            double tz = 2.0 * axp * bxp * dz[EIDX(n1 + 1, n2 + 1, 0)];
+                            if (n1) tz -= n1 * bxp * dz[EIDX(n1 - 1, n2 + 1, 0)];
+                            if (n2) {
+                                tz -= n2 * axp * dz[EIDX(n1 + 1, n2 - 1, 0)];
+                                if (n1) tz += 0.5 * n1 * n2 * dz[EIDX(n1 - 1, n2 - 1, 0)];
+                            }
+                            tv[kc] += nfct * s12 * (tx * ey * ez + ty * ex * ez + tz * ex * ey);
+                            double sum = 0.0;
+                            for (int t = 0; t <= l1 + l2; t++)
+                                for (int u = 0; u <= m1 + m2; u++)
+                                    for (int v = 0; v <= n1 + n2; v++)
+                                        sum += dx[EIDX(l1, l2, t)] * dy[EIDX(m1, m2, u)] * dz[EIDX(n1, n2, v)] *
+                                               rsum[RIDX(t, u, v)];
+                            vv[kc] += nfct * sum;
+                        }
+                }
+            }
+            for (int ia = 0; ia < nst1; ++ia) {
+                int ir = s1->off + ia;
+                for (int ib = 0; ib < nst2; ++ib) {
+                    int jr = s2->off + ib;
+                    if (jr > ir) break;
+                    int ij = ir * (ir + 1) / 2 + jr, kc = ia * nst2 + ib;
+                    S[ij] = sv[kc];
+                    Tm[ij] = tv[kc];
+                    H[ij] = tv[kc] + vv[kc];
+                }
+            }
+        }
+    }
+    free(rsum);
+    free(rterm);
+}
+
+/* DPM potential, restating dpm_ints/calcPDPM (dpmints.c:18-199):
+ * hmat[ij] -= sum_k c_k (ij | e^{-a_k |r-Q|^2}) with the fixed six-term fit
+ * of a unit charge (dpmints.c:31-36). */
+void or_dpm_ints(const OBasis *bs, const double *Q, double *hmat)
+{
+    static const double palf[] = {6.8505018000, 4.0491646000, 3.5941062989,
+                                  1.2478274000, 0.7927690989, 0.3377107978};
+    static const double pcof[] = {0.0766926784, 0.1483475912, 0.0462334641,
+                                  0.0717376427, 0.0447149792, 0.0069678529};
+    double *dx = (double *)malloc(sizeof(double) * EMAX * EMAX * (2 * EMAX + 1));
+    double *dy = (double *)malloc(sizeof(double) * EMAX * EMAX * (2 * EMAX + 1));
+    double *dz = (double *)malloc(sizeof(double) * EMAX * EMAX * (2 * EMAX + 1));
+    double *rs = (double *)malloc(sizeof(double) * RMAX * RMAX * RMAX);
+    for (int ish = 0; ish < bs->nshell; ++ish) {
+        const OShell *s1 = &bs->sh[ish];
+        const double *a = bs->cen[s1->cen].r;
+        int lv1 = s1->lsh;
+        for (int jsh = 0; jsh <= ish; ++jsh) {
+            const OShell *s2 = &bs->sh[jsh];
+            const double *b = bs->cen[s2->cen].r;
+            int lv2 = s2->lsh, lvt = lv1 + lv2;
+            double absq = 0;
+            for (int x = 0; x < 3; ++x) absq += (a[x] - b[x]) * (a[x] - b[x]);
+            double vints[OR_MAXLST * OR_MAXLST];
+            memset(vints, 0, sizeof vints);
+            for (int ipr = 0; ipr < s1->npr; ++ipr) {
+                double axp = s1->al[ipr], c1 = s1->co[ipr], f12 = 1.0;
+                int je = s2->npr;
+                if (s1 == s2) {
+                    je = ipr + 1;
+                    f12 = 2.0;
+                }
+                for (int jpr = 0; jpr < je; ++jpr) {
+                    if (jpr == ipr) f12 = 1.0;
+                    double c12 = c1 * f12 * s2->co[jpr], bxp = s2->al[jpr];
+                    double pxp = axp + bxp, abi = 1.0 / pxp, p[3], pa[3], pb[3];
+                    for (int x = 0; x < 3; ++x) {
+                        p[x] = (axp * a[x] + bxp * b[x]) * abi;
+                        pa[x] = p[x] - a[x];
+                        pb[x] = p[x] - b[x];
+                    }
+                    double s12 = exp(-abi * axp * bxp * absq);
+                    abi *= 0.5;
+                    or_hermite_e(dx, lv1, lv2, abi, pa[0], pb[0]);
+                    or_hermite_e(dy, lv1, lv2, abi, pa[1], pb[1]);
+                    or_hermite_e(dz, lv1, lv2, abi, pa[2], pb[2]);
+                    for (int lpr = 0; lpr < 6; ++lpr) {
+                        double c34 = pcof[lpr], qxp = palf[lpr], cdi = 1.0 / qxp;
+                        double pq[3] = {p[0] - Q[0], p[1] - Q[1], p[2] - Q[2]};
+                        double pq2 = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
+                        double txp = pxp + qxp;
+                        double sr = 2. * c12 * c34 * SRTERM * s12 * abi * cdi / sqrt(txp);
+                        double w = pxp * qxp / txp, t = w * pq2;
+                        or_hermite_r(rs, lvt, sr, t, w, pq);
+                        for (int ia = 0; ia < s1->nst; ++ia)
+                            for (int ib = 0; ib < s2->nst; ++ib) {
+                                int l1 = bs->lxyz[lv1][ia][0], m1 = bs->lxyz[lv1][ia][1], n1 = bs->lxyz[lv1][ia][2];
+                                int l2 = bs->lxyz[lv2][ib][0], m2 = bs->lxyz[lv2][ib][1], n2 = bs->lxyz[lv2][ib][2];
+                                double sum = 0.0;
+                                for (int ix = 0; ix <= l1 + l2; ++ix)
+                                    for (int iy = 0; iy <= m1 + m2; ++iy)
+                                        for (int iz = 0; iz <= n1 + n2; ++iz)
+                                            sum += dx[EIDX(l1, l2, ix)] * dy[EIDX(m1, m2, iy)] *
+                                                   dz[EIDX(n1, n2, iz)] * rs[RIDX(ix, iy, iz)];
+                                vints[ia * s2->nst + ib] += sum * bs->nf[lv1][ia] * bs->nf[lv2][ib];
+                            }
+                    }
+                }
+            }
+            for (int ia = 0; ia < s1->nst; ++ia) {
+                int ir = s1->off + ia;
+                for (int ib = 0; ib < s2->nst; ++ib) {
+                    int jr = s2->off + ib;
+                    if (jr > ir) break;
+                    hmat[ir * (ir + 1) / 2 + jr] -= vints[ia * s2->nst + ib];
+                }
+            }
+        }
+    }
+    free(dx);
+    free(dy);
+    free(dz);
+    free(rs);
+}
+
+/* ------------------------------------------------------------------ */
+/* Symmetric eigenproblem: Householder tridiagonalisation + implicit QL  */
+/* with eigenvectors (textbook; Golub & Van Loan secs. 8.3).  Eigenvalues */
+/* ascending; evecs[k*n + j] = component k of eigenvector j (column j).   */
+/* ------------------------------------------------------------------ */
+int or_eigh(int n, const double *Ain, double *d, double *z)
+{
+    double *e = (double *)calloc(n, sizeof(double));
+    for (int i = 0; i < n * n; ++i) z[i] = Ain[i];
+#define Z(i, j) z[(size_t)(i) * n + (j)]
+    for (int i = n - 1; i > 0; --i) {
+        int l = i - 1;
+        double h = 0.0, scale = 0.0;
+        if (l > 0) {
+            for (int k = 0; k <= l; ++k) scale += fabs(Z(i, k));
+            if (scale == 0.0) {
+                e[i] = Z(i, l);
+            } else {
+                for (int k = 0; k <= l; ++k) {
+                    Z(i, k) /= scale;
+                    h += Z(i, k) * Z(i, k);
+                }
+                double f = Z(i, l);
+                double g = (f >= 0.0 ? -sqrt(h) : sqrt(h));
+                e[i] = scale * g;
+                h -= f * g;
+                Z(i, l) = f - g;
+                f = 0.0;
+                for (int j = 0; j <= l; ++j) {
+                    Z(j, i) = Z(i, j) / h;
+                    g = 0.0;
+                    for (int k = 0; k <= j; ++k) g += Z(j, k) * Z(i, k);
+                    for (int k = j + 1; k <= l; ++k) g += Z(k, j) * Z(i, k);
+                    e[j] = g / h;
+                    f += e[j] * Z(i, j);
+                }
+                double hh = f / (h + h);
+                for (int j = 0; j <= l; ++j) {
+                    f = Z(i, j);
+                    e[j] = g = e[j] - hh * f;
+                    for (int k = 0; k <= j; ++k) Z(j, k) -= (f * e[k] + g * Z(i, k));
+                }
+            }
+        } else {
+            e[i] = Z(i, l);
+        }
+        d[i] = h;
+    }
+    d[0] = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        int l = i - 1;
+        if (d[i] != 0.0) {
+            for (int j = 0; j <= l; ++j) {
+                double g = 0.0;
+                for (int k = 0; k <= l; ++k) g += Z(i, k) * Z(k, j);
+                for (int k = 0; k <= l; ++k) Z(k, j) -= g * Z(k, i);
+            }
+        }
+        d[i] = Z(i, i);
+        Z(i, i) = 1.0;
+        for (int j = 0; j <= l; ++j) Z(j, i) = Z(i, j) = 0.0;
+    }
+    /* QL with implicit shifts */
+    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                volatile double dd = fabs(d[m]) + fabs(d[m + 1]);
+                volatile double ed = fabs(e[m]) + dd;
+                if (ed == dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 100) {
+                    free(e);
+                    return -1;
+                }
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i], b = c * e[i];
+                    e[i + 1] = (r = hypot(f, g));
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    d[i + 1] = g + (p = s * r);
+                    g = c * r - b;
+                    for (int k = 0; k < n; ++k) {
+                        f = Z(k, i + 1);
+                        Z(k, i + 1) = s * Z(k, i) + c * f;
+                        Z(k, i) = c * Z(k, i) - s * f;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    /* sort ascending */
+    for (int i = 0; i < n - 1; ++i) {
+        int k = i;
+        double p = d[i];
+        for (int j = i + 1; j < n; ++j)
+            if (d[j] < p) p = d[k = j];
+        if (k != i) {
+            d[k] = d[i];
+            d[i] = p;
+            for (int j = 0; j < n; ++j) {
+                double t = Z(j, i);
+                Z(j, i) = Z(j, k);
+                Z(j, k) = t;
+            }
+        }
+    }
+#undef Z
+    free(e);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* SCF drivers                                                          */
+/* ------------------------------------------------------------------ */
+static void unpack(int n, const double *p, double *f)
+{
+    for (int i = 0, ij = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j, ++ij) f[(size_t)i * n + j] = f[(size_t)j * n + i] = p[ij];
+}
+
+/* X = U s^{-1/2} (rhfscf.c:432-452) */
+static int form_x(int n, const double *S, double *X)
+{
+    double *Sf = (double *)malloc(sizeof(double) * n * n), *ev = (double *)malloc(sizeof(double) * n);
+    double *U = (double *)malloc(sizeof(double) * n * n);
+    unpack(n, S, Sf);
+    or_eigh(n, Sf, ev, U);
+    int bad = 0;
+    for (int j = 0; j < n; ++j) {
+        if (ev[j] <= 2.220446049250313e-16) bad = 1;
+        ev[j] = 1.0 / sqrt(ev[j]);
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) X[(size_t)i * n + j] = U[(size_t)i * n + j] * ev[j];
+    free(Sf);
+    free(ev);
+    free(U);
+    return bad ? -1 : 0;
+}
+
+/* F (packed) -> C = X V with V the eigenvectors of X^T F X
+ * (spack_trans + spack_diag + form_cmatrix, rhfscf.c:38-42). */
+static void diag_fock(int n, const double *Fp, const double *X, double *C, double *ev)
+{
+    double *F = (double *)malloc(sizeof(double) * n * n), *T = (double *)malloc(sizeof(double) * n * n);
+    double *Fp2 = (double *)malloc(sizeof(double) * n * n), *V = (double *)malloc(sizeof(double) * n * n);
+    unpack(n, Fp, F);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += F[(size_t)i * n + k] * X[(size_t)k * n + j];
+            T[(size_t)i * n + j] = s;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += X[(size_t)k * n + i] * T[(size_t)k * n + j];
+            Fp2[(size_t)i * n + j] = s;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            double a = 0.5 * (Fp2[(size_t)i * n + j] + Fp2[(size_t)j * n + i]);
+            Fp2[(size_t)i * n + j] = Fp2[(size_t)j * n + i] = a;
+        }
+    or_eigh(n, Fp2, ev, V);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += X[(size_t)i * n + k] * V[(size_t)k * n + j];
+            C[(size_t)i * n + j] = s;
+        }
+    free(F);
+    free(T);
+    free(Fp2);
+    free(V);
+}
+
+/* scfaux.c:24-41 */
+static void form_p(int n, int nocc, const double *C, double *P)
+{
+    for (int i = 0, ij = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j, ++ij) {
+            double s = 0.0;
+            for (int k = 0; k < nocc; ++k) s += C[(size_t)i * n + k] * C[(size_t)j * n + k];
+            P[ij] = s;
+        }
+}
+
+/* scfaux.c:99-118 */
+static double calc_energy(int n, const double *pm, const double *hm, const double *gm)
+{
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0, ij = 0; i < n; ++i) {
+        for (int j = 0; j < i; ++j, ++ij) {
+            s1 += hm[ij] * pm[ij];
+            s2 += gm[ij] * pm[ij];
+        }
+        s1 += pm[ij] * hm[ij] * 0.5;
+        s2 += pm[ij] * gm[ij] * 0.5;
+        ++ij;
+    }
+    return s1 + s1 + s2;
+}
+
+/* scfaux.c:120-136 */
+static double pdiff_norm(int n, const double *p0, const double *p1)
+{
+    double sum = 0.0;
+    for (int i = 0, ij = 0; i < n; ++i, ++ij) {
+        for (int j = 0; j < i; ++j, ++ij) {
+            double t = p0[ij] - p1[ij];
+            sum += t * t;
+        }
+        double t = p0[ij] - p1[ij];
+        sum += t * t * 0.5;
+    }
+    return sqrt(sum * 2.) / n;
+}
+
+/* scfaux.c:63-95 */
+static void scf_accelerator(int iter, int no2, double *p0, double *p1, double *p2)
+{
+    if (iter > 2) {
+        double p00 = 0, p01 = 0, p11 = 0;
+        for (int i = 0; i < no2; ++i) {
+            double s0 = p0[i] - p1[i], s1 = p1[i] - p2[i];
+            p00 += s0 * s0;
+            p01 += s0 * s1;
+            p11 += s1 * s1;
+        }
+        double beta = (p00 - p01) / (p00 - 2. * p01 + p11);
+        if (beta < 1.e-3) beta = 1.e-3;
+        if (beta > 1.5) beta = 1.5;
+        double bm1 = 1. - beta;
+        for (int i = 0; i < no2; ++i) {
+            p2[i] = p1[i];
+            p1[i] = p0[i];
+            p0[i] = p1[i] * beta + p2[i] * bm1;
+        }
+        return;
+    }
+    for (int i = 0; i < no2; ++i) {
+        p2[i] = p1[i];
+        p1[i] = p0[i];
+        p0[i] = 0.5 * (p1[i] + p2[i]);
+    }
+}
+
+int or_rhf_scf(const OBasis *b, int nelec, int maxits, double eps, int scf_a, const OERINT *ints,
+               int64_t nints, double *P_out, double *evals_out, OScfResult *res, int nthreads)
+{
+    const double EPS10 = 10 * 2.220446049250313e-16;
+    int n = b->norb, nocc = nelec / 2;
+    size_t no2 = (size_t)n * (n + 1) / 2;
+    double *S = calloc(no2, 8), *Tm = calloc(no2, 8), *H = calloc(no2, 8), *G = calloc(no2, 8);
+    double *F = calloc(no2, 8), *P = calloc(no2, 8), *P1 = calloc(no2, 8), *P2 = calloc(no2, 8);
+    double *X = calloc((size_t)n * n, 8), *C = calloc((size_t)n * n, 8), *ev = calloc(n, 8);
+    or_one_electron_ints(b, S, Tm, H);
+    if (eps < EPS10 * n * n) eps = EPS10 * n * n; /* rhfscf.c:161-164 */
+    if (form_x(n, S, X)) return -1;
+    diag_fock(n, H, X, C, ev); /* core guess, rhfscf.c:454-471 */
+    form_p(n, nocc, C, P);
+    double enuc = or_nuclear_repulsion(b), eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0;
+    int iteration = 0, done = 0, first = 1;
+    for (;;) {
+        ++iteration; /* rhf_scf_update, rhfscf.c:21-71 */
+        memset(G, 0, no2 * 8);
+        or_rhf_form_gmatrix_mt(ints, nints, n, P, G, nthreads);
+        for (size_t i = 0; i < no2; ++i) F[i] = H[i] + G[i];
+        diag_fock(n, F, X, C, ev);
+        memcpy(P2, P1, no2 * 8);
+        memcpy(P1, P, no2 * 8);
+        form_p(n, nocc, C, P);
+        pdiff = pdiff_norm(n, P, P1);
+        enrg = 2. * calc_energy(n, P1, H, G);
+        ediff = enrg - eold;
+        eold = enrg;
+        if (or_verbose)
+            fprintf(stderr, "it %3d E %.15f ediff % .3e pdiff %.3e\n", iteration, enrg, ediff, pdiff);
+        int ret = 0;
+        if (iteration == 1) {
+            ret = 0;
+        } else if (fabs(ediff) < EPS10 || pdiff < eps) {
+            ret = 1;
+        } else if (ediff > 0.) {
+            if (scf_a == 1)
+                for (size_t i = 0; i < no2; ++i) P[i] = 0.5 * (P1[i] + P2[i]);
+            if (scf_a == 2) scf_accelerator(iteration, (int)no2, P, P1, P2);
+        }
+        if (first) { /* rhf_energy, rhfscf.c:80-95 */
+            first = 0;
+            iteration = 0;
+            continue;
+        }
+        done = ret;
+        if (done || iteration >= maxits) break;
+    }
+    res->energy = enrg + enuc;
+    res->eelec = enrg;
+    res->enuc = enuc;
+    res->ediff = ediff;
+    res->pdiff = pdiff;
+    res->iterations = iteration;
+    res->converged = done;
+    if (P_out) memcpy(P_out, P, no2 * 8);
+    if (evals_out) memcpy(evals_out, ev, n * 8);
+    free(S); free(Tm); free(H); free(G); free(F); free(P); free(P1); free(P2);
+    free(X); free(C); free(ev);
+    return 0;
+}
+
+int or_uhf_scf(const OBasis *b, int nelec, int mult, int maxits, double eps, int scf_a,
+               const OERINT *ints, int64_t nints, double *PA_out, double *PB_out, OScfResult *res,
+               int nthreads)
+{
+    (void)nthreads;
+    const double EPS10 = 10 * 2.220446049250313e-16;
+    int n = b->norb, noccB = (nelec - (mult - 1)) / 2, noccA = nelec - noccB;
+    size_t no2 = (size_t)n * (n + 1) / 2;
+    double *S = calloc(no2, 8), *Tm = calloc(no2, 8), *H = calloc(no2, 8), *F = calloc(no2, 8);
+    double *GA = calloc(no2, 8), *GB = calloc(no2, 8);
+    double *PA = calloc(no2, 8), *PA1 = calloc(no2, 8), *PA2 = calloc(no2, 8);
+    double *PB = calloc(no2, 8), *PB1 = calloc(no2, 8), *PB2 = calloc(no2, 8);
+    double *X = calloc((size_t)n * n, 8), *C = calloc((size_t)n * n, 8), *ev = calloc(n, 8);
+    or_one_electron_ints(b, S, Tm, H);
+    if (eps < EPS10 * n * n) eps = EPS10 * n * n;
+    if (form_x(n, S, X)) return -1;
+    diag_fock(n, H, X, C, ev); /* uhfscf.c:507-527 */
+    form_p(n, noccA, C, PA);
+    form_p(n, noccB, C, PB);
+    double enuc = or_nuclear_repulsion(b), eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0;
+    int iteration = 0, done = 0, first = 1;
+    for (;;) {
+        ++iteration; /* uhf_scf_update, uhfscf.c:22-83 */
+        memset(GA, 0, no2 * 8);
+        memset(GB, 0, no2 * 8);
+        or_uhf_form_gmatrix(ints, nints, PA, PB, GA, GB);
+        for (size_t i = 0; i < no2; ++i) F[i] = H[i] + GA[i];
+        diag_fock(n, F, X, C, ev);
+        memcpy(PA2, PA1, no2 * 8);
+        memcpy(PA1, PA, no2 * 8);
+        form_p(n, noccA, C, PA);
+        pdiff = pdiff_norm(n, PA, PA1);
+        for (size_t i = 0; i < no2; ++i) F[i] = H[i] + GB[i];
+        diag_fock(n, F, X, C, ev);
+        memcpy(PB2, PB1, no2 * 8);
+        memcpy(PB1, PB, no2 * 8);
+        form_p(n, noccB, C, PB);
+        pdiff += pdiff_norm(n, PB, PB1);
+        enrg = calc_energy(n, PA1, H, GA) + calc_energy(n, PB1, H, GB);
+        ediff = enrg - eold;
+        eold = enrg;
+        int ret = 0;
+        if (iteration == 1) {
+            ret = 0;
+        } else if (fabs(ediff) < EPS10 || pdiff < eps) {
+            ret = 1;
+        } else if (ediff > 0.) {
+            if (scf_a == 1)
+                for (size_t i = 0; i < no2; ++i) {
+                    PA[i] = 0.5 * (PA1[i] + PA2[i]);
+                    PB[i] = 0.5 * (PB1[i] + PB2[i]);
+                }
+            if (scf_a == 2) {
+                scf_accelerator(iteration, (int)no2, PA, PA1, PA2);
+                scf_accelerator(iteration, (int)no2, PB, PB1, PB2);
+            }
+        }
+        if (first) {
+            first = 0;
+            iteration = 0;
+            continue;
+        }
+        done = ret;
+        if (done || iteration >= maxits) break;
+    }
+    res->energy = enrg + enuc;
+    res->eelec = enrg;
+    res->enuc = enuc;
+    res->ediff = ediff;
+    res->pdiff = pdiff;
+    res->iterations = iteration;
+    res->converged = done;
+    if (PA_out) memcpy(PA_out, PA, no2 * 8);
+    if (PB_out) memcpy(PB_out, PB, no2 * 8);
+    free(S); free(Tm); free(H); free(F); free(GA); free(GB);
+    free(PA); free(PA1); free(PA2); free(PB); free(PB1); free(PB2);
+    free(X); free(C); free(ev);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Screened quartet task list (CPU mirror).  Bra and ket pair lists are  */
+/* sorted by Schwarz bound, descending.  For bra b the surviving kets    */
+/* form the prefix [0, n(b)) with qbra[b]*qket[k] >= tau; same-list      */
+/* (same class) keeps k <= b.  Kets with qbra*qket < tau32 are FP32      */
+/* (tprec 1), the rest FP64 (tprec 0); tasks never straddle that split  */
+/* and hold at most `chunk` kets.  Returns the task count (writes at    */
+/* most cap).                                                           */
+/* ------------------------------------------------------------------ */
+static int prefix_count(const double *q, int n, double qb, double thr)
+{
+    int lo = 0, hi = n; /* first index with qb*q[k] < thr */
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (qb * q[mid] >= thr)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+int64_t or_quartet_tasks(int nbra, const double *qbra, int nket, const double *qket, int same,
+                         double tau, double tau32, int chunk, int *tb, int *tk0, int *tk1, int *tprec,
+                         int64_t cap)
+{
+    int64_t nt = 0;
+    for (int b = 0; b < nbra; ++b) {
+        int n = prefix_count(qket, nket, qbra[b], tau);
+        if (same && n > b + 1) n = b + 1;
+        int n64 = tau32 > 0.0 ? prefix_count(qket, nket, qbra[b], tau32) : n;
+        if (n64 > n) n64 = n;
+        for (int seg = 0; seg < 2; ++seg) {
+            int lo = seg ? n64 : 0, hi = seg ? n : n64;
+            for (int k0 = lo; k0 < hi; k0 += chunk) {
+                if (nt < cap) {
+                    tb[nt] = b;
+                    tk0[nt] = k0;
+                    tk1[nt] = (k0 + chunk < hi) ? k0 + chunk : hi;
+                    tprec[nt] = seg;
+                }
+                ++nt;
+            }
+        }
+    }
+    return nt;
+}

@@ -269,3 +269,73 @@ def packed_to_full(p: np.ndarray, n: int) -> np.ndarray:
 
 def full_to_packed(F: np.ndarray) -> np.ndarray:
     return F[np.tril_indices(F.shape[0])].copy()
+
+
+# ---------------------------------------------------------------------------
+# CPU mirror of the screened work list (tile quartets) the GPU library
+# builds in two_electron_ints.  Shells sorted by (l, centre, index), tiles of
+# 8 (s, p) or 4 (d) shells, tile-pair bound = max Schwarz Q in the tile pair,
+# items = canonical tile-pair pairs with bound product >= tau, bra = higher
+# pair class, sorted by (bra class, ket class, bound) descending.
+# ---------------------------------------------------------------------------
+def work_items_mirror(system, q_orig, tau, start=0):
+    ls = [s.l for s in system.shells]
+    cens = [s.cen for s in system.shells]
+    ns = len(ls)
+    order = sorted(range(ns), key=lambda i: (ls[i], cens[i], i))
+    lsort = [ls[i] for i in order]
+
+    def tile_of(l):
+        return 8 if l <= 1 else 4
+
+    tf, tsz, tl = [], [], []
+    i = 0
+    while i < ns:
+        l = lsort[i]
+        n = 0
+        while i + n < ns and lsort[i + n] == l and n < tile_of(l):
+            n += 1
+        tf.append(i)
+        tsz.append(n)
+        tl.append(l)
+        i += n
+    nt = len(tf)
+
+    def qs(I, J):  # sorted indices
+        a, b = order[I], order[J]
+        if a < b:
+            a, b = b, a
+        return q_orig[a * (a + 1) // 2 + b]
+
+    tmaxorig = [max(order[tf[t] + k] for k in range(tsz[t])) for t in range(nt)]
+    tps = []
+    for ta in range(nt):
+        for tb in range(ta + 1):
+            q = 0.0
+            for ii in range(tsz[ta]):
+                I = tf[ta] + ii
+                for jj in range(tsz[tb]):
+                    J = tf[tb] + jj
+                    if J > I:
+                        continue
+                    q = max(q, qs(I, J))
+            la, lb = tl[ta], tl[tb]
+            tps.append((ta, tb, la * (la + 1) // 2 + lb, q, max(tmaxorig[ta], tmaxorig[tb])))
+    items = []
+    for p1 in range(len(tps)):
+        A = tps[p1]
+        for p2 in range(p1 + 1):
+            B = tps[p2]
+            bound = A[3] * B[3]
+            if bound < tau:
+                continue
+            if start > 0 and max(A[4], B[4]) < start:
+                continue
+            bra, ket = (A, B) if A[2] >= B[2] else (B, A)
+            items.append(((bra[0], bra[1], ket[0], ket[1]), bra[2], ket[2], bound))
+    items.sort(key=lambda x: (-x[1], -x[2], -x[3]))
+    tiles = np.array([it[0] for it in items], dtype=np.int32).reshape(-1, 4)
+    classes = np.array([(it[1], it[2]) for it in items], dtype=np.int32).reshape(-1, 2)
+    bounds = np.array([it[3] for it in items])
+    return dict(tiles=tiles, classes=classes, bounds=bounds, sorted_orig=np.array(order, dtype=np.int32),
+                tile_first=np.array(tf, dtype=np.int32), tile_size=np.array(tsz, dtype=np.int32))

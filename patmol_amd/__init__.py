@@ -1,0 +1,285 @@
+"""patmol_amd -- MI355X (gfx950) two-electron hot path of Patmol.
+
+Python front end over the two in-tree shared libraries:
+
+* ``libpatmol_hip.so``  -- HIP kernels and the reference-compatible C ABI
+  (init_ERIFS, two_electron_ints, rhf_form_gmatrix, uhf_form_gmatrix,
+  dpm_ints, clean_ERIFS) plus extensions (device-resident G build, stats,
+  ERI export, work-list export, options);
+* ``libpatmol_host.so`` -- the caller side: basis model behind the
+  reference's basis.h accessors, one-electron integrals and the RHF/UHF
+  SCF drivers.
+
+There is no CPU fallback: :func:`lib` raises if the libraries are missing,
+and every call checks the library's status code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import systems  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_HIP = os.path.join(HERE, "libpatmol_hip.so")
+LIB_HOST = os.path.join(HERE, "libpatmol_host.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class PatmolError(RuntimeError):
+    pass
+
+
+class HipStats(C.Structure):
+    _fields_ = [("nshell", C.c_int), ("norb", C.c_int), ("start", C.c_int),
+                ("pairs", C.c_int64), ("shell_quartets_total", C.c_int64),
+                ("shell_quartets_kept", C.c_int64), ("prim_quartets_kept", C.c_int64),
+                ("flops_per_build", C.c_double), ("work_items", C.c_int64),
+                ("work_items_fp32", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ScfResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("ediff", C.c_double), ("pdiff", C.c_double),
+                ("enuc", C.c_double), ("eelec", C.c_double), ("iterations", C.c_int),
+                ("converged", C.c_int), ("gbuild_seconds", C.c_double),
+                ("diag_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build(jobs: int = 8) -> None:
+    """Compile both libraries in-tree (hipcc cross-compiles gfx950 without a GPU)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", os.path.join(HERE, "csrc")], check=True)
+
+
+_host = None
+_hip = None
+
+
+def lib():
+    """(host, hip) ctypes handles; raises if the libraries are not built."""
+    global _host, _hip
+    if _host is None:
+        for p in (LIB_HIP, LIB_HOST):
+            if not os.path.exists(p):
+                raise PatmolError(f"{p} is missing: run patmol_amd.build() (no CPU fallback exists)")
+        H = C.CDLL(LIB_HOST, mode=C.RTLD_GLOBAL)
+        G = C.CDLL(LIB_HIP, mode=C.RTLD_GLOBAL)
+        H.patmol_basis_load.restype = C.c_int
+        H.patmol_basis_load.argtypes = [C.c_int, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp,
+                                        C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
+        H.patmol_basis_add_dpm_center.restype = C.c_int
+        H.number_of_orbitals.restype = C.c_int
+        H.number_of_shells.restype = C.c_int
+        H.nuclear_repulsion.restype = C.c_double
+        H.set_center_position.argtypes = [C.c_int, _dp]
+        H.one_electron_ints.argtypes = [_dp, _dp, _dp]
+        H.patmol_rhf_scf.argtypes = [_dp, _dp, C.POINTER(ScfResult)]
+        H.patmol_uhf_scf.argtypes = [_dp, _dp, C.POINTER(ScfResult)]
+        G.init_ERIFS.restype = C.c_void_p
+        G.init_ERIFS.argtypes = [C.c_int, C.c_int, C.c_char_p]
+        G.clean_ERIFS.argtypes = [C.c_void_p]
+        G.two_electron_ints.argtypes = [C.c_void_p, C.c_int]
+        G.rhf_form_gmatrix.argtypes = [C.c_void_p, _dp, _dp]
+        G.uhf_form_gmatrix.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        G.dpm_ints.argtypes = [_dp]
+        G.patmol_hip_last_error.restype = C.c_int
+        G.patmol_hip_error_string.restype = C.c_char_p
+        G.patmol_hip_set_strict.argtypes = [C.c_int]
+        G.patmol_hip_set_schwarz_threshold.argtypes = [C.c_double]
+        G.patmol_hip_set_precision.argtypes = [C.c_int, C.c_double]
+        G.patmol_hip_rhf_gmatrix_device.restype = C.c_int
+        G.patmol_hip_rhf_gmatrix_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        G.patmol_hip_stats.restype = C.c_int
+        G.patmol_hip_stats.argtypes = [C.c_void_p, C.POINTER(HipStats)]
+        G.patmol_hip_pair_schwarz.restype = C.c_int
+        G.patmol_hip_pair_schwarz.argtypes = [C.c_void_p, _dp]
+        G.patmol_hip_work_items.restype = C.c_int64
+        G.patmol_hip_work_items.argtypes = [C.c_void_p] + [C.c_void_p] * 7
+        G.patmol_hip_eri_blocks.restype = C.c_int
+        G.patmol_hip_eri_blocks.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        G.patmol_hip_last_timing.restype = C.c_int
+        G.patmol_hip_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), _dp]
+        G.patmol_hip_set_timing.argtypes = [C.c_int]
+        _host, _hip = H, G
+    return _host, _hip
+
+
+def _check(what: str = ""):
+    _, G = lib()
+    code = G.patmol_hip_last_error()
+    if code:
+        raise PatmolError(f"{what}: error {code}: {G.patmol_hip_error_string().decode()}")
+
+
+def exported_symbols():
+    """Names the C ABI declares (include/patmol_hip.h)."""
+    return ["init_ERIFS", "clean_ERIFS", "two_electron_ints", "rhf_form_gmatrix", "uhf_form_gmatrix",
+            "dpm_ints", "patmol_hip_last_error", "patmol_hip_error_string", "patmol_hip_set_strict",
+            "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision",
+            "patmol_hip_rhf_gmatrix_device", "patmol_hip_stats", "patmol_hip_pair_schwarz",
+            "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
+            "patmol_hip_set_timing"]
+
+
+def load_system(sys_: "systems.System") -> None:
+    """Make `sys_` the current basis (the reference's global basis model)."""
+    H, _ = lib()
+    q, xyz, npr, lsh, cen, al, co = sys_.flat()
+    rc = H.patmol_basis_load(len(q), q, xyz, len(npr), npr, lsh, cen, al, co, sys_.nelec, sys_.mult,
+                             sys_.maxits, sys_.scf_eps, sys_.scf_a)
+    if rc:
+        raise PatmolError(f"patmol_basis_load failed ({rc})")
+
+
+def set_options(tau: Optional[float] = None, precision: Optional[int] = None,
+                mixed_threshold: float = 1e-6, strict: Optional[bool] = None, timing: Optional[bool] = None):
+    _, G = lib()
+    if tau is not None:
+        G.patmol_hip_set_schwarz_threshold(tau)
+    if precision is not None:
+        G.patmol_hip_set_precision(precision, mixed_threshold)
+    if strict is not None:
+        G.patmol_hip_set_strict(int(strict))
+    if timing is not None:
+        G.patmol_hip_set_timing(int(timing))
+
+
+def one_electron_ints():
+    H, _ = lib()
+    n = H.number_of_orbitals()
+    no2 = n * (n + 1) // 2
+    S, T, Hm = np.zeros(no2), np.zeros(no2), np.zeros(no2)
+    H.one_electron_ints(S, T, Hm)
+    return S, T, Hm
+
+
+class EriStore:
+    """An ERIFS handle of the current basis (the reference's integral store,
+    here device-resident pair tables and screened work lists)."""
+
+    def __init__(self, start: int = 0, name: str = "MINTS.DAT"):
+        H, G = lib()
+        self.norb = H.number_of_orbitals()
+        self.nshell = H.number_of_shells()
+        self.h = G.init_ERIFS(0, self.norb, name.encode())
+        _check("init_ERIFS")
+        G.two_electron_ints(self.h, start)
+        _check("two_electron_ints")
+
+    def close(self):
+        if getattr(self, "h", None) and _hip is not None:
+            _hip.clean_ERIFS(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def rhf_gmatrix(self, P: np.ndarray, G: Optional[np.ndarray] = None) -> np.ndarray:
+        _, L = lib()
+        G = np.zeros_like(P) if G is None else G
+        L.rhf_form_gmatrix(self.h, np.ascontiguousarray(P, dtype=np.float64), G)
+        _check("rhf_form_gmatrix")
+        return G
+
+    def uhf_gmatrix(self, PA, PB):
+        _, L = lib()
+        GA, GB = np.zeros_like(PA), np.zeros_like(PB)
+        L.uhf_form_gmatrix(self.h, np.ascontiguousarray(PA), np.ascontiguousarray(PB), GA, GB)
+        _check("uhf_form_gmatrix")
+        return GA, GB
+
+    def rhf_gmatrix_device(self, d_P: int, d_G: int, stream: int = 0) -> None:
+        _, L = lib()
+        rc = L.patmol_hip_rhf_gmatrix_device(self.h, C.c_void_p(d_P), C.c_void_p(d_G), C.c_void_p(stream))
+        if rc:
+            _check("patmol_hip_rhf_gmatrix_device")
+
+    def stats(self) -> dict:
+        _, L = lib()
+        s = HipStats()
+        if L.patmol_hip_stats(self.h, C.byref(s)):
+            _check("patmol_hip_stats")
+        return s.as_dict()
+
+    def pair_schwarz(self) -> np.ndarray:
+        _, L = lib()
+        q = np.zeros(self.nshell * (self.nshell + 1) // 2)
+        if L.patmol_hip_pair_schwarz(self.h, q):
+            _check("patmol_hip_pair_schwarz")
+        return q
+
+    def work_items(self):
+        _, L = lib()
+        n = L.patmol_hip_work_items(self.h, None, None, None, None, None, None, None)
+        if n < 0:
+            _check("patmol_hip_work_items")
+        t4 = np.zeros(4 * max(n, 1), np.int32)
+        c2 = np.zeros(2 * max(n, 1), np.int32)
+        b = np.zeros(max(n, 1))
+        so = np.zeros(self.nshell, np.int32)
+        tf = np.zeros(self.nshell, np.int32)
+        ts = np.zeros(self.nshell, np.int32)
+        nt = C.c_int()
+        L.patmol_hip_work_items(self.h, t4.ctypes.data, c2.ctypes.data, b.ctypes.data, so.ctypes.data,
+                                tf.ctypes.data, ts.ctypes.data, C.addressof(nt))
+        k = nt.value
+        return dict(tiles=t4[:4 * n].reshape(n, 4), classes=c2[:2 * n].reshape(n, 2), bounds=b[:n],
+                    sorted_orig=so, tile_first=tf[:k], tile_size=ts[:k])
+
+    def eri_blocks(self, quartets: Sequence[Sequence[int]], nst: Sequence[int]):
+        _, L = lib()
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1))
+        nq = len(q) // 4
+        sizes = [int(nst[a] * nst[b] * nst[c] * nst[d]) for a, b, c, d in np.asarray(quartets).reshape(-1, 4)]
+        out = np.zeros(max(sum(sizes), 1))
+        if L.patmol_hip_eri_blocks(self.h, nq, q, out):
+            _check("patmol_hip_eri_blocks")
+        res, o = [], 0
+        for (a, b, c, d), s in zip(np.asarray(quartets).reshape(-1, 4), sizes):
+            res.append(out[o:o + s].reshape(nst[a], nst[b], nst[c], nst[d]))
+            o += s
+        return res
+
+    def last_timing(self):
+        _, L = lib()
+        tot = C.c_double()
+        cls = np.zeros(36)
+        L.patmol_hip_last_timing(self.h, C.byref(tot), cls)
+        return tot.value, cls
+
+
+def rhf_scf():
+    """Run the SCF driver on the current basis; returns (result, P, orbital energies)."""
+    H, _ = lib()
+    n = H.number_of_orbitals()
+    P = np.zeros(n * (n + 1) // 2)
+    ev = np.zeros(n)
+    res = ScfResult()
+    rc = H.patmol_rhf_scf(P, ev, C.byref(res))
+    if rc:
+        _check("patmol_rhf_scf")
+        raise PatmolError(f"patmol_rhf_scf failed ({rc})")
+    return res.as_dict(), P, ev
+
+
+def uhf_scf():
+    H, _ = lib()
+    n = H.number_of_orbitals()
+    PA, PB = np.zeros(n * (n + 1) // 2), np.zeros(n * (n + 1) // 2)
+    res = ScfResult()
+    rc = H.patmol_uhf_scf(PA, PB, C.byref(res))
+    if rc:
+        _check("patmol_uhf_scf")
+        raise PatmolError(f"patmol_uhf_scf failed ({rc})")
+    return res.as_dict(), PA, PB

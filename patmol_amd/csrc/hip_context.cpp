@@ -1,0 +1,944 @@
+// hip_context.cpp -- host side of libpatmol_hip.so: the reference's ERIFS
+// boundary (init_ERIFS, two_electron_ints, rhf/uhf_form_gmatrix,
+// clean_ERIFS) over device-resident pair tables and screened tile-quartet
+// work lists, plus the extension entry points declared in patmol_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+
+#define RYS_STORAGE static const
+#include "rys_table.inc"
+static constexpr int kTableNC = RYS_NC;
+#undef RYS_NC
+#undef RYS_NMAX
+
+#include "eri_kernels.h"
+#include "patmol_hip.h"
+
+extern "C" {
+// basis.h accessors (reference basis.c, or libpatmol_host.so)
+const Shell *shells(void);
+const Center *centers(void);
+int number_of_shells(void);
+int number_of_orbitals(void);
+int number_of_centers(void);
+int skip_center(void);
+}
+
+namespace pm {
+
+// ---------------------------------------------------------------------------
+// status
+// ---------------------------------------------------------------------------
+static thread_local int g_err = PATMOL_HIP_OK;
+static thread_local char g_msg[512] = "";
+static int g_strict = -1;
+
+static bool strict_mode()
+{
+    if (g_strict < 0) {
+        const char *e = getenv("PATMOL_HIP_STRICT");
+        g_strict = (e && atoi(e)) ? 1 : 0;
+    }
+    return g_strict == 1;
+}
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+    va_end(ap);
+    g_err = code;
+    if (strict_mode()) {
+        fprintf(stderr, "FATAL ERROR: %s\n", g_msg);
+        abort();
+    }
+    return code;
+}
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return fail(PATMOL_HIP_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// options
+// ---------------------------------------------------------------------------
+static double g_tau = 1e-12;
+static int g_prec_mode = 0;
+static double g_prec_thr = 1e-6;
+static bool g_timing = false;
+
+// ---------------------------------------------------------------------------
+// device Rys tables (one copy per process)
+// ---------------------------------------------------------------------------
+static RysTables g_rys = {nullptr, nullptr, nullptr, nullptr};
+static std::mutex g_mu;
+
+static int ensure_tables()
+{
+    if (g_rys.cheb) return 0;
+    double *c = nullptr, *hr = nullptr, *hh = nullptr;
+    float *c32 = nullptr;
+    std::vector<float> f(RYS_TABLE_SIZE);
+    for (int i = 0; i < RYS_TABLE_SIZE; ++i) f[i] = (float)rys_cheb[i];
+    HIPCHK(hipMalloc(&c, sizeof(double) * RYS_TABLE_SIZE));
+    HIPCHK(hipMalloc(&c32, sizeof(float) * RYS_TABLE_SIZE));
+    HIPCHK(hipMalloc(&hr, sizeof(rys_herm_r)));
+    HIPCHK(hipMalloc(&hh, sizeof(rys_herm_h)));
+    HIPCHK(hipMemcpy(c, rys_cheb, sizeof(double) * RYS_TABLE_SIZE, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c32, f.data(), sizeof(float) * RYS_TABLE_SIZE, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(hr, rys_herm_r, sizeof(rys_herm_r), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(hh, rys_herm_h, sizeof(rys_herm_h), hipMemcpyHostToDevice));
+    g_rys = {c, c32, hr, hh};
+    static_assert(kTableNC == pm::RYS_NC, "table layout");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// per-handle context
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t cnt)
+    {
+        if (cnt <= n && p) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (cnt == 0) cnt = 1;
+        if (hipMalloc(&p, sizeof(T) * cnt) != hipSuccess) return fail(PATMOL_HIP_EDEVICE, "hipMalloc of %zu bytes failed", sizeof(T) * cnt);
+        n = cnt;
+        return 0;
+    }
+    int upload(const std::vector<T> &v)
+    {
+        if (alloc(v.size())) return g_err;
+        if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+        return 0;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct ClassRange {
+    int bra_pc, ket_pc;
+    int first, count;
+};
+
+struct Ctx {
+    bool built = false;
+    int N = 0, nshell = 0, start = 0;
+    double tau = 1e-12;
+    // host tables
+    std::vector<int> sorted_orig, orig_to_sorted, shell_l, shell_nst, shell_off;
+    std::vector<double> shell_xyz;
+    std::vector<int> tile_first, tile_size, tile_l;
+    std::vector<PairRef> pairs;
+    std::vector<PrimPair> pp;
+    std::vector<double> pair_q;
+    std::vector<int4> items;
+    std::vector<double> item_bound;
+    std::vector<ClassRange> ranges;
+    // device tables
+    DevBuf<double> d_xyz, d_q, d_bound;
+    DevBuf<int> d_off, d_orig, d_tile_first, d_tile_size;
+    DevBuf<PairRef> d_pairs;
+    DevBuf<PrimPair> d_pp;
+    DevBuf<int4> d_items;
+    // work buffers
+    DevBuf<double> d_P, d_D, d_D2, d_J, d_K0, d_K1, d_G, d_G2, d_dmax;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> cls_ev;
+    double last_ms = 0.0;
+    double class_ms[36] = {0};
+    int64_t items_fp32 = 0;
+};
+
+static std::unordered_map<const ERIFS *, Ctx *> g_ctx;
+
+static Ctx *ctx_of(const ERIFS *e)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_ctx.find(e);
+    return it == g_ctx.end() ? nullptr : it->second;
+}
+
+static inline int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+
+// ---------------------------------------------------------------------------
+// build: shells, pairs, Schwarz, tiles, work items
+// ---------------------------------------------------------------------------
+static int build_tables(Ctx &c, int start)
+{
+    const Shell *sh = shells();
+    const Center *ce = centers();
+    const int ns = number_of_shells();
+    const int N = number_of_orbitals();
+    if (!sh || !ce || ns <= 0) return fail(PATMOL_HIP_ESTATE, "no basis loaded");
+    if (N > 65535) return fail(PATMOL_HIP_EUNSUPPORTED, "more than 65535 functions");
+    for (int s = 0; s < ns; ++s)
+        if (sh[s].lsh > PM_LMAX || sh[s].lsh < 0)
+            return fail(PATMOL_HIP_EUNSUPPORTED, "shell %d has l=%d (supported: 0..%d)", s, sh[s].lsh, PM_LMAX);
+    c.N = N;
+    c.nshell = ns;
+    c.start = start;
+    c.tau = g_tau;
+    // sort shells by (l, centre, original index)
+    c.sorted_orig.resize(ns);
+    std::iota(c.sorted_orig.begin(), c.sorted_orig.end(), 0);
+    std::stable_sort(c.sorted_orig.begin(), c.sorted_orig.end(), [&](int a, int b) {
+        if (sh[a].lsh != sh[b].lsh) return sh[a].lsh < sh[b].lsh;
+        if (sh[a].cen != sh[b].cen) return sh[a].cen < sh[b].cen;
+        return a < b;
+    });
+    c.orig_to_sorted.assign(ns, 0);
+    c.shell_l.resize(ns);
+    c.shell_nst.resize(ns);
+    c.shell_off.resize(ns);
+    c.shell_xyz.resize(3 * ns);
+    for (int i = 0; i < ns; ++i) {
+        const Shell &s = sh[c.sorted_orig[i]];
+        c.orig_to_sorted[c.sorted_orig[i]] = i;
+        c.shell_l[i] = s.lsh;
+        c.shell_nst[i] = (s.lsh + 1) * (s.lsh + 2) / 2;
+        c.shell_off[i] = s.off;
+        for (int x = 0; x < 3; ++x) c.shell_xyz[3 * i + x] = ce[s.cen].r[x];
+    }
+    // pairs I >= J (sorted) and their primitive pairs
+    const int64_t np = (int64_t)ns * (ns + 1) / 2;
+    c.pairs.assign(np, PairRef{0, 0});
+    c.pp.clear();
+    for (int I = 0; I < ns; ++I) {
+        const Shell &a = sh[c.sorted_orig[I]];
+        const double *A = ce[a.cen].r;
+        for (int J = 0; J <= I; ++J) {
+            const Shell &b = sh[c.sorted_orig[J]];
+            const double *B = ce[b.cen].r;
+            const bool samec = a.cen == b.cen;
+            double ab2 = 0;
+            for (int x = 0; x < 3; ++x) ab2 += (A[x] - B[x]) * (A[x] - B[x]);
+            PairRef &r = c.pairs[(int64_t)I * (I + 1) / 2 + J];
+            r.pp0 = (int)c.pp.size();
+            r.npp = a.npr * b.npr;
+            for (int i = 0; i < a.npr; ++i)
+                for (int j = 0; j < b.npr; ++j) {
+                    PrimPair q;
+                    double al = a.al[i], be = b.al[j], p = al + be;
+                    q.p = p;
+                    if (samec) {
+                        // same centre: product at A, unit overlap factor (twoints.c:48-57)
+                        for (int x = 0; x < 3; ++x) q.P[x] = A[x];
+                        q.s = 1.0;
+                    } else {
+                        for (int x = 0; x < 3; ++x) q.P[x] = (al * A[x] + be * B[x]) / p;
+                        q.s = exp(-al * be * ab2 / p);
+                    }
+                    q.c = a.co[i] * b.co[j];
+                    c.pp.push_back(q);
+                }
+        }
+    }
+    // tiles per l
+    c.tile_first.clear();
+    c.tile_size.clear();
+    c.tile_l.clear();
+    for (int i = 0; i < ns;) {
+        int l = c.shell_l[i], T = tile_of(l), n = 0;
+        while (i + n < ns && c.shell_l[i + n] == l && n < T) ++n;
+        c.tile_first.push_back(i);
+        c.tile_size.push_back(n);
+        c.tile_l.push_back(l);
+        i += n;
+    }
+    std::vector<int> orig_sorted_arr(ns);
+    for (int i = 0; i < ns; ++i) orig_sorted_arr[i] = c.sorted_orig[i];
+    if (c.d_xyz.upload(c.shell_xyz) || c.d_off.upload(c.shell_off) || c.d_orig.upload(orig_sorted_arr) ||
+        c.d_pairs.upload(c.pairs) || c.d_pp.upload(c.pp) || c.d_tile_first.upload(c.tile_first) ||
+        c.d_tile_size.upload(c.tile_size))
+        return g_err;
+    // Schwarz bounds on the device
+    if (c.d_q.alloc(np)) return g_err;
+    for (int la = 0; la <= PM_LMAX; ++la)
+        for (int lb = 0; lb <= la; ++lb) {
+            std::vector<int> pl, pi, pj;
+            for (int I = 0; I < ns; ++I)
+                for (int J = 0; J <= I; ++J)
+                    if (c.shell_l[I] == la && c.shell_l[J] == lb) {
+                        pl.push_back((int)((int64_t)I * (I + 1) / 2 + J));
+                        pi.push_back(I);
+                        pj.push_back(J);
+                    }
+            if (pl.empty()) continue;
+            DevBuf<int> dl, di, dj;
+            if (dl.upload(pl) || di.upload(pi) || dj.upload(pj)) return g_err;
+            SchwarzArgs sa{dl.p, di.p, dj.p, (int)pl.size(), c.d_xyz.p, c.d_pairs.p, c.d_pp.p, c.d_q.p, g_rys};
+            if (launch_schwarz(pair_class(la, lb), sa, 0)) return fail(PATMOL_HIP_EUNSUPPORTED, "no Schwarz kernel");
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipDeviceSynchronize());
+            dl.release();
+            di.release();
+            dj.release();
+        }
+    c.pair_q.resize(np);
+    HIPCHK(hipMemcpy(c.pair_q.data(), c.d_q.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+    // tile-pair bounds
+    const int nt = (int)c.tile_first.size();
+    std::vector<int> tmaxorig(nt, -1);
+    for (int t = 0; t < nt; ++t)
+        for (int i = 0; i < c.tile_size[t]; ++i)
+            tmaxorig[t] = std::max(tmaxorig[t], c.sorted_orig[c.tile_first[t] + i]);
+    struct TP {
+        int a, b, pc;
+        double q;
+        int maxorig;
+    };
+    std::vector<TP> tps;
+    for (int ta = 0; ta < nt; ++ta)
+        for (int tb = 0; tb <= ta; ++tb) {
+            double q = 0.0;
+            for (int i = 0; i < c.tile_size[ta]; ++i) {
+                int I = c.tile_first[ta] + i;
+                for (int j = 0; j < c.tile_size[tb]; ++j) {
+                    int J = c.tile_first[tb] + j;
+                    if (J > I) continue;
+                    q = std::max(q, c.pair_q[(int64_t)I * (I + 1) / 2 + J]);
+                }
+            }
+            tps.push_back({ta, tb, pair_class(c.tile_l[ta], c.tile_l[tb]), q, std::max(tmaxorig[ta], tmaxorig[tb])});
+        }
+    // tile quartets: canonical (P1 >= P2), screened by the bound, bra = higher pair class
+    struct Item {
+        int4 t;
+        int bpc, kpc;
+        double bound;
+    };
+    std::vector<Item> items;
+    for (size_t p1 = 0; p1 < tps.size(); ++p1)
+        for (size_t p2 = 0; p2 <= p1; ++p2) {
+            const TP &A = tps[p1], &B = tps[p2];
+            double bound = A.q * B.q;
+            if (bound < c.tau) continue;
+            if (start > 0 && std::max(A.maxorig, B.maxorig) < start) continue;
+            const TP &bra = (A.pc >= B.pc) ? A : B;
+            const TP &ket = (A.pc >= B.pc) ? B : A;
+            items.push_back({make_int4(bra.a, bra.b, ket.a, ket.b), bra.pc, ket.pc, bound});
+        }
+    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+        if (x.bpc != y.bpc) return x.bpc > y.bpc;
+        if (x.kpc != y.kpc) return x.kpc > y.kpc;
+        return x.bound > y.bound;
+    });
+    c.items.resize(items.size());
+    c.item_bound.resize(items.size());
+    c.ranges.clear();
+    for (size_t i = 0; i < items.size(); ++i) {
+        c.items[i] = items[i].t;
+        c.item_bound[i] = items[i].bound;
+        if (c.ranges.empty() || c.ranges.back().bra_pc != items[i].bpc || c.ranges.back().ket_pc != items[i].kpc)
+            c.ranges.push_back({items[i].bpc, items[i].kpc, (int)i, 0});
+        c.ranges.back().count++;
+    }
+    if (c.d_items.upload(c.items) || c.d_bound.upload(c.item_bound)) return g_err;
+    c.built = true;
+    return 0;
+}
+
+// unpack packed lower triangle (no2) to full symmetric N x N
+__global__ void unpack_kernel(const double *__restrict__ p, double *__restrict__ f, int N)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)N * N) return;
+    int i = (int)(idx / N), j = (int)(idx % N);
+    int a = i >= j ? i : j, b = i >= j ? j : i;
+    f[idx] = p[(int64_t)a * (a + 1) / 2 + b];
+}
+
+__global__ void add_kernel(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ o, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = a[i] + b[i];
+}
+
+// G[ij] += cj (J'ij + J'ji) - (K'ij + K'ji), packed i >= j
+__global__ void finalize_kernel(const double *__restrict__ J, const double *__restrict__ K, double *__restrict__ G,
+                                int N, double cj)
+{
+    int64_t ij = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t no2 = (int64_t)N * (N + 1) / 2;
+    if (ij >= no2) return;
+    int i = (int)((sqrt(8.0 * (double)ij + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(i + 1) * (i + 2) / 2 <= ij) ++i;
+    while ((int64_t)i * (i + 1) / 2 > ij) --i;
+    int j = (int)(ij - (int64_t)i * (i + 1) / 2);
+    double jj = J[(int64_t)i * N + j] + J[(int64_t)j * N + i];
+    double kk = K[(int64_t)i * N + j] + K[(int64_t)j * N + i];
+    G[ij] += cj * jj - kk;
+}
+
+__global__ void absmax_kernel(const double *__restrict__ p, int64_t n, double *out)
+{
+    __shared__ double s[256];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(p[i]));
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// Launch every class over the device densities already in c.d_D (+ d_D2).
+// nk = 1: K from d_D; nk = 2: K0 from d_D, K1 from d_D2, J from d_D + d_D2
+// (passed as DJ).
+static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1, int nk, double dmax, hipStream_t s)
+{
+    const int64_t NN = (int64_t)c.N * c.N;
+    HIPCHK(hipMemsetAsync(c.d_J.p, 0, sizeof(double) * NN, s));
+    HIPCHK(hipMemsetAsync(c.d_K0.p, 0, sizeof(double) * NN, s));
+    if (nk == 2) HIPCHK(hipMemsetAsync(c.d_K1.p, 0, sizeof(double) * NN, s));
+    if (g_timing) {
+        if (!c.ev0) {
+            HIPCHK(hipEventCreate(&c.ev0));
+            HIPCHK(hipEventCreate(&c.ev1));
+            c.cls_ev.resize(2 * 64);
+            for (auto &e : c.cls_ev) HIPCHK(hipEventCreate(&e));
+        }
+        HIPCHK(hipEventRecord(c.ev0, s));
+    }
+    c.items_fp32 = 0;
+    int ci = 0;
+    for (const ClassRange &r : c.ranges) {
+        // items are sorted by bound within a class: FP32 items form a suffix
+        int n64 = r.count;
+        if (g_prec_mode == 1 && dmax > 0.0) {
+            double thr = g_prec_thr / dmax;
+            n64 = (int)(std::lower_bound(c.item_bound.begin() + r.first, c.item_bound.begin() + r.first + r.count, thr,
+                                         [](double b, double t) { return b >= t; }) -
+                        (c.item_bound.begin() + r.first));
+        }
+        JKArgs a;
+        a.tile_first = c.d_tile_first.p;
+        a.tile_size = c.d_tile_size.p;
+        a.shell_xyz = c.d_xyz.p;
+        a.shell_off = c.d_off.p;
+        a.shell_orig = c.d_orig.p;
+        a.pairs = c.d_pairs.p;
+        a.pair_q = c.d_q.p;
+        a.pp = c.d_pp.p;
+        a.DJ = DJ;
+        a.DK0 = DK0;
+        a.DK1 = DK1 ? DK1 : DK0;
+        a.J = c.d_J.p;
+        a.K0 = c.d_K0.p;
+        a.K1 = nk == 2 ? c.d_K1.p : c.d_K0.p;
+        a.N = c.N;
+        a.tau = c.tau;
+        a.start_shell = c.start;
+        a.rys = g_rys;
+        if (g_timing && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
+        if (n64 > 0) {
+            a.items = c.d_items.p + r.first;
+            a.nitems = n64;
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 0, nk, s)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
+        }
+        if (r.count > n64) {
+            a.items = c.d_items.p + r.first + n64;
+            a.nitems = r.count - n64;
+            c.items_fp32 += r.count - n64;
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, nk, s)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
+        }
+        if (g_timing && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci + 1], s));
+        HIPCHK(hipGetLastError());
+        ++ci;
+    }
+    if (g_timing) HIPCHK(hipEventRecord(c.ev1, s));
+    return 0;
+}
+
+static int collect_timing(Ctx &c)
+{
+    if (!g_timing || !c.ev0) return 0;
+    HIPCHK(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.last_ms = ms;
+    for (double &x : c.class_ms) x = 0.0;
+    int ci = 0;
+    for (const ClassRange &r : c.ranges) {
+        if (ci >= 64) break;
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, c.cls_ev[2 * ci], c.cls_ev[2 * ci + 1]));
+        c.class_ms[r.bra_pc * 6 + r.ket_pc] += t;
+        ++ci;
+    }
+    return 0;
+}
+
+static int alloc_work(Ctx &c, int nk)
+{
+    const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
+    if (c.d_D.alloc(NN) || c.d_J.alloc(NN) || c.d_K0.alloc(NN) || c.d_P.alloc(no2) || c.d_G.alloc(no2) ||
+        c.d_dmax.alloc(1))
+        return g_err;
+    if (nk == 2 && (c.d_D2.alloc(NN) || c.d_K1.alloc(NN) || c.d_G2.alloc(no2))) return g_err;
+    return 0;
+}
+
+static double device_absmax(Ctx &c, const double *p, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(256), 0, s, p, n, c.d_dmax.p);
+    double m = 0.0;
+    if (hipMemcpyAsync(&m, c.d_dmax.p, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return -1.0;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1.0;
+    return m;
+}
+
+static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
+{
+    if (alloc_work(c, 1)) return g_err;
+    const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dP, c.d_D.p, c.N);
+    double dmax = 0.0;
+    if (g_prec_mode == 1) dmax = device_absmax(c, dP, no2, s);
+    if (run_jk(c, c.d_D.p, c.d_D.p, nullptr, 1, dmax, s)) return g_err;
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, c.d_J.p, c.d_K0.p, dG, c.N, 2.0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int dpm_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+
+RysTables dpm_tables()
+{
+    g_err = PATMOL_HIP_OK;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (ensure_tables()) return RysTables{nullptr, nullptr, nullptr, nullptr};
+    return g_rys;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int patmol_hip_last_error(void) { return g_err; }
+const char *patmol_hip_error_string(void) { return g_msg; }
+void patmol_hip_set_strict(int s) { g_strict = s ? 1 : 0; }
+void patmol_hip_set_schwarz_threshold(double tau) { g_tau = tau < 0 ? 0 : tau; }
+void patmol_hip_set_precision(int mode, double thr)
+{
+    g_prec_mode = mode ? 1 : 0;
+    if (thr > 0) g_prec_thr = thr;
+}
+void patmol_hip_set_timing(int on) { g_timing = on != 0; }
+
+// erifs.c:32-60: the handle keeps the reference's public fields; nfiles
+// stays 0 because no integral files are written.
+ERIFS *init_ERIFS(int nst, int nend, const char *sname)
+{
+    g_err = PATMOL_HIP_OK;
+    (void)nst;
+    (void)nend;
+    ERIFS *e = (ERIFS *)calloc(1, sizeof(ERIFS));
+    if (!e) {
+        fail(PATMOL_HIP_EBADARG, "out of host memory");
+        return nullptr;
+    }
+    e->rank = 0;
+    e->nproc = 1;
+    e->nfiles = 0;
+    e->numints = (int *)calloc(1, sizeof(int));
+    const char *nm = sname ? sname : "";
+    e->basename = (char *)malloc(strlen(nm) + 1);
+    strcpy(e->basename, nm);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ctx[e] = new Ctx();
+    return e;
+}
+
+void clean_ERIFS(ERIFS *e)
+{
+    if (!e) return;
+    Ctx *c = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_ctx.find(e);
+        if (it != g_ctx.end()) {
+            c = it->second;
+            g_ctx.erase(it);
+        }
+    }
+    if (c) {
+        for (auto *b : {&c->d_xyz, &c->d_q, &c->d_bound, &c->d_P, &c->d_D, &c->d_D2, &c->d_J, &c->d_K0, &c->d_K1,
+                        &c->d_G, &c->d_G2, &c->d_dmax})
+            b->release();
+        for (auto *b : {&c->d_off, &c->d_orig, &c->d_tile_first, &c->d_tile_size}) b->release();
+        c->d_pairs.release();
+        c->d_pp.release();
+        c->d_items.release();
+        if (c->ev0) {
+            (void)hipEventDestroy(c->ev0);
+            (void)hipEventDestroy(c->ev1);
+            for (auto ev : c->cls_ev) (void)hipEventDestroy(ev);
+        }
+        delete c;
+    }
+    free(e->numints);
+    free(e->basename);
+    free(e);
+}
+
+void two_electron_ints(ERIFS *e, int start)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *c = ctx_of(e);
+    if (!c) {
+        fail(PATMOL_HIP_EBADARG, "two_electron_ints: unknown ERIFS handle");
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (ensure_tables()) return;
+    }
+    build_tables(*c, start < 0 ? 0 : start);
+}
+
+void rhf_form_gmatrix(const ERIFS *e, const double *P, double *G)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *c = ctx_of(e);
+    if (!c || !P || !G) {
+        fail(PATMOL_HIP_EBADARG, "rhf_form_gmatrix: bad handle or null matrix");
+        return;
+    }
+    if (!c->built) {
+        fail(PATMOL_HIP_ESTATE, "rhf_form_gmatrix: two_electron_ints was not run for this handle");
+        return;
+    }
+    const int64_t no2 = (int64_t)c->N * (c->N + 1) / 2;
+    if (alloc_work(*c, 1)) return;
+    if (hipMemcpy(c->d_P.p, P, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->d_G.p, 0, sizeof(double) * no2) != hipSuccess) {
+        fail(PATMOL_HIP_EDEVICE, "rhf_form_gmatrix: transfer failed");
+        return;
+    }
+    if (rhf_device(*c, c->d_P.p, c->d_G.p, 0)) return;
+    std::vector<double> g(no2);
+    if (hipMemcpy(g.data(), c->d_G.p, sizeof(double) * no2, hipMemcpyDeviceToHost) != hipSuccess) {
+        fail(PATMOL_HIP_EDEVICE, "rhf_form_gmatrix: transfer failed");
+        return;
+    }
+    collect_timing(*c);
+    for (int64_t i = 0; i < no2; ++i) G[i] += g[i];
+}
+
+void uhf_form_gmatrix(const ERIFS *e, const double *PA, const double *PB, double *GA, double *GB)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *c = ctx_of(e);
+    if (!c || !PA || !PB || !GA || !GB) {
+        fail(PATMOL_HIP_EBADARG, "uhf_form_gmatrix: bad handle or null matrix");
+        return;
+    }
+    if (!c->built) {
+        fail(PATMOL_HIP_ESTATE, "uhf_form_gmatrix: two_electron_ints was not run for this handle");
+        return;
+    }
+    Ctx &x = *c;
+    const int64_t NN = (int64_t)x.N * x.N, no2 = (int64_t)x.N * (x.N + 1) / 2;
+    if (alloc_work(x, 2)) return;
+    DevBuf<double> dPB, dDJ;
+    if (dPB.alloc(no2) || dDJ.alloc(NN)) return;
+    if (hipMemcpy(x.d_P.p, PA, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dPB.p, PB, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(x.d_G.p, 0, sizeof(double) * no2) != hipSuccess ||
+        hipMemset(x.d_G2.p, 0, sizeof(double) * no2) != hipSuccess) {
+        fail(PATMOL_HIP_EDEVICE, "uhf_form_gmatrix: transfer failed");
+        return;
+    }
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, x.d_P.p, x.d_D.p, x.N);
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, dPB.p, x.d_D2.p, x.N);
+    hipLaunchKernelGGL(add_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, x.d_D.p, x.d_D2.p, dDJ.p, NN);
+    double dmax = 0.0;
+    if (g_prec_mode == 1) dmax = std::max(device_absmax(x, x.d_P.p, no2, 0), device_absmax(x, dPB.p, no2, 0));
+    if (run_jk(x, dDJ.p, x.d_D.p, x.d_D2.p, 2, dmax, 0)) return;
+    // GA += J[PA+PB] - K[PA], GB += J[PA+PB] - K[PB]  (J = J' + J'^T)
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, 0, x.d_J.p, x.d_K0.p, x.d_G.p, x.N, 1.0);
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, 0, x.d_J.p, x.d_K1.p, x.d_G2.p, x.N, 1.0);
+    std::vector<double> ga(no2), gb(no2);
+    if (hipMemcpy(ga.data(), x.d_G.p, sizeof(double) * no2, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(gb.data(), x.d_G2.p, sizeof(double) * no2, hipMemcpyDeviceToHost) != hipSuccess) {
+        fail(PATMOL_HIP_EDEVICE, "uhf_form_gmatrix: transfer failed");
+        return;
+    }
+    collect_timing(x);
+    dPB.release();
+    dDJ.release();
+    for (int64_t i = 0; i < no2; ++i) {
+        GA[i] += ga[i];
+        GB[i] += gb[i];
+    }
+}
+
+int patmol_hip_rhf_gmatrix_device(const ERIFS *e, const double *dP, double *dG, void *stream)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *c = ctx_of(e);
+    if (!c || !dP || !dG) return fail(PATMOL_HIP_EBADARG, "bad handle or null pointer");
+    if (!c->built) return fail(PATMOL_HIP_ESTATE, "two_electron_ints was not run for this handle");
+    return rhf_device(*c, dP, dG, (hipStream_t)stream);
+}
+
+int patmol_hip_last_timing(const ERIFS *e, double *total_ms, double *class_ms)
+{
+    Ctx *c = ctx_of(e);
+    if (!c) return fail(PATMOL_HIP_EBADARG, "bad handle");
+    if (collect_timing(*c)) return g_err;
+    if (total_ms) *total_ms = c->last_ms;
+    if (class_ms) memcpy(class_ms, c->class_ms, sizeof(c->class_ms));
+    return 0;
+}
+
+int patmol_hip_pair_schwarz(const ERIFS *e, double *q)
+{
+    Ctx *c = ctx_of(e);
+    if (!c || !c->built) return fail(PATMOL_HIP_ESTATE, "no tables");
+    for (int I = 0; I < c->nshell; ++I)
+        for (int J = 0; J <= I; ++J) {
+            int a = c->orig_to_sorted[I], b = c->orig_to_sorted[J];
+            int hi = std::max(a, b), lo = std::min(a, b);
+            q[(int64_t)I * (I + 1) / 2 + J] = c->pair_q[(int64_t)hi * (hi + 1) / 2 + lo];
+        }
+    return 0;
+}
+
+int64_t patmol_hip_work_items(const ERIFS *e, int *tiles4, int *classes2, double *bounds, int *sorted_orig,
+                              int *tile_first, int *tile_size, int *ntiles)
+{
+    Ctx *c = ctx_of(e);
+    if (!c || !c->built) return -fail(PATMOL_HIP_ESTATE, "no tables");
+    for (size_t i = 0; i < c->items.size(); ++i) {
+        if (tiles4) {
+            tiles4[4 * i] = c->items[i].x;
+            tiles4[4 * i + 1] = c->items[i].y;
+            tiles4[4 * i + 2] = c->items[i].z;
+            tiles4[4 * i + 3] = c->items[i].w;
+        }
+        if (bounds) bounds[i] = c->item_bound[i];
+    }
+    if (classes2)
+        for (const ClassRange &r : c->ranges)
+            for (int i = 0; i < r.count; ++i) {
+                classes2[2 * (r.first + i)] = r.bra_pc;
+                classes2[2 * (r.first + i) + 1] = r.ket_pc;
+            }
+    if (sorted_orig) memcpy(sorted_orig, c->sorted_orig.data(), sizeof(int) * c->nshell);
+    if (tile_first) memcpy(tile_first, c->tile_first.data(), sizeof(int) * c->tile_first.size());
+    if (tile_size) memcpy(tile_size, c->tile_size.data(), sizeof(int) * c->tile_size.size());
+    if (ntiles) *ntiles = (int)c->tile_first.size();
+    return (int64_t)c->items.size();
+}
+
+// FLOPs of one primitive quartet of class (la lb|lc ld), la>=lb, lc>=ld:
+// roots + per-root recursion + per-component transfer and product, the
+// algorithmic count used for roofline reporting.
+static double class_flops(int la, int lb, int lc, int ld)
+{
+    int L12 = la + lb, L34 = lc + ld, nr = (L12 + L34) / 2 + 1;
+    double F = 40 + 40.0 * nr + nr * (12 + 21.0 * (L12 + 1) * (L34 + 1));
+    auto comps = [](int l, std::vector<std::array<int, 3>> &v) {
+        for (int x = l; x >= 0; --x)
+            for (int y = l - x; y >= 0; --y) v.push_back({x, y, l - x - y});
+    };
+    std::vector<std::array<int, 3>> A, B, C, D;
+    comps(la, A);
+    comps(lb, B);
+    comps(lc, C);
+    comps(ld, D);
+    for (size_t a = 0; a < A.size(); ++a)
+        for (auto &b : B)
+            for (size_t cc = 0; cc < C.size(); ++cc)
+                for (auto &d : D) {
+                    double s = 0;
+                    for (int x = 0; x < 3; ++x) s += (b[x] + 1) * (d[x] + 1);
+                    F += nr * (4 + 3 * s) + 12;
+                }
+    return F;
+}
+
+int patmol_hip_stats(const ERIFS *e, PatmolHipStats *out)
+{
+    Ctx *cp = ctx_of(e);
+    if (!cp || !cp->built || !out) return fail(PATMOL_HIP_ESTATE, "no tables");
+    Ctx &c = *cp;
+    memset(out, 0, sizeof(*out));
+    out->nshell = c.nshell;
+    out->norb = c.N;
+    out->start = c.start;
+    const int ns = c.nshell;
+    const int64_t np = (int64_t)ns * (ns + 1) / 2;
+    out->pairs = np;
+    out->shell_quartets_total = np * (np + 1) / 2;
+    out->work_items = (int64_t)c.items.size();
+    out->work_items_fp32 = c.items_fp32;
+    double fl[6][6];
+    int pcl[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+    for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) fl[a][b] = class_flops(pcl[a][0], pcl[a][1], pcl[b][0], pcl[b][1]);
+    // exact count over the kernels' own enumeration
+    int64_t kept = 0, pkept = 0;
+    double flops = 0;
+    for (const ClassRange &r : c.ranges) {
+        for (int it = r.first; it < r.first + r.count; ++it) {
+            int4 t = c.items[it];
+            bool sameBra = t.x == t.y, sameKet = t.z == t.w, samePair = t.x == t.z && t.y == t.w;
+            int64_t k = 0, pk = 0;
+            for (int i = 0; i < c.tile_size[t.x]; ++i) {
+                int I = c.tile_first[t.x] + i;
+                for (int j = 0; j < c.tile_size[t.y]; ++j) {
+                    int J = c.tile_first[t.y] + j;
+                    if (sameBra && J > I) continue;
+                    int64_t pb = (int64_t)I * (I + 1) / 2 + J;
+                    double qb = c.pair_q[pb];
+                    for (int kq = 0; kq < c.tile_size[t.z]; ++kq) {
+                        int K = c.tile_first[t.z] + kq;
+                        for (int lq = 0; lq < c.tile_size[t.w]; ++lq) {
+                            int L = c.tile_first[t.w] + lq;
+                            if (sameKet && L > K) continue;
+                            if (samePair && !(I > K || (I == K && J >= L))) continue;
+                            int64_t pk2 = (int64_t)K * (K + 1) / 2 + L;
+                            if (qb * c.pair_q[pk2] < c.tau) continue;
+                            if (c.start > 0 &&
+                                std::max(std::max(c.sorted_orig[I], c.sorted_orig[J]), std::max(c.sorted_orig[K], c.sorted_orig[L])) < c.start)
+                                continue;
+                            ++k;
+                            pk += (int64_t)c.pairs[pb].npp * c.pairs[pk2].npp;
+                        }
+                    }
+                }
+            }
+            kept += k;
+            pkept += pk;
+            flops += pk * fl[r.bra_pc][r.ket_pc];
+        }
+    }
+    out->shell_quartets_kept = kept;
+    out->prim_quartets_kept = pkept;
+    out->flops_per_build = flops;
+    return 0;
+}
+
+int patmol_hip_eri_blocks(const ERIFS *e, int nq, const int *quartets, double *out)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *cp = ctx_of(e);
+    if (!cp || !cp->built) return fail(PATMOL_HIP_ESTATE, "no tables");
+    Ctx &c = *cp;
+    if (nq <= 0) return 0;
+    struct Q {
+        int s[4];      // sorted indices in kernel orientation
+        int perm[4];   // kernel position of requested position
+        int bpc, kpc;
+        size_t out_off, dev_off;
+    };
+    std::vector<Q> qs(nq);
+    size_t total = 0;
+    for (int q = 0; q < nq; ++q) {
+        int o[4];
+        for (int k = 0; k < 4; ++k) {
+            o[k] = quartets[4 * q + k];
+            if (o[k] < 0 || o[k] >= c.nshell) return fail(PATMOL_HIP_EBADARG, "shell index out of range");
+        }
+        int s[4] = {c.orig_to_sorted[o[0]], c.orig_to_sorted[o[1]], c.orig_to_sorted[o[2]], c.orig_to_sorted[o[3]]};
+        int pos[4] = {0, 1, 2, 3};  // kernel slot -> requested slot
+        auto lo = [&](int slot) { return c.shell_l[s[pos[slot]]]; };
+        if (lo(0) < lo(1)) std::swap(pos[0], pos[1]);
+        if (lo(2) < lo(3)) std::swap(pos[2], pos[3]);
+        int bpc = pair_class(lo(0), lo(1)), kpc = pair_class(lo(2), lo(3));
+        if (bpc < kpc) {
+            std::swap(pos[0], pos[2]);
+            std::swap(pos[1], pos[3]);
+            std::swap(bpc, kpc);
+        }
+        Q &x = qs[q];
+        for (int k = 0; k < 4; ++k) {
+            x.s[k] = s[pos[k]];
+            x.perm[pos[k]] = k;
+        }
+        x.bpc = bpc;
+        x.kpc = kpc;
+        x.out_off = total;
+        total += (size_t)c.shell_nst[s[0]] * c.shell_nst[s[1]] * c.shell_nst[s[2]] * c.shell_nst[s[3]];
+    }
+    // group by class
+    std::vector<double> res(total);
+    for (int b = 0; b < 6; ++b)
+        for (int k = 0; k <= b; ++k) {
+            std::vector<int> idx;
+            for (int q = 0; q < nq; ++q)
+                if (qs[q].bpc == b && qs[q].kpc == k) idx.push_back(q);
+            if (idx.empty()) continue;
+            std::vector<int> qv;
+            for (int q : idx)
+                for (int t = 0; t < 4; ++t) qv.push_back(qs[q].s[t]);
+            size_t ncomp = (size_t)c.shell_nst[qs[idx[0]].s[0]] * c.shell_nst[qs[idx[0]].s[1]] * c.shell_nst[qs[idx[0]].s[2]] *
+                           c.shell_nst[qs[idx[0]].s[3]];
+            DevBuf<int> dq;
+            DevBuf<double> dout;
+            if (dq.upload(qv) || dout.alloc(ncomp * idx.size())) return g_err;
+            EriOutArgs a{dq.p, (int)idx.size(), c.d_xyz.p, c.d_pairs.p, c.d_pp.p, dout.p, g_rys};
+            if (launch_eri_out(b, k, a, 0)) return fail(PATMOL_HIP_EUNSUPPORTED, "no ERI kernel");
+            HIPCHK(hipGetLastError());
+            std::vector<double> h(ncomp * idx.size());
+            HIPCHK(hipMemcpy(h.data(), dout.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+            dq.release();
+            dout.release();
+            for (size_t m = 0; m < idx.size(); ++m) {
+                const Q &x = qs[idx[m]];
+                int n[4] = {c.shell_nst[x.s[0]], c.shell_nst[x.s[1]], c.shell_nst[x.s[2]], c.shell_nst[x.s[3]]};
+                // requested dims
+                int rn[4];
+                for (int r = 0; r < 4; ++r) rn[r] = n[x.perm[r]];
+                const double *src = h.data() + m * ncomp;
+                int ki[4];
+                for (ki[0] = 0; ki[0] < n[0]; ++ki[0])
+                    for (ki[1] = 0; ki[1] < n[1]; ++ki[1])
+                        for (ki[2] = 0; ki[2] < n[2]; ++ki[2])
+                            for (ki[3] = 0; ki[3] < n[3]; ++ki[3]) {
+                                int r0 = ki[x.perm[0]], r1 = ki[x.perm[1]], r2 = ki[x.perm[2]], r3 = ki[x.perm[3]];
+                                size_t dst = ((size_t)(r0 * rn[1] + r1) * rn[2] + r2) * rn[3] + r3;
+                                res[x.out_off + dst] = src[((ki[0] * n[1] + ki[1]) * n[2] + ki[2]) * n[3] + ki[3]];
+                            }
+            }
+        }
+    memcpy(out, res.data(), sizeof(double) * total);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+"""GPU parity of the HIP two-electron path against the oracle (FP64).
+
+Every test calls the HIP library through its C ABI and compares with the
+oracle's restatement of the reference algorithm on the same inputs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import patmol_amd as pm
+from patmol_amd import systems
+from tests.gpu_util import note, packed, random_density
+
+pytestmark = pytest.mark.gpu
+
+ERI_TOL = 1e-11      # absolute, FP64 integrals
+G_TOL = 1e-10        # absolute, FP64 G elements (sums of ~1e5 integrals)
+
+
+def _store(sys_, tau=0.0, start=0):
+    pm.load_system(sys_)
+    pm.set_options(tau=tau, precision=0, strict=False)
+    return pm.EriStore(start=start)
+
+
+def _quartets(ns, n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, ns, size=(n, 4))
+
+
+@pytest.mark.parametrize("name,factory,nq", [
+    ("h2o", systems.water, 400),
+    ("h2o_d", lambda: systems.water(d_shells=True), 600),
+])
+def test_eri_blocks_vs_oracle(name, factory, nq):
+    s = factory()
+    note(f"eri_blocks {name} nshell={s.nshell}")
+    st = _store(s)
+    b = oracle.Basis(s)
+    qs = _quartets(s.nshell, nq, 11)
+    # every class at least once: add diagonal-ish quartets over l combinations
+    by_l = {}
+    for i, sh in enumerate(s.shells):
+        by_l.setdefault(sh.l, i)
+    extra = [(by_l[a], by_l[b], by_l[c], by_l[d]) for a in by_l for b in by_l for c in by_l for d in by_l]
+    qs = np.vstack([qs, np.array(extra)])
+    got = st.eri_blocks(qs, b.nst)
+    worst = 0.0
+    for (I, J, K, L), g in zip(qs, got):
+        ref = b.eri_block(int(I), int(J), int(K), int(L))
+        worst = max(worst, float(np.abs(g - ref).max()))
+    note(f"  max |dERI| = {worst:.3e}")
+    assert worst < ERI_TOL
+
+
+def test_eri_blocks_sf6_contracted(golden_dir):
+    s = systems.read_patin(os.path.join(golden_dir, "sf6_patin.dat"))
+    st = _store(s)
+    b = oracle.Basis(s)
+    qs = _quartets(s.nshell, 300, 5)
+    got = st.eri_blocks(qs, b.nst)
+    worst = max(float(np.abs(g - b.eri_block(*map(int, q))).max()) for q, g in zip(qs, got))
+    note(f"sf6 contracted max |dERI| = {worst:.3e}")
+    assert worst < ERI_TOL
+
+
+@pytest.mark.parametrize("name,factory", [("h2o", systems.water), ("h2o_d", lambda: systems.water(d_shells=True))])
+def test_rhf_gmatrix_vs_oracle_scatter(name, factory):
+    s = factory()
+    note(f"rhf G {name}")
+    st = _store(s, tau=0.0)
+    b = oracle.Basis(s)
+    ints = b.two_electron_ints()
+    P = packed(random_density(s.norb, 3))
+    G_ref = ints.rhf_gmatrix(P)
+    G = st.rhf_gmatrix(P)
+    d = float(np.abs(G - G_ref).max())
+    note(f"  max |dG| = {d:.3e} (|G|max {np.abs(G_ref).max():.2f})")
+    assert d < G_TOL
+    # accumulate semantics: G += 2J-K
+    G2 = st.rhf_gmatrix(P, G.copy())
+    assert np.allclose(G2, 2 * G, rtol=0, atol=1e-12)
+
+
+def test_uhf_gmatrix_vs_oracle_scatter():
+    s = systems.water(d_shells=True)
+    st = _store(s, tau=0.0)
+    b = oracle.Basis(s)
+    ints = b.two_electron_ints()
+    PA = packed(random_density(s.norb, 4))
+    PB = packed(random_density(s.norb, 5))
+    GA_ref, GB_ref = ints.uhf_gmatrix(PA, PB)
+    GA, GB = st.uhf_gmatrix(PA, PB)
+    d = max(float(np.abs(GA - GA_ref).max()), float(np.abs(GB - GB_ref).max()))
+    note(f"uhf max |dG| = {d:.3e}")
+    assert d < G_TOL
+
+
+def test_work_items_match_cpu_mirror():
+    s = systems.water_cluster(2, 1, 1, seed=7)
+    tau = 1e-10
+    st = _store(s, tau=tau)
+    gpu = st.work_items()
+    q = st.pair_schwarz()
+    cpu = oracle.work_items_mirror(s, q, tau)
+    for k in ("sorted_orig", "tile_first", "tile_size", "tiles", "classes"):
+        assert np.array_equal(gpu[k], cpu[k]), k
+    assert np.array_equal(gpu["bounds"], cpu["bounds"])
+    note(f"work items: {len(gpu['tiles'])} identical to the CPU mirror")
+
+
+def test_schwarz_bounds_vs_oracle():
+    s = systems.water(d_shells=True)
+    st = _store(s)
+    b = oracle.Basis(s)
+    q = st.pair_schwarz()
+    for I in range(s.nshell):
+        for J in range(I + 1):
+            blk = b.eri_block(I, J, I, J)
+            n1, n2 = blk.shape[0], blk.shape[1]
+            diag = max(abs(blk[a, c, a, c]) for a in range(n1) for c in range(n2))
+            assert abs(q[I * (I + 1) // 2 + J] - np.sqrt(diag)) <= 1e-12 * max(1.0, np.sqrt(diag))
+
+
+def test_screened_gmatrix_close_to_unscreened():
+    s = systems.water_cluster(2, 1, 1, seed=7)
+    P = packed(random_density(s.norb, 9))
+    G0 = _store(s, tau=0.0).rhf_gmatrix(P)
+    G1 = _store(s, tau=1e-12).rhf_gmatrix(P)
+    d = float(np.abs(G1 - G0).max())
+    note(f"screened vs unscreened |dG| = {d:.3e}")
+    assert d < 1e-9
+
+
+def test_dpm_ints_vs_oracle():
+    s = systems.water(d_shells=True)
+    pm.load_system(s)
+    H, G = pm.lib()
+    H.patmol_basis_add_dpm_center()
+    H.dpm_augment()
+    Q = np.array([0.3, -0.4, 2.0])
+    H.set_center_position(s.ncen, Q)
+    n = H.number_of_orbitals()
+    hm = np.zeros(n * (n + 1) // 2)
+    G.dpm_ints(hm)
+    pm._check("dpm_ints")
+    # oracle on the same augmented basis
+    import copy
+    aug = copy.deepcopy(s)
+    aug.charges = np.append(s.charges, 1.0)
+    aug.coords = np.vstack([s.coords, Q])
+    alf = [13.3615, 2.0133, 0.453757, 1.2331, 2.5719, 0.59976, 0.139365]
+    lsh = [0, 0, 0, 0, 1, 1, 1]
+    aug.shells = list(s.shells)
+    b = oracle.Basis(aug, normalize=True)
+    # DPM shells are unnormalised (co = 1): rebuild them without normalisation
+    extra = systems.System(aug.charges, aug.coords,
+                           [systems.Shell(l, s.ncen, [a], [1.0]) for a, l in zip(alf, lsh)], nelec=0)
+    full = systems.System(aug.charges, aug.coords, list(s.shells) + extra.shells, nelec=s.nelec)
+    q_, xyz, npr, lsh_, cen, al, co = full.flat()
+    co_norm = oracle.Basis(aug).coefs()
+    co = np.concatenate([co_norm, np.ones(7)])
+    bh = oracle.lib().or_basis_new(len(q_), q_, xyz, len(npr), npr, lsh_, cen, al, co, 0)
+    href = np.zeros_like(hm)
+    oracle.lib().or_dpm_ints(bh, np.ascontiguousarray(Q), href)
+    oracle.lib().or_basis_free(bh)
+    d = float(np.abs(hm - href).max())
+    note(f"dpm max |dV| = {d:.3e} (|V|max {np.abs(href).max():.3f})")
+    assert d < 1e-12
+
+
+def test_rhf_scf_water_vs_oracle():
+    s = systems.water()
+    pm.load_system(s)
+    pm.set_options(tau=0.0, precision=0)
+    res, P, ev = pm.rhf_scf()
+    b = oracle.Basis(s)
+    ints = b.two_electron_ints()
+    ref, Pr, evr = oracle.rhf_scf(b, ints, s.nelec, maxits=s.maxits, eps=s.scf_eps, scf_a=s.scf_a)
+    note(f"water SCF: gpu E={res['energy']:.12f} it={res['iterations']}  oracle E={ref.energy:.12f} it={ref.iterations}")
+    assert abs(res["energy"] - ref.energy) < 1e-8
+    assert res["iterations"] == ref.iterations
+
+
+def test_sf6_golden_energy(golden_dir):
+    gold = json.load(open(os.path.join(golden_dir, "sf6_golden.json")))
+    s = systems.read_patin(os.path.join(golden_dir, "sf6_patin.dat"))
+    pm.load_system(s)
+    pm.set_options(tau=1e-14, precision=0)
+    res, P, ev = pm.rhf_scf()
+    note(f"SF6: E={res['energy']:.12f} it={res['iterations']} (golden {gold['energy']:.12f}, {gold['iterations']})")
+    assert abs(res["enuc"] - gold["nuclear_repulsion"]) < 1e-9
+    # the reference's own Rys fits carry ~5e-9 relative error (see oracle
+    # tests); with exact roots the energy sits 5.1e-6 Eh below the golden
+    assert abs(res["energy"] - gold["energy"]) < 2e-5
+    assert abs(res["iterations"] - gold["iterations"]) <= 3

@@ -62,6 +62,7 @@ def lib():
         L.or_hermite_e.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
         L.or_hermite_r.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
         L.or_eri_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.or_eri_block_noscreen.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.or_two_electron_ints.restype = C.c_int64
         L.or_two_electron_ints.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.or_free.argtypes = [C.c_void_p]
@@ -155,6 +156,12 @@ class Basis:
         n = [int(self.nst[x]) for x in (I, J, K, L)]
         out = np.zeros(int(np.prod(n)))
         lib().or_eri_block(self.h, I, J, K, L, out)
+        return out.reshape(n)
+
+    def eri_block_unscreened(self, I, J, K, L) -> np.ndarray:
+        n = [int(self.nst[x]) for x in (I, J, K, L)]
+        out = np.zeros(int(np.prod(n)))
+        lib().or_eri_block_noscreen(self.h, I, J, K, L, out)
         return out.reshape(n)
 
     def eri_tensor(self) -> np.ndarray:

@@ -393,7 +393,7 @@ static double rys_hrr(double abx, double cdx, double G[GMAX][GMAX], int l12, int
  * arranges (twoints.c:249-266,297-314).  out[((a*n2+b)*n3+c)*n4+d], each
  * multiplied by the component normalisation product (twoints.c:356). */
 static void quartet_raw(const OBasis *bs, const OShell *s1, const OShell *s2, const OShell *s3,
-                        const OShell *s4, int cA, int cB, int cC, int cD, double *out)
+                        const OShell *s4, int cA, int cB, int cC, int cD, double *out, int screen)
 {
     const double *a = bs->cen[cA].r, *b = bs->cen[cB].r, *c = bs->cen[cC].r, *d = bs->cen[cD].r;
     const int l1 = s1->lsh, l2 = s2->lsh, l3 = s3->lsh, l4 = s4->lsh;
@@ -454,7 +454,7 @@ static void quartet_raw(const OBasis *bs, const OShell *s1, const OShell *s2, co
                     }
                     double txp = pxp + qxp;
                     double sr = SRTERM * s12 * s34 * abi * cdi / sqrt(txp);
-                    if (sr < 1e-15) continue; /* twoints.c:110-111 */
+                    if (screen && sr < 1e-15) continue; /* twoints.c:110-111 */
                     sr *= c12 * c34;
                     double pq[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
                     double pq2 = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
@@ -508,7 +508,20 @@ static void quartet_raw(const OBasis *bs, const OShell *s1, const OShell *s2, co
 
 /* Full block (IJ|KL) in the caller's shell order, swapping internally so the
  * higher angular momentum leads each pair (twoints.c:249-266,297-314). */
+static void eri_block_impl(const OBasis *bs, int I, int J, int K, int L, double *out, int screen);
+
 void or_eri_block(const OBasis *bs, int I, int J, int K, int L, double *out)
+{
+    eri_block_impl(bs, I, J, K, L, out, 1);
+}
+
+/* The same without the primitive prefactor screen (for Schwarz bounds). */
+void or_eri_block_noscreen(const OBasis *bs, int I, int J, int K, int L, double *out)
+{
+    eri_block_impl(bs, I, J, K, L, out, 0);
+}
+
+static void eri_block_impl(const OBasis *bs, int I, int J, int K, int L, double *out, int screen)
 {
     const OShell *sI = &bs->sh[I], *sJ = &bs->sh[J], *sK = &bs->sh[K], *sL = &bs->sh[L];
     int sw12 = sI->lsh < sJ->lsh, sw34 = sK->lsh < sL->lsh;
@@ -516,7 +529,7 @@ void or_eri_block(const OBasis *bs, int I, int J, int K, int L, double *out)
     const OShell *s3 = sw34 ? sL : sK, *s4 = sw34 ? sK : sL;
     int n1 = s1->nst, n2 = s2->nst, n3 = s3->nst, n4 = s4->nst;
     double *raw = (double *)malloc(sizeof(double) * n1 * n2 * n3 * n4);
-    quartet_raw(bs, s1, s2, s3, s4, s1->cen, s2->cen, s3->cen, s4->cen, raw);
+    quartet_raw(bs, s1, s2, s3, s4, s1->cen, s2->cen, s3->cen, s4->cen, raw, screen);
     int nI = sI->nst, nJ = sJ->nst, nK = sK->nst, nL = sL->nst;
     for (int i = 0; i < nI; ++i)
         for (int j = 0; j < nJ; ++j)

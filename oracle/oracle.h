@@ -66,6 +66,7 @@ void or_hermite_r(double *r, int ltot, double sr, double T, double w, const doub
 
 /* ---- ERIs ---- */
 void or_eri_block(const OBasis *b, int I, int J, int K, int L, double *out);
+void or_eri_block_noscreen(const OBasis *b, int I, int J, int K, int L, double *out);
 int64_t or_two_electron_ints(const OBasis *b, int start, OERINT **out);
 void or_free(void *p);
 void or_eri_count(const OBasis *b, int start, int64_t *counts);

@@ -269,7 +269,7 @@ struct PairGeom {
 // coefficients and the component normalisation of basis.c:309-319.
 // Returns false when the primitive quartet falls under the prefactor screen
 // (its integrals are then passed as exact zeros).
-template <typename R, int LA, int LB, int LC, int LD, typename F>
+template <typename R, int LA, int LB, int LC, int LD, bool SCREEN = true, typename F>
 __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const PrimPair &kp, const PairGeom<R> &bra,
                                                   const PairGeom<R> &ket, const RysTables &tab, R scale,
                                                   F &&f)
@@ -278,8 +278,10 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
     const double p = bp.p, q = kp.p, pq = p + q;
     const double sr0 = SRTERM * bp.s * kp.s / (p * q * sqrt(pq));
     // Screened primitive quartets still run (with a zero prefactor) so that
-    // every lane reaches the cross-lane reductions inside f.
-    const bool keep = sr0 >= PRIM_SCREEN;
+    // every lane reaches the cross-lane reductions inside f.  Bounds (SCREEN
+    // false) must not use the screen: it can zero a diagonal (ab|ab) whose
+    // off-diagonal partners survive it.
+    const bool keep = !SCREEN || sr0 >= PRIM_SCREEN;
     const R sr = keep ? R(sr0 * bp.c * kp.c) * scale : R(0);
     const R PQ[3] = {R(bp.P[0] - kp.P[0]), R(bp.P[1] - kp.P[1]), R(bp.P[2] - kp.P[2])};
     const R rp = R(p), rq = R(q), ipq = R(1.0 / pq);
@@ -371,11 +373,12 @@ struct OneDTable {
     // Returns false when the primitive quartet is under the prefactor screen
     // (the table then holds exact zeros in the z factors).
     __device__ __forceinline__ bool build(const PrimPair &bp, const PrimPair &kp, const PairGeom<R> &bra,
-                                          const PairGeom<R> &ket, const RysTables &tab, R scale)
+                                          const PairGeom<R> &ket, const RysTables &tab, R scale,
+                                          bool screen = true)
     {
         const double p = bp.p, q = kp.p, pq = p + q;
         const double sr0 = SRTERM * bp.s * kp.s / (p * q * sqrt(pq));
-        const bool keep = sr0 >= PRIM_SCREEN;
+        const bool keep = !screen || sr0 >= PRIM_SCREEN;
         const R sr = keep ? R(sr0 * bp.c * kp.c) * scale : R(0);
         const R PQ[3] = {R(bp.P[0] - kp.P[0]), R(bp.P[1] - kp.P[1]), R(bp.P[2] - kp.P[2])};
         const R rp = R(p), rq = R(q), ipq = R(1.0 / pq);

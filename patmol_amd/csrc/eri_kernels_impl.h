@@ -320,7 +320,7 @@ __global__ __launch_bounds__(64) void schwarz_kernel(SchwarzArgs a)
         for (int ik = 0; ik < ref.npp; ++ik) {
             PrimPair kp = a.pp[ref.pp0 + ik];
             if constexpr (!HeavyClass<LA, LB, LA, LB>::value) {
-                primitive_quartet<double, LA, LB, LA, LB>(bp, kp, g, g, a.rys, 1.0,
+                primitive_quartet<double, LA, LB, LA, LB, false>(bp, kp, g, g, a.rys, 1.0,
                                                           [&](auto ia, auto ib_, auto ic, auto id, double v) {
                                                               constexpr int IA = decltype(ia)::value, IB = decltype(ib_)::value;
                                                               if constexpr (IA == decltype(ic)::value && IB == decltype(id)::value)
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(64) void schwarz_kernel(SchwarzArgs a)
                                                           });
             } else {
                 OneDTable<double, LA, LB, LA, LB> tbl;
-                tbl.build(bp, kp, g, g, a.rys, 1.0);
+                tbl.build(bp, kp, g, g, a.rys, 1.0, false);
                 static_for<ncart(LA)>([&](auto ic) {
                     static_for<ncart(LB)>([&](auto id) {
                         constexpr int IC = decltype(ic)::value, ID = decltype(id)::value;
@@ -350,19 +350,14 @@ __global__ __launch_bounds__(64) void schwarz_kernel(SchwarzArgs a)
 template <int LA, int LB, int LC, int LD>
 inline void launch_jk_t(const JKArgs &args, int prec, int nk, hipStream_t s)
 {
-    if (args.nitems <= 0) return;
+    // One exchange density per launch: the two-density instantiation is not
+    // built (UHF runs two single-density passes).
+    if (args.nitems <= 0 || nk != 1) return;
     dim3 grid(args.nitems), block(256);
-    if (prec == 0) {
-        if (nk == 1)
-            hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, double, 1>), grid, block, 0, s, args);
-        else
-            hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, double, 2>), grid, block, 0, s, args);
-    } else {
-        if (nk == 1)
-            hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, float, 1>), grid, block, 0, s, args);
-        else
-            hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, float, 2>), grid, block, 0, s, args);
-    }
+    if (prec == 0)
+        hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, double, 1>), grid, block, 0, s, args);
+    else
+        hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, float, 1>), grid, block, 0, s, args);
 }
 
 template <int LA, int LB, int LC, int LD>

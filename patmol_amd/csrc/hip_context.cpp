@@ -687,10 +687,12 @@ void uhf_form_gmatrix(const ERIFS *e, const double *PA, const double *PB, double
     hipLaunchKernelGGL(add_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, x.d_D.p, x.d_D2.p, dDJ.p, NN);
     double dmax = 0.0;
     if (g_prec_mode == 1) dmax = std::max(device_absmax(x, x.d_P.p, no2, 0), device_absmax(x, dPB.p, no2, 0));
-    if (run_jk(x, dDJ.p, x.d_D.p, x.d_D2.p, 2, dmax, 0)) return;
-    // GA += J[PA+PB] - K[PA], GB += J[PA+PB] - K[PB]  (J = J' + J'^T)
+    // Two single-density passes: (J[PA+PB], K[PA]) then (J[PA+PB], K[PB]);
+    // GA += J - K[PA], GB += J - K[PB] with J = J' + J'^T.
+    if (run_jk(x, dDJ.p, x.d_D.p, nullptr, 1, dmax, 0)) return;
     hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, 0, x.d_J.p, x.d_K0.p, x.d_G.p, x.N, 1.0);
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, 0, x.d_J.p, x.d_K1.p, x.d_G2.p, x.N, 1.0);
+    if (run_jk(x, dDJ.p, x.d_D2.p, nullptr, 1, dmax, 0)) return;
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, 0, x.d_J.p, x.d_K0.p, x.d_G2.p, x.N, 1.0);
     std::vector<double> ga(no2), gb(no2);
     if (hipMemcpy(ga.data(), x.d_G.p, sizeof(double) * no2, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(gb.data(), x.d_G2.p, sizeof(double) * no2, hipMemcpyDeviceToHost) != hipSuccess) {

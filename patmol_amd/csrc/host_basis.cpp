@@ -30,6 +30,7 @@ struct BasisState {
     std::vector<std::vector<double>> al, co;
     int ***lxyz = nullptr;
     double **normFact = nullptr;
+    int tab_l = -1;  // highest l of the allocated lxyz/normFact tables
 } B;
 
 const double kDfact[] = {1., 1., 3., 15., 105., 945., 10395.};
@@ -50,7 +51,7 @@ void normalize_shell(Shell &sh)
 void free_tables()
 {
     if (B.lxyz) {
-        for (int l = 0; l <= B.maxlsh; ++l) {
+        for (int l = 0; l <= B.tab_l; ++l) {
             free(B.lxyz[l][0]);
             free(B.lxyz[l]);
             free(B.normFact[l]);
@@ -60,12 +61,14 @@ void free_tables()
     }
     B.lxyz = nullptr;
     B.normFact = nullptr;
+    B.tab_l = -1;
 }
 
 void build_tables()
 {
     free_tables();
     B.maxlst = (B.maxlsh + 1) * (B.maxlsh + 2) / 2;
+    B.tab_l = B.maxlsh;
     B.lxyz = (int ***)malloc(sizeof(int **) * (B.maxlsh + 1));
     B.normFact = (double **)malloc(sizeof(double *) * (B.maxlsh + 1));
     for (int l = 0; l <= B.maxlsh; ++l) {

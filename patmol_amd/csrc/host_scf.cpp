@@ -317,9 +317,12 @@ extern "C" int patmol_rhf_scf(double *P_out, double *evals_out, PatmolScfResult 
         if (first) {
             first = false;
             iteration = 0;
-            continue;
+        } else {
+            done = ret;
         }
-        done = ret;
+        // iteration log as rhfscf.c:89-95
+        fprintf(stderr, "ITERATION = %12d EDIFF= %lg\n  energy = %.12f\n", iteration, ediff, enrg + enuc);
+        if (iteration == 0) continue;
         if (done || iteration >= maxits) break;
     }
     clean_ERIFS(mints);
@@ -412,9 +415,11 @@ extern "C" int patmol_uhf_scf(double *PA_out, double *PB_out, PatmolScfResult *r
         if (first) {
             first = false;
             iteration = 0;
-            continue;
+        } else {
+            done = ret;
         }
-        done = ret;
+        fprintf(stderr, "ITERATION = %12d EDIFF= %lg\n  energy = %.12f\n", iteration, ediff, enrg + enuc);
+        if (iteration == 0) continue;
         if (done || iteration >= maxits) break;
     }
     clean_ERIFS(mints);

@@ -78,6 +78,9 @@ typedef struct {
     int64_t work_items_fp32;       /* of which run in FP32 at the last G build */
 } PatmolHipStats;
 int patmol_hip_stats(const ERIFS *tfiles, PatmolHipStats *out);
+/* Kept shell quartets and algorithmic FLOPs per class (36 entries each,
+ * indexed bra_pc*6 + ket_pc, pair class pc = l(l+1)/2 + l' with l >= l'). */
+int patmol_hip_class_stats(const ERIFS *tfiles, int64_t *quartets36, double *flops36);
 
 /* Schwarz bound of every shell pair, indexed by original shell indices
  * (I >= J) at I*(I+1)/2 + J. */

@@ -102,6 +102,8 @@ def lib():
         G.patmol_hip_rhf_gmatrix_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         G.patmol_hip_stats.restype = C.c_int
         G.patmol_hip_stats.argtypes = [C.c_void_p, C.POINTER(HipStats)]
+        G.patmol_hip_class_stats.restype = C.c_int
+        G.patmol_hip_class_stats.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.int64), _dp]
         G.patmol_hip_pair_schwarz.restype = C.c_int
         G.patmol_hip_pair_schwarz.argtypes = [C.c_void_p, _dp]
         G.patmol_hip_work_items.restype = C.c_int64
@@ -127,7 +129,8 @@ def exported_symbols():
     return ["init_ERIFS", "clean_ERIFS", "two_electron_ints", "rhf_form_gmatrix", "uhf_form_gmatrix",
             "dpm_ints", "patmol_hip_last_error", "patmol_hip_error_string", "patmol_hip_set_strict",
             "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision",
-            "patmol_hip_rhf_gmatrix_device", "patmol_hip_stats", "patmol_hip_pair_schwarz",
+            "patmol_hip_rhf_gmatrix_device", "patmol_hip_stats", "patmol_hip_class_stats",
+            "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
             "patmol_hip_set_timing"]
 
@@ -211,6 +214,15 @@ class EriStore:
         if L.patmol_hip_stats(self.h, C.byref(s)):
             _check("patmol_hip_stats")
         return s.as_dict()
+
+    def class_stats(self):
+        """(kept quartets, algorithmic FLOPs) per class, arrays of 36."""
+        _, L = lib()
+        q = np.zeros(36, np.int64)
+        f = np.zeros(36)
+        if L.patmol_hip_class_stats(self.h, q, f):
+            _check("patmol_hip_class_stats")
+        return q, f
 
     def pair_schwarz(self) -> np.ndarray:
         _, L = lib()

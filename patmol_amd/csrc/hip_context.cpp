@@ -172,6 +172,7 @@ struct Ctx {
     double last_ms = 0.0;
     double class_ms[36] = {0};
     int64_t items_fp32 = 0;
+    int64_t prim_kept = 0;
 };
 
 static std::unordered_map<const ERIFS *, Ctx *> g_ctx;
@@ -794,6 +795,15 @@ static double class_flops(int la, int lb, int lc, int ld)
     return F;
 }
 
+static int class_stats_impl(Ctx &c, int64_t *q36, double *f36);
+
+int patmol_hip_class_stats(const ERIFS *e, int64_t *quartets36, double *flops36)
+{
+    Ctx *cp = ctx_of(e);
+    if (!cp || !cp->built) return fail(PATMOL_HIP_ESTATE, "no tables");
+    return class_stats_impl(*cp, quartets36, flops36);
+}
+
 int patmol_hip_stats(const ERIFS *e, PatmolHipStats *out)
 {
     Ctx *cp = ctx_of(e);
@@ -809,13 +819,32 @@ int patmol_hip_stats(const ERIFS *e, PatmolHipStats *out)
     out->shell_quartets_total = np * (np + 1) / 2;
     out->work_items = (int64_t)c.items.size();
     out->work_items_fp32 = c.items_fp32;
+    int64_t q36[36];
+    double f36[36];
+    class_stats_impl(c, q36, f36);
+    int64_t kept = 0;
+    double flops = 0;
+    for (int i = 0; i < 36; ++i) {
+        kept += q36[i];
+        flops += f36[i];
+    }
+    out->shell_quartets_kept = kept;
+    out->flops_per_build = flops;
+    out->prim_quartets_kept = c.prim_kept;
+    return 0;
+}
+
+static int class_stats_impl(Ctx &c, int64_t *q36, double *f36)
+{
     double fl[6][6];
     int pcl[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
     for (int a = 0; a < 6; ++a)
         for (int b = 0; b < 6; ++b) fl[a][b] = class_flops(pcl[a][0], pcl[a][1], pcl[b][0], pcl[b][1]);
-    // exact count over the kernels' own enumeration
-    int64_t kept = 0, pkept = 0;
-    double flops = 0;
+    for (int i = 0; i < 36; ++i) {
+        q36[i] = 0;
+        f36[i] = 0.0;
+    }
+    int64_t pkept_all = 0;
     for (const ClassRange &r : c.ranges) {
         for (int it = r.first; it < r.first + r.count; ++it) {
             int4 t = c.items[it];
@@ -845,14 +874,12 @@ int patmol_hip_stats(const ERIFS *e, PatmolHipStats *out)
                     }
                 }
             }
-            kept += k;
-            pkept += pk;
-            flops += pk * fl[r.bra_pc][r.ket_pc];
+            q36[r.bra_pc * 6 + r.ket_pc] += k;
+            f36[r.bra_pc * 6 + r.ket_pc] += pk * fl[r.bra_pc][r.ket_pc];
+            pkept_all += pk;
         }
     }
-    out->shell_quartets_kept = kept;
-    out->prim_quartets_kept = pkept;
-    out->flops_per_build = flops;
+    c.prim_kept = pkept_all;
     return 0;
 }
 

@@ -39,8 +39,17 @@ __device__ __forceinline__ void atomic_add(double *p, double v)
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
 
+// Classes with many Cartesian components need more than 256 registers per
+// lane; they run one wavefront per SIMD so the compiler may use the full
+// register file (arch + accumulation registers) instead of scratch memory.
+template <int LA, int LB, int LC, int LD>
+constexpr int jk_waves_per_eu()
+{
+    return ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD) >= 81 ? 1 : 2;
+}
+
 template <int LA, int LB, int LC, int LD, typename R, int NK>
-__global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
+__global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_kernel(JKArgs a)
 {
     constexpr int TI = tile_of(LA), TJ = tile_of(LB), TK = tile_of(LC), TL = tile_of(LD);
     constexpr int KLN = TK * TL, JSUB = 64 / KLN;
@@ -243,6 +252,238 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// J/K kernel with bra shell pairs fixed per lane (all classes except the
+// largest).  Lane layout lane = (ks*TI + ii)*TJ + jj: lane holds bra pair
+// (I,J) = (fI+ii, fJ+jj) for the whole work item, and KSUB = 64/(TI*TJ)
+// ket pairs are processed per step.  A wavefront owns ket shells K = wave,
+// wave+4, ... and steps over the L shells of each K.  Hence:
+//   J'(ab)          lane-private for the whole item (flushed once),
+//   K'(ac), K'(bc)  accumulated over all L of one K (flushed per K),
+//   J'(cd), K'(ad), K'(bd)  flushed per step,
+// each flush being a butterfly over the lane bits its target does not
+// depend on, then one FP64 atomic per target from a leader lane.  Since the
+// bra class is the larger one, the per-step targets are the small ones.
+// ---------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+constexpr bool jk2_preload()
+{
+    return ncart(LA) * ncart(LB) + ncart(LC) * ncart(LD) +
+               (ncart(LA) + ncart(LB)) * (ncart(LC) + ncart(LD)) <= 64;
+}
+
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_kernel2(JKArgs a)
+{
+    constexpr int TI = tile_of(LA), TJ = tile_of(LB), TK = tile_of(LC), TL = tile_of(LD);
+    constexpr int IJN = TI * TJ, KSUB = 64 / IJN;
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int JBITS = ilog2(TJ), IBITS = ilog2(TI), SBITS = ilog2(KSUB);
+    constexpr int MASK_J = (1 << JBITS) - 1;
+    constexpr int MASK_I = ((1 << IBITS) - 1) << JBITS;
+    constexpr int MASK_S = ((1 << SBITS) - 1) << (JBITS + IBITS);
+    constexpr int NLSTEP = (TL + KSUB - 1) / KSUB;
+    constexpr bool PRE = jk2_preload<LA, LB, LC, LD>();
+    static_assert(IJN * KSUB == 64, "bra tile pair must divide a wavefront");
+
+    const int item = blockIdx.x;
+    if (item >= a.nitems) return;
+    const int4 it = a.items[item];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int jj = lane & (TJ - 1), ii = (lane >> JBITS) & (TI - 1), ks = lane >> (JBITS + IBITS);
+    const int fI = a.tile_first[it.x], nI = a.tile_size[it.x];
+    const int fJ = a.tile_first[it.y], nJ = a.tile_size[it.y];
+    const int fK = a.tile_first[it.z], nK = a.tile_size[it.z];
+    const int fL = a.tile_first[it.w], nL = a.tile_size[it.w];
+    const bool sameBra = it.x == it.y, sameKet = it.z == it.w, samePair = it.x == it.z && it.y == it.w;
+    const int N = a.N;
+    const double *__restrict__ D = a.DJ;     // J and K use one density here
+    const double *__restrict__ DK = a.DK0;
+
+    // ---- bra pair (fixed per lane) ----
+    const bool iexist = ii < nI, jexist = jj < nJ;
+    const int I = fI + (iexist ? ii : 0), J = fJ + (jexist ? jj : 0);
+    const bool bvalid = iexist && jexist && (!sameBra || I >= J);
+    const int Is = bvalid ? I : fI, Js = bvalid ? J : fJ;
+    const int Ip = Is >= Js ? Is : Js, Jp = Is >= Js ? Js : Is;
+    const int pb = Ip * (Ip + 1) / 2 + Jp;
+    const double qbra = a.pair_q[pb];
+    const PairRef bref = a.pairs[pb];
+    PairGeom<R> bg;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        bg.A[x] = R(a.shell_xyz[3 * Is + x]);
+        bg.AB[x] = R(a.shell_xyz[3 * Is + x] - a.shell_xyz[3 * Js + x]);
+    }
+    const int offI = a.shell_off[I], offJ = a.shell_off[J];
+    const int origI = a.shell_orig[Is], origJ = a.shell_orig[Js];
+
+    double jab[NA * NB];
+#pragma unroll
+    for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
+    double Dab[PRE ? NA * NB : 1];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int x = 0; x < NA; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y) Dab[x * NB + y] = D[(size_t)(offI + x) * N + offJ + y];
+    }
+
+    for (int kq = wave; kq < TK; kq += 4) {
+        if (kq >= nK) break;  // uniform
+        const int K = fK + kq;
+        const int offK = a.shell_off[K];
+        double kac[NA * NC], kbc[NB * NC];
+#pragma unroll
+        for (int i = 0; i < NA * NC; ++i) kac[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB * NC; ++i) kbc[i] = 0.0;
+        for (int lc = 0; lc < NLSTEP; ++lc) {
+            const int lq = lc * KSUB + ks;
+            const bool lexist = lq < nL;
+            const int L = fL + (lexist ? lq : 0);
+            const bool kvalid = lexist && (!sameKet || K >= L);
+            bool valid = bvalid && kvalid;
+            if (samePair) valid = valid && (I > K || (I == K && J >= L));
+            const int Ls = kvalid ? L : (sameKet ? K : fL);  // a valid partner of K for loads
+            const int Kp = K >= Ls ? K : Ls, Lp = K >= Ls ? Ls : K;
+            const int pk = Kp * (Kp + 1) / 2 + Lp;
+            const double qket = a.pair_q[pk];
+            valid = valid && (qbra * qket >= a.tau);
+            if (a.start_shell > 0) {
+                int mo = max(max(origI, origJ), max(a.shell_orig[K], a.shell_orig[Ls]));
+                valid = valid && (mo >= a.start_shell);
+            }
+            if (!__any(valid)) continue;  // uniform
+            double wdeg = 1.0;
+            if (I == J) wdeg *= 0.5;
+            if (K == L) wdeg *= 0.5;
+            if (I == K && J == L) wdeg *= 0.5;
+            const R scale = valid ? R(wdeg) : R(0);
+            const PairRef kref = a.pairs[pk];
+            PairGeom<R> kg;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                kg.A[x] = R(a.shell_xyz[3 * K + x]);
+                kg.AB[x] = R(a.shell_xyz[3 * K + x] - a.shell_xyz[3 * Ls + x]);
+            }
+            const int offL = a.shell_off[L];
+            double jcd[NC * ND], kad[NA * ND], kbd[NB * ND];
+#pragma unroll
+            for (int i = 0; i < NC * ND; ++i) jcd[i] = 0.0;
+#pragma unroll
+            for (int i = 0; i < NA * ND; ++i) kad[i] = 0.0;
+#pragma unroll
+            for (int i = 0; i < NB * ND; ++i) kbd[i] = 0.0;
+            // density blocks of this step (rows uniform across the ket
+            // group, columns along the lane's bra shells: D is symmetric)
+            double Dcd[PRE ? NC * ND : 1], Dbd[PRE ? NB * ND : 1], Dbc[PRE ? NB * NC : 1];
+            double Dad[PRE ? NA * ND : 1], Dac[PRE ? NA * NC : 1];
+            if constexpr (PRE) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) Dcd[c * ND + d] = D[(size_t)(offK + c) * N + offL + d];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) Dbd[b * ND + d] = DK[(size_t)(offL + d) * N + offJ + b];
+#pragma unroll
+                    for (int x = 0; x < NA; ++x) Dad[x * ND + d] = DK[(size_t)(offL + d) * N + offI + x];
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) Dbc[b * NC + c] = DK[(size_t)(offK + c) * N + offJ + b];
+#pragma unroll
+                    for (int x = 0; x < NA; ++x) Dac[x * NC + c] = DK[(size_t)(offK + c) * N + offI + x];
+                }
+            }
+            int nbp = bref.npp, nkp = kref.npp;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                nbp = max(nbp, __shfl_xor(nbp, 1 << b, 64));
+                nkp = max(nkp, __shfl_xor(nkp, 1 << b, 64));
+            }
+            for (int ib = 0; ib < nbp; ++ib) {
+                const PrimPair bp = a.pp[bref.pp0 + min(ib, bref.npp - 1)];
+                for (int ik = 0; ik < nkp; ++ik) {
+                    const PrimPair kp = a.pp[kref.pp0 + min(ik, kref.npp - 1)];
+                    const R sc = (ib < bref.npp && ik < kref.npp) ? scale : R(0);
+                    primitive_quartet<R, LA, LB, LC, LD>(
+                        bp, kp, bg, kg, a.rys, sc, [&](auto ia, auto ib_, auto ic, auto id, R vr) {
+                            constexpr int A_ = decltype(ia)::value, B_ = decltype(ib_)::value;
+                            constexpr int C_ = decltype(ic)::value, D_ = decltype(id)::value;
+                            const double v = double(vr);
+                            if constexpr (PRE) {
+                                jab[A_ * NB + B_] += v * Dcd[C_ * ND + D_];
+                                jcd[C_ * ND + D_] += v * Dab[A_ * NB + B_];
+                                kac[A_ * NC + C_] += v * Dbd[B_ * ND + D_];
+                                kad[A_ * ND + D_] += v * Dbc[B_ * NC + C_];
+                                kbc[B_ * NC + C_] += v * Dad[A_ * ND + D_];
+                                kbd[B_ * ND + D_] += v * Dac[A_ * NC + C_];
+                            } else {
+                                const size_t ra = offI + A_, rb = offJ + B_, rc = offK + C_, rd = offL + D_;
+                                jab[A_ * NB + B_] += v * D[rc * N + rd];
+                                jcd[C_ * ND + D_] += v * D[ra * N + rb];
+                                kac[A_ * NC + C_] += v * DK[rd * N + rb];
+                                kad[A_ * ND + D_] += v * DK[rc * N + rb];
+                                kbc[B_ * NC + C_] += v * DK[rd * N + ra];
+                                kbd[B_ * ND + D_] += v * DK[rc * N + ra];
+                            }
+                        });
+                }
+            }
+            // per-step flushes: J'(cd), K'(ad), K'(bd)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    double t = lane_reduce<MASK_I | MASK_J>(jcd[c * ND + d]);
+                    if ((lane & (MASK_I | MASK_J)) == 0 && lexist)
+                        atomic_add(&a.J[(size_t)(offK + c) * N + offL + d], 2.0 * t);
+                }
+#pragma unroll
+            for (int x = 0; x < NA; ++x)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    double t = lane_reduce<MASK_J>(kad[x * ND + d]);
+                    if ((lane & MASK_J) == 0 && iexist && lexist) atomic_add(&a.K0[(size_t)(offI + x) * N + offL + d], t);
+                }
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    double t = lane_reduce<MASK_I>(kbd[b * ND + d]);
+                    if ((lane & MASK_I) == 0 && jexist && lexist) atomic_add(&a.K0[(size_t)(offJ + b) * N + offL + d], t);
+                }
+        }
+        // per-K flushes: K'(ac), K'(bc)
+#pragma unroll
+        for (int x = 0; x < NA; ++x)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                double t = lane_reduce<MASK_J | MASK_S>(kac[x * NC + c]);
+                if ((lane & (MASK_J | MASK_S)) == 0 && iexist) atomic_add(&a.K0[(size_t)(offI + x) * N + offK + c], t);
+            }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                double t = lane_reduce<MASK_I | MASK_S>(kbc[b * NC + c]);
+                if ((lane & (MASK_I | MASK_S)) == 0 && jexist) atomic_add(&a.K0[(size_t)(offJ + b) * N + offK + c], t);
+            }
+    }
+    // per-item flush: J'(ab)
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double t = lane_reduce<MASK_S>(jab[x * NB + b]);
+            if ((lane & MASK_S) == 0 && iexist && jexist) atomic_add(&a.J[(size_t)(offI + x) * N + offJ + b], 2.0 * t);
+        }
+}
+
+// ---------------------------------------------------------------------------
 // ERI export: all Cartesian components of listed shell quartets (tests).
 // ---------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD>
@@ -354,10 +595,17 @@ inline void launch_jk_t(const JKArgs &args, int prec, int nk, hipStream_t s)
     // built (UHF runs two single-density passes).
     if (args.nitems <= 0 || nk != 1) return;
     dim3 grid(args.nitems), block(256);
-    if (prec == 0)
-        hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, double, 1>), grid, block, 0, s, args);
-    else
-        hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, float, 1>), grid, block, 0, s, args);
+    if constexpr (HeavyClass<LA, LB, LC, LD>::value) {
+        if (prec == 0)
+            hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, double, 1>), grid, block, 0, s, args);
+        else
+            hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, float, 1>), grid, block, 0, s, args);
+    } else {
+        if (prec == 0)
+            hipLaunchKernelGGL((jk_kernel2<LA, LB, LC, LD, double>), grid, block, 0, s, args);
+        else
+            hipLaunchKernelGGL((jk_kernel2<LA, LB, LC, LD, float>), grid, block, 0, s, args);
+    }
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -169,6 +169,10 @@ struct Ctx {
     DevBuf<double> d_P, d_D, d_D2, d_J, d_K0, d_K1, d_G, d_G2, d_dmax;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> cls_ev;
+    // side streams for concurrent class launches (default priority)
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;
+    hipEvent_t fork_ev = nullptr;
     double last_ms = 0.0;
     double class_ms[36] = {0};
     int64_t items_fp32 = 0;
@@ -429,8 +433,26 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1
         HIPCHK(hipEventRecord(c.ev0, s));
     }
     c.items_fp32 = 0;
+    // Classes are independent: spread them over side streams so small grids
+    // overlap.  With timing on, launches stay on the caller's stream so each
+    // class's events bracket only its own kernels.
+    const int nside = g_timing ? 0 : 4;
+    if (nside && c.side.empty()) {
+        c.side.resize(nside);
+        c.side_ev.resize(nside);
+        for (int i = 0; i < nside; ++i) {
+            HIPCHK(hipStreamCreateWithFlags(&c.side[i], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&c.side_ev[i], hipEventDisableTiming));
+        }
+        HIPCHK(hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming));
+    }
+    if (nside) {
+        HIPCHK(hipEventRecord(c.fork_ev, s));
+        for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(c.side[i], c.fork_ev, 0));
+    }
     int ci = 0;
     for (const ClassRange &r : c.ranges) {
+        hipStream_t ls = nside ? c.side[ci % nside] : s;
         // items are sorted by bound within a class: FP32 items form a suffix
         int n64 = r.count;
         if (g_prec_mode == 1 && dmax > 0.0) {
@@ -462,17 +484,23 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1
         if (n64 > 0) {
             a.items = c.d_items.p + r.first;
             a.nitems = n64;
-            if (launch_jk(r.bra_pc, r.ket_pc, a, 0, nk, s)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 0, nk, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
         if (r.count > n64) {
             a.items = c.d_items.p + r.first + n64;
             a.nitems = r.count - n64;
             c.items_fp32 += r.count - n64;
-            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, nk, s)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, nk, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
         if (g_timing && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci + 1], s));
         HIPCHK(hipGetLastError());
         ++ci;
+    }
+    if (nside) {
+        for (int i = 0; i < nside; ++i) {
+            HIPCHK(hipEventRecord(c.side_ev[i], c.side[i]));
+            HIPCHK(hipStreamWaitEvent(s, c.side_ev[i], 0));
+        }
     }
     if (g_timing) HIPCHK(hipEventRecord(c.ev1, s));
     return 0;
@@ -603,6 +631,9 @@ void clean_ERIFS(ERIFS *e)
         c->d_pairs.release();
         c->d_pp.release();
         c->d_items.release();
+        for (auto st : c->side) (void)hipStreamDestroy(st);
+        for (auto ev : c->side_ev) (void)hipEventDestroy(ev);
+        if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
         if (c->ev0) {
             (void)hipEventDestroy(c->ev0);
             (void)hipEventDestroy(c->ev1);

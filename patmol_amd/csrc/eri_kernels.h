@@ -7,6 +7,8 @@ namespace pm {
 
 constexpr int PM_LMAX = 2;
 constexpr int PM_NPC = (PM_LMAX + 1) * (PM_LMAX + 2) / 2;  // pair classes
+// private J'/K' copies, one per XCD (MI355X has 8; a power of two)
+constexpr int PM_XCD_SLOTS = 8;
 
 // Shells per tile for angular momentum l (a power of two; the ket tile pair
 // TK*TL must divide 64).
@@ -25,7 +27,7 @@ struct JKArgs {
     const PrimPair *pp;
     const double *DJ;          // density contracted into J (full N x N)
     const double *DK0, *DK1;   // densities contracted into K
-    double *J, *K0, *K1;       // accumulators J', K' (full N x N)
+    double *J, *K0, *K1;       // accumulators J', K': PM_XCD_SLOTS copies of N x N each
     int N;
     double tau;                // Schwarz threshold on Q_IJ Q_KL
     int start_shell;           // keep quartets whose largest original shell index >= start_shell

@@ -32,24 +32,42 @@ __device__ __forceinline__ double lane_reduce(double v)
     return v;
 }
 
+// J'/K' accumulate in one private copy per XCD (the XCD's L2 is the
+// coherence point for its own workgroups), so the atomics are workgroup
+// scope and stay in L2; the copies are summed after the kernels finish.
 __device__ __forceinline__ void atomic_add(double *p, double v)
 {
-    if (v != 0.0) unsafeAtomicAdd(p, v);
+    if (v != 0.0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// XCD this workgroup runs on (hardware register XCC_ID).
+__device__ __forceinline__ int xcd_slot()
+{
+    int x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+    return x & (PM_XCD_SLOTS - 1);
 }
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
 
-// Classes with many Cartesian components need more than 256 registers per
-// lane; they run one wavefront per SIMD so the compiler may use the full
-// register file (arch + accumulation registers) instead of scratch memory.
+// Register budget per class: the requested waves per SIMD bound the
+// registers per lane (4 -> 128, 2 -> 256, 1 -> 512 with the accumulation
+// registers), chosen so the larger classes do not spill to scratch.
 template <int LA, int LB, int LC, int LD>
 constexpr int jk_waves_per_eu()
 {
-    return ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD) >= 81 ? 1 : 2;
+    // live doubles: J/K accumulators, preloaded density blocks and the
+    // 2-D recursion arrays of all roots
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int acc = NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
+    constexpr int pre = NA * NB + NC * ND + (NA + NB) * (NC + ND);
+    constexpr int g = ((LA + LB + LC + LD) / 2 + 1) * 3 * (LA + LB + 1) * (LC + LD + 1);
+    constexpr int live = acc + (pre <= 64 ? pre : 0) + g;
+    return live <= 40 ? 4 : (live <= 96 ? 2 : 1);
 }
 
 template <int LA, int LB, int LC, int LD, typename R, int NK>
-__global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_kernel(JKArgs a)
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(jk_waves_per_eu<LA, LB, LC, LD>()))) void jk_kernel(JKArgs a)
 {
     constexpr int TI = tile_of(LA), TJ = tile_of(LB), TK = tile_of(LC), TL = tile_of(LD);
     constexpr int KLN = TK * TL, JSUB = 64 / KLN;
@@ -72,6 +90,10 @@ __global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_k
     const int fL = a.tile_first[it.w], nL = a.tile_size[it.w];
     const bool sameBra = it.x == it.y, sameKet = it.z == it.w, samePair = it.x == it.z && it.y == it.w;
     const int N = a.N;
+    const size_t xoff = (size_t)xcd_slot() * N * N;
+    double *const Jx = a.J + xoff;
+    double *const K0x = a.K0 + xoff;
+    double *const K1x = a.K1 + xoff;
 
     // ---- ket (fixed per lane) ----
     const bool kexist = kk < nK, lexist = ll < nL;
@@ -175,11 +197,11 @@ __global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_k
                             // end of the (a,b) block
                             double t = lane_reduce<MASK_K | MASK_L>(jab);
                             if ((lane & (MASK_K | MASK_L)) == 0 && jexist)
-                                atomic_add(&a.J[(size_t)ra * N + rb], 2.0 * t);
+                                atomic_add(&Jx[(size_t)ra * N + rb], 2.0 * t);
                             jab = 0.0;
 #pragma unroll
                             for (int s = 0; s < NK; ++s) {
-                                double *Kt = s ? a.K1 : a.K0;
+                                double *Kt = s ? K1x : K0x;
 #pragma unroll
                                 for (int c = 0; c < NC; ++c) {
                                     double u = lane_reduce<MASK_L>(kbc[s][c]);
@@ -247,7 +269,7 @@ __global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_k
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             double t = lane_reduce<MASK_J>(jcd[c * ND + d]);
-            if ((lane & MASK_J) == 0 && kexist && lexist) atomic_add(&a.J[(size_t)(offK + c) * N + offL + d], 2.0 * t);
+            if ((lane & MASK_J) == 0 && kexist && lexist) atomic_add(&Jx[(size_t)(offK + c) * N + offL + d], 2.0 * t);
         }
 }
 
@@ -272,7 +294,7 @@ constexpr bool jk2_preload()
 }
 
 template <int LA, int LB, int LC, int LD, typename R>
-__global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_kernel2(JKArgs a)
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(jk_waves_per_eu<LA, LB, LC, LD>()))) void jk_kernel2(JKArgs a)
 {
     constexpr int TI = tile_of(LA), TJ = tile_of(LB), TK = tile_of(LC), TL = tile_of(LD);
     constexpr int IJN = TI * TJ, KSUB = 64 / IJN;
@@ -298,6 +320,9 @@ __global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_k
     const int N = a.N;
     const double *__restrict__ D = a.DJ;     // J and K use one density here
     const double *__restrict__ DK = a.DK0;
+    const size_t xoff = (size_t)xcd_slot() * N * N;
+    double *const Jx = a.J + xoff;
+    double *const K0x = a.K0 + xoff;
 
     // ---- bra pair (fixed per lane) ----
     const bool iexist = ii < nI, jexist = jj < nJ;
@@ -440,21 +465,21 @@ __global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_k
                 for (int d = 0; d < ND; ++d) {
                     double t = lane_reduce<MASK_I | MASK_J>(jcd[c * ND + d]);
                     if ((lane & (MASK_I | MASK_J)) == 0 && lexist)
-                        atomic_add(&a.J[(size_t)(offK + c) * N + offL + d], 2.0 * t);
+                        atomic_add(&Jx[(size_t)(offK + c) * N + offL + d], 2.0 * t);
                 }
 #pragma unroll
             for (int x = 0; x < NA; ++x)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     double t = lane_reduce<MASK_J>(kad[x * ND + d]);
-                    if ((lane & MASK_J) == 0 && iexist && lexist) atomic_add(&a.K0[(size_t)(offI + x) * N + offL + d], t);
+                    if ((lane & MASK_J) == 0 && iexist && lexist) atomic_add(&K0x[(size_t)(offI + x) * N + offL + d], t);
                 }
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     double t = lane_reduce<MASK_I>(kbd[b * ND + d]);
-                    if ((lane & MASK_I) == 0 && jexist && lexist) atomic_add(&a.K0[(size_t)(offJ + b) * N + offL + d], t);
+                    if ((lane & MASK_I) == 0 && jexist && lexist) atomic_add(&K0x[(size_t)(offJ + b) * N + offL + d], t);
                 }
         }
         // per-K flushes: K'(ac), K'(bc)
@@ -463,14 +488,14 @@ __global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_k
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 double t = lane_reduce<MASK_J | MASK_S>(kac[x * NC + c]);
-                if ((lane & (MASK_J | MASK_S)) == 0 && iexist) atomic_add(&a.K0[(size_t)(offI + x) * N + offK + c], t);
+                if ((lane & (MASK_J | MASK_S)) == 0 && iexist) atomic_add(&K0x[(size_t)(offI + x) * N + offK + c], t);
             }
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 double t = lane_reduce<MASK_I | MASK_S>(kbc[b * NC + c]);
-                if ((lane & (MASK_I | MASK_S)) == 0 && jexist) atomic_add(&a.K0[(size_t)(offJ + b) * N + offK + c], t);
+                if ((lane & (MASK_I | MASK_S)) == 0 && jexist) atomic_add(&K0x[(size_t)(offJ + b) * N + offK + c], t);
             }
     }
     // per-item flush: J'(ab)
@@ -479,7 +504,7 @@ __global__ __launch_bounds__(256, (jk_waves_per_eu<LA, LB, LC, LD>())) void jk_k
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             double t = lane_reduce<MASK_S>(jab[x * NB + b]);
-            if ((lane & MASK_S) == 0 && iexist && jexist) atomic_add(&a.J[(size_t)(offI + x) * N + offJ + b], 2.0 * t);
+            if ((lane & MASK_S) == 0 && iexist && jexist) atomic_add(&Jx[(size_t)(offI + x) * N + offJ + b], 2.0 * t);
         }
 }
 

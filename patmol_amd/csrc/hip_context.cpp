@@ -384,7 +384,8 @@ __global__ void add_kernel(const double *__restrict__ a, const double *__restric
     if (i < n) o[i] = a[i] + b[i];
 }
 
-// G[ij] += cj (J'ij + J'ji) - (K'ij + K'ji), packed i >= j
+// G[ij] += cj (J'ij + J'ji) - (K'ij + K'ji), packed i >= j, with J' and K'
+// the sums of the per-XCD copies
 __global__ void finalize_kernel(const double *__restrict__ J, const double *__restrict__ K, double *__restrict__ G,
                                 int N, double cj)
 {
@@ -395,8 +396,12 @@ __global__ void finalize_kernel(const double *__restrict__ J, const double *__re
     while ((int64_t)(i + 1) * (i + 2) / 2 <= ij) ++i;
     while ((int64_t)i * (i + 1) / 2 > ij) --i;
     int j = (int)(ij - (int64_t)i * (i + 1) / 2);
-    double jj = J[(int64_t)i * N + j] + J[(int64_t)j * N + i];
-    double kk = K[(int64_t)i * N + j] + K[(int64_t)j * N + i];
+    const int64_t NN = (int64_t)N * N, a = (int64_t)i * N + j, b = (int64_t)j * N + i;
+    double jj = 0.0, kk = 0.0;
+    for (int x = 0; x < PM_XCD_SLOTS; ++x) {
+        jj += J[x * NN + a] + J[x * NN + b];
+        kk += K[x * NN + a] + K[x * NN + b];
+    }
     G[ij] += cj * jj - kk;
 }
 
@@ -419,7 +424,7 @@ __global__ void absmax_kernel(const double *__restrict__ p, int64_t n, double *o
 // (passed as DJ).
 static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1, int nk, double dmax, hipStream_t s)
 {
-    const int64_t NN = (int64_t)c.N * c.N;
+    const int64_t NN = (int64_t)c.N * c.N * PM_XCD_SLOTS;
     HIPCHK(hipMemsetAsync(c.d_J.p, 0, sizeof(double) * NN, s));
     HIPCHK(hipMemsetAsync(c.d_K0.p, 0, sizeof(double) * NN, s));
     if (nk == 2) HIPCHK(hipMemsetAsync(c.d_K1.p, 0, sizeof(double) * NN, s));
@@ -528,10 +533,10 @@ static int collect_timing(Ctx &c)
 static int alloc_work(Ctx &c, int nk)
 {
     const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
-    if (c.d_D.alloc(NN) || c.d_J.alloc(NN) || c.d_K0.alloc(NN) || c.d_P.alloc(no2) || c.d_G.alloc(no2) ||
-        c.d_dmax.alloc(1))
+    if (c.d_D.alloc(NN) || c.d_J.alloc(NN * PM_XCD_SLOTS) || c.d_K0.alloc(NN * PM_XCD_SLOTS) || c.d_P.alloc(no2) ||
+        c.d_G.alloc(no2) || c.d_dmax.alloc(1))
         return g_err;
-    if (nk == 2 && (c.d_D2.alloc(NN) || c.d_K1.alloc(NN) || c.d_G2.alloc(no2))) return g_err;
+    if (nk == 2 && (c.d_D2.alloc(NN) || c.d_K1.alloc(NN * PM_XCD_SLOTS) || c.d_G2.alloc(no2))) return g_err;
     return 0;
 }
 

@@ -13,6 +13,8 @@ is reported.  value = surviving shell quartets per second of the mixed
 build (of the FP64 build if the mixed one misses |dE| <= 1e-6).
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+       python bench.py --calibrate-copy BYTES   (plain device copy of BYTES,
+       three launches, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)
 """
 import argparse
 import json
@@ -79,6 +81,23 @@ def time_builds(torch, store, dP, dG, steps, warmup, stream):
     return ev0.elapsed_time(ev1) / steps
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_water8_fp64.json")
+
+
+def dominant_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), if it lists it."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for name, v in data.get("kernels", {}).items():
+        if kernel and name.startswith(kernel):
+            return v
+    return None
+
+
 def cpu_baseline(system, P):
     """Oracle direct G build (reference algorithm, no screening) on host cores,
     over a bounded sample: the first shell rows of the reference's canonical
@@ -95,17 +114,39 @@ def cpu_baseline(system, P):
                       f"no screening) of the same workload in {dt:.1f} s"}
 
 
+def calibrate_copy(nbytes):
+    """Three launches of a plain device-to-device copy of nbytes: the
+    reference reading for the HBM traffic counters."""
+    import torch
+
+    n = max(1, nbytes // 8)
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--serial-classes", action="store_true",
+                    help="launch the class kernels of every build on one stream (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the G build runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         return 2
+
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
 
     import torch
     import patmol_amd as pm
@@ -135,7 +176,7 @@ def main():
     time_builds(torch, store, dP, dG, 1, 1, stream)
     tot_ms, cls_ms = store.last_timing()
     # FP64 timed pass: classes concurrent on side streams
-    pm.set_options(timing=False)
+    pm.set_options(timing=args.serial_classes)
     ms64 = time_builds(torch, store, dP, dG, args.steps, args.warmup, stream)
     G64 = dG.cpu().numpy().copy()
     E64 = elec_energy(P, H, G64, n)
@@ -145,6 +186,8 @@ def main():
     achieved = f36[k] / (cls_ms[k] * 1e-3) / 1e12 if cls_ms[k] > 0 else 0.0
     names = ["ss", "ps", "pp", "ds", "dp", "dd"]
     dom = f"({names[k // 6]}|{names[k % 6]})"
+    dom_kernel = store.class_kernel_name(k // 6, k % 6)
+    traffic = dominant_traffic(dom_kernel)
     per_class = {f"({names[i // 6]}|{names[i % 6]})": {"ms": round(float(cls_ms[i]), 4), "quartets": int(q36[i]),
                                                       "tflops": round(f36[i] / (cls_ms[i] * 1e-3) / 1e12, 3)}
                  for i in range(36) if q36[i] > 0 and cls_ms[i] > 0}
@@ -159,7 +202,16 @@ def main():
     dE = abs(Emx - E64)
     log(f"mixed: {msmx:.3f} ms/build, |dE| {dE:.3e}, FP32 items {stm['work_items_fp32']}/{stm['work_items']}")
 
-    use_mixed = dE <= 1e-6
+    # reference-compatible entry point: host P in, host G out (PCIe included)
+    nh = max(1, min(args.steps, 5))
+    store.rhf_gmatrix(P)
+    t0 = time.perf_counter()
+    for _ in range(nh):
+        store.rhf_gmatrix(P)
+    ms_host = (time.perf_counter() - t0) * 1e3 / nh
+    log(f"host-array entry (rhf_form_gmatrix, PCIe included): {ms_host:.3f} ms/build")
+
+    use_mixed = dE <= 1e-6 and stm["work_items_fp32"] > 0
     ms = msmx if use_mixed else ms64
     kept = st["shell_quartets_kept"]
     value = kept / (ms * 1e-3)
@@ -191,6 +243,7 @@ def main():
         "dE_mixed_vs_fp64": dE,
         "quartets_per_s_fp64": kept / (ms64 * 1e-3),
         "fp32_items": stm["work_items_fp32"],
+        "ms_per_step_host_entry": ms_host,
         "roofline": {
             "bound": "mfma",
             "kernel": f"jk_kernel {dom} FP64",
@@ -198,7 +251,10 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_fetch": traffic["fetch_bytes"] if traffic else None,
+            "traffic_write": traffic["write_bytes"] if traffic else None,
+            "kernel_symbol": dom_kernel,
             "build_tflops": st["flops_per_build"] / (ms64 * 1e-3) / 1e12,
             "build_frac": st["flops_per_build"] / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         },

@@ -224,6 +224,20 @@ class EriStore:
             _check("patmol_hip_class_stats")
         return q, f
 
+    @staticmethod
+    def class_kernel_name(bra_pc: int, ket_pc: int, fp32: bool = False) -> str:
+        """Symbol prefix (as profilers print it) of the FP64 J/K kernel that
+        runs class (bra_pc | ket_pc); pair class pc = l(l+1)/2 + l'.  Mirrors
+        the dispatch rule of launch_jk_t in eri_kernels_impl.h."""
+        ls = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+        (la, lb), (lc, ld) = ls[bra_pc], ls[ket_pc]
+        nc = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
+        heavy = nc[0] * nc[1] * nc[2] * nc[3] >= 324 and lb == 2
+        r = "float" if fp32 else "double"
+        if heavy:
+            return f"void pm::jk_kernel<{la}, {lb}, {lc}, {ld}, {r}, 1>"
+        return f"void pm::jk_kernel2<{la}, {lb}, {lc}, {ld}, {r}>"
+
     def pair_schwarz(self) -> np.ndarray:
         _, L = lib()
         q = np.zeros(self.nshell * (self.nshell + 1) // 2)

@@ -207,6 +207,59 @@ __device__ __forceinline__ void rys_eval(float T, float (&x)[N], float (&w)[N], 
     }
 }
 
+// One root r (run-time index) of the N-point rule: the same fits, two
+// Clenshaw series instead of 2N.
+template <int N>
+__device__ __forceinline__ void rys_eval_one(double T, int r, double &x, double &w, const RysTables &tb)
+{
+    if (T < (double)rys_tmax(N)) {
+        int k = (int)T;
+        double u = 2.0 * (T - k) - 1.0, u2 = 2.0 * u;
+        const double *__restrict__ c = tb.cheb + rys_off(N) + k * (RYS_NC * 2 * N) + r;
+        double bx1 = 0.0, bx2 = 0.0, bw1 = 0.0, bw2 = 0.0;
+#pragma unroll
+        for (int j = RYS_NC - 1; j >= 1; --j) {
+            double tx = fma(u2, bx1, c[j * 2 * N] - bx2);
+            double tw = fma(u2, bw1, c[j * 2 * N + N] - bw2);
+            bx2 = bx1;
+            bx1 = tx;
+            bw2 = bw1;
+            bw1 = tw;
+        }
+        x = fma(u, bx1, c[0] - bx2);
+        w = fma(u, bw1, c[N] - bw2);
+    } else {
+        x = tb.herm_r[N * (N - 1) / 2 + r] / T;
+        w = tb.herm_h[N * (N - 1) / 2 + r] / sqrt(T);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void rys_eval_one(float T, int r, float &x, float &w, const RysTables &tb)
+{
+    constexpr int NC32 = 9;
+    if (T < (float)rys_tmax(N)) {
+        int k = (int)T;
+        float u = 2.0f * (T - (float)k) - 1.0f, u2 = 2.0f * u;
+        const float *__restrict__ c = tb.cheb32 + rys_off(N) + k * (RYS_NC * 2 * N) + r;
+        float bx1 = 0.0f, bx2 = 0.0f, bw1 = 0.0f, bw2 = 0.0f;
+#pragma unroll
+        for (int j = NC32 - 1; j >= 1; --j) {
+            float tx = fmaf(u2, bx1, c[j * 2 * N] - bx2);
+            float tw = fmaf(u2, bw1, c[j * 2 * N + N] - bw2);
+            bx2 = bx1;
+            bx1 = tx;
+            bw2 = bw1;
+            bw1 = tw;
+        }
+        x = fmaf(u, bx1, c[0] - bx2);
+        w = fmaf(u, bw1, c[N] - bw2);
+    } else {
+        x = (float)tb.herm_r[N * (N - 1) / 2 + r] / T;
+        w = (float)tb.herm_h[N * (N - 1) / 2 + r] / sqrtf(T);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 2-D recursion, restating RysRecur (rys.c:2269-2292):
 //   G[0][0] = 1, G[1][0] = C, G[0][1] = C', G[1][1] = B00 + C C',
@@ -337,6 +390,73 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
         });
     });
     return keep;
+}
+
+// Same integrals, one Rys root at a time: f(ia, ib, ic, id, v_r) receives
+// the contribution of each root separately (the integral is the sum over
+// roots, and every use of it here is linear), so only one root's 2-D tables
+// are live instead of all of them.
+template <typename R, int LA, int LB, int LC, int LD, typename F>
+__device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, const PrimPair &kp,
+                                                           const PairGeom<R> &bra, const PairGeom<R> &ket,
+                                                           const RysTables &tab, R scale, F &&f)
+{
+    constexpr int L12 = LA + LB, L34 = LC + LD, NR = (L12 + L34) / 2 + 1;
+    const double p = bp.p, q = kp.p, pq = p + q;
+    const double sr0 = SRTERM * bp.s * kp.s / (p * q * sqrt(pq));
+    const bool keep = sr0 >= PRIM_SCREEN;
+    const R sr = keep ? R(sr0 * bp.c * kp.c) * scale : R(0);
+    const R PQ[3] = {R(bp.P[0] - kp.P[0]), R(bp.P[1] - kp.P[1]), R(bp.P[2] - kp.P[2])};
+    const R rp = R(p), rq = R(q), ipq = R(1.0 / pq);
+    const R T = (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]) * rp * rq * ipq;
+    const R PA[3] = {R(bp.P[0]) - bra.A[0], R(bp.P[1]) - bra.A[1], R(bp.P[2]) - bra.A[2]};
+    const R QC[3] = {R(kp.P[0]) - ket.A[0], R(kp.P[1]) - ket.A[1], R(kp.P[2]) - ket.A[2]};
+    R abp[3][LB + 1], cdp[3][LD + 1];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        abp[a][0] = R(1);
+#pragma unroll
+        for (int i = 1; i <= LB; ++i) abp[a][i] = abp[a][i - 1] * bra.AB[a];
+        cdp[a][0] = R(1);
+#pragma unroll
+        for (int i = 1; i <= LD; ++i) cdp[a][i] = cdp[a][i - 1] * ket.AB[a];
+    }
+    // a rolled loop: one root's tables at a time; the memory clobber keeps the
+    // caller's density reads inside the loop instead of hoisted into registers
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {
+        __asm__ volatile("" ::: "memory");
+        R xr, wr;
+        rys_eval_one<NR>(T, r, xr, wr, tab);
+        R Gx[L12 + 1][L34 + 1], Gy[L12 + 1][L34 + 1], Gz[L12 + 1][L34 + 1];
+        const R fff = xr * ipq;
+        const R B00 = R(0.5) * fff;
+        const R B1 = (R(0.5) - B00 * rq) / rp;
+        const R B1p = (R(0.5) - B00 * rp) / rq;
+        rys_vrr<R, L12, L34>(Gx, B00, B1, B1p, PA[0] - rq * PQ[0] * fff, QC[0] + rp * PQ[0] * fff, R(1));
+        rys_vrr<R, L12, L34>(Gy, B00, B1, B1p, PA[1] - rq * PQ[1] * fff, QC[1] + rp * PQ[1] * fff, R(1));
+        rys_vrr<R, L12, L34>(Gz, B00, B1, B1p, PA[2] - rq * PQ[2] * fff, QC[2] + rp * PQ[2] * fff, wr * sr);
+        static_for<ncart(LA)>([&](auto ia) {
+            static_for<ncart(LB)>([&](auto ib) {
+                static_for<ncart(LC)>([&](auto ic) {
+                    static_for<ncart(LD)>([&](auto id) {
+                        constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+                        constexpr int IC = decltype(ic)::value, ID = decltype(id)::value;
+                        constexpr int ax = cart_x(LA, IA), ay = cart_y(LA, IA), az = cart_z(LA, IA);
+                        constexpr int bx = cart_x(LB, IB), by = cart_y(LB, IB), bz = cart_z(LB, IB);
+                        constexpr int cx = cart_x(LC, IC), cy = cart_y(LC, IC), cz = cart_z(LC, IC);
+                        constexpr int dx = cart_x(LD, ID), dy = cart_y(LD, ID), dz = cart_z(LD, ID);
+                        constexpr double nrm = 1.0 / csqrt(cart_dfprod(LA, IA) * cart_dfprod(LB, IB) *
+                                                           cart_dfprod(LC, IC) * cart_dfprod(LD, ID));
+                        const R ix = rys_hrr<R, ax, bx, cx, dx, L12, L34, LB, LD>(Gx, abp[0], cdp[0]);
+                        const R iy = rys_hrr<R, ay, by, cy, dy, L12, L34, LB, LD>(Gy, abp[1], cdp[1]);
+                        const R iz = rys_hrr<R, az, bz, cz, dz, L12, L34, LB, LD>(Gz, abp[2], cdp[2]);
+                        f(ia, ib, ic, id, ix * iy * (iz * R(nrm)));
+                    });
+                });
+            });
+        });
+    }
 }
 
 }  // namespace pm

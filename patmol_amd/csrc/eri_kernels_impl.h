@@ -63,7 +63,11 @@ constexpr int jk_waves_per_eu()
     constexpr int pre = NA * NB + NC * ND + (NA + NB) * (NC + ND);
     constexpr int g = ((LA + LB + LC + LD) / 2 + 1) * 3 * (LA + LB + 1) * (LC + LD + 1);
     constexpr int live = acc + (pre <= 64 ? pre : 0) + g;
+#ifdef PM_JK_WAVES
+    return PM_JK_WAVES;
+#else
     return live <= 40 ? 4 : (live <= 96 ? 2 : 1);
+#endif
 }
 
 template <int LA, int LB, int LC, int LD, typename R, int NK>
@@ -611,6 +615,12 @@ __global__ __launch_bounds__(64) void schwarz_kernel(SchwarzArgs a)
     a.q[pidx] = sqrt(m);
 }
 
+}  // namespace pm
+
+#include "eri_kernel3.h"
+
+namespace pm {
+
 // ---------------------------------------------------------------------------
 // launch helpers
 template <int LA, int LB, int LC, int LD>
@@ -626,10 +636,17 @@ inline void launch_jk_t(const JKArgs &args, int prec, int nk, hipStream_t s)
         else
             hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, float, 1>), grid, block, 0, s, args);
     } else {
+#ifdef PM_USE_K2
         if (prec == 0)
             hipLaunchKernelGGL((jk_kernel2<LA, LB, LC, LD, double>), grid, block, 0, s, args);
         else
             hipLaunchKernelGGL((jk_kernel2<LA, LB, LC, LD, float>), grid, block, 0, s, args);
+#else
+        if (prec == 0)
+            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, double>), grid, block, 0, s, args);
+        else
+            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, float>), grid, block, 0, s, args);
+#endif
     }
 }
 

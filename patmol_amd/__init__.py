@@ -232,7 +232,7 @@ class EriStore:
         ls = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
         (la, lb), (lc, ld) = ls[bra_pc], ls[ket_pc]
         nc = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
-        heavy = nc[0] * nc[1] * nc[2] * nc[3] >= 324 and lb == 2
+        heavy = (la, lb, lc, ld) == (2, 2, 2, 2)
         r = "float" if fp32 else "double"
         if heavy:
             return f"void pm::jk_kernel<{la}, {lb}, {lc}, {ld}, {r}, 1>"

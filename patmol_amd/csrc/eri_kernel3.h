@@ -14,7 +14,8 @@
 //     element, LDS atomics where waves share it), and one pass of global
 //     atomics at the end of the item moves them to the per-XCD J'/K' copies.
 // Classes whose 2-D tables for all roots would not fit the register file
-// digest one Rys root at a time (primitive_quartet_rootwise).
+// digest one Rys root at a time (primitive_quartet_rootwise); the choice and
+// the register budget per class are measured (jk3_shape).
 #pragma once
 #include "eri_kernels.h"
 
@@ -27,13 +28,46 @@ struct K3 {
     static constexpr int FI = TI * NA, FJ = TJ * NB, FK = TK * NC, FL = TL * ND;
     static constexpr int KLP = TK * TL;
     static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
-    static constexpr int ACC = NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
-    static constexpr int GALL = NR * 3 * (LA + LB + 1) * (LC + LD + 1);
-    // all roots' 2-D tables plus the accumulators must stay well inside the
-    // 128 doubles one lane can hold in architectural registers
-    static constexpr bool ROOTWISE = GALL + ACC > 100;
     static constexpr int PPCAP = 128;  // staged ket primitive pairs
 };
+
+// Per-class launch shape, from timing every class on (H2O)8 with both
+// digestion orders at 1 and 2 waves per SIMD: code = waves * 2 + rootwise.
+constexpr int jk3_shape(int la, int lb, int lc, int ld)
+{
+    const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
+    switch (c) {
+    case 0: return 1 * 2 + 1;    // (ss|ss)
+    case 27: return 2 * 2 + 1;   // (ps|ss)
+    case 30: return 2 * 2 + 0;   // (ps|ps)
+    case 36: return 2 * 2 + 0;   // (pp|ss)
+    case 39: return 2 * 2 + 0;   // (pp|ps)
+    case 40: return 1 * 2 + 1;   // (pp|pp)
+    case 54: return 2 * 2 + 0;   // (ds|ss)
+    case 57: return 2 * 2 + 1;   // (ds|ps)
+    case 58: return 1 * 2 + 0;   // (ds|pp)
+    case 60: return 1 * 2 + 0;   // (ds|ds)
+    case 63: return 2 * 2 + 1;   // (dp|ss)
+    case 66: return 1 * 2 + 0;   // (dp|ps)
+    case 67: return 1 * 2 + 1;   // (dp|pp)
+    case 69: return 1 * 2 + 1;   // (dp|ds)
+    case 70: return 1 * 2 + 1;   // (dp|dp)
+    case 72: return 1 * 2 + 0;   // (dd|ss)
+    case 75: return 1 * 2 + 1;   // (dd|ps)
+    case 78: return 1 * 2 + 1;   // (dd|ds)
+    default: return 1 * 2 + 1;
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+constexpr bool jk3_rootwise()
+{
+#ifdef PM_K3_ROOTWISE
+    return PM_K3_ROOTWISE;
+#else
+    return jk3_shape(LA, LB, LC, LD) & 1;
+#endif
+}
 
 template <int LA, int LB, int LC, int LD>
 constexpr int jk3_waves_per_eu()
@@ -41,7 +75,7 @@ constexpr int jk3_waves_per_eu()
 #ifdef PM_JK_WAVES
     return PM_JK_WAVES;
 #else
-    return (LA + LB + LC + LD) <= 1 ? 2 : 1;
+    return jk3_shape(LA, LB, LC, LD) >> 1;
 #endif
 }
 
@@ -266,7 +300,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 for (int ik = 0; ik < nkp; ++ik) {
                     const PrimPair kp = kpp[min(ik, kref.npp - 1)];
                     const R sc = (ib < nbp_lane && ik < kref.npp) ? scale : R(0);
-                    if constexpr (S::ROOTWISE)
+                    if constexpr (jk3_rootwise<LA, LB, LC, LD>())
                         primitive_quartet_rootwise<R, LA, LB, LC, LD>(bp, kp, bg, kg, a.rys, sc, digest);
                     else
                         primitive_quartet<R, LA, LB, LC, LD>(bp, kp, bg, kg, a.rys, sc, digest);

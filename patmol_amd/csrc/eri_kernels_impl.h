@@ -66,6 +66,8 @@ constexpr int jk_waves_per_eu()
 #ifdef PM_JK_WAVES
     return PM_JK_WAVES;
 #else
+    // (dd|pp) measured faster at 2 waves per SIMD
+    if (LA == 2 && LB == 2 && LC == 1 && LD == 1) return 2;
     return live <= 40 ? 4 : (live <= 96 ? 2 : 1);
 #endif
 }
@@ -630,7 +632,9 @@ inline void launch_jk_t(const JKArgs &args, int prec, int nk, hipStream_t s)
     // built (UHF runs two single-density passes).
     if (args.nitems <= 0 || nk != 1) return;
     dim3 grid(args.nitems), block(256);
-    if constexpr (HeavyClass<LA, LB, LC, LD>::value) {
+    // (dd|dd): the ket-pair-per-lane kernel with run-time component loops;
+    // every other class runs the LDS-staged kernel
+    if constexpr (LA == 2 && LB == 2 && LC == 2 && LD == 2) {
         if (prec == 0)
             hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, double, 1>), grid, block, 0, s, args);
         else

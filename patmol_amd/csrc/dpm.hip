@@ -71,6 +71,7 @@ __global__ __launch_bounds__(64) void dpm_kernel(DpmArgs a)
             kp.P[2] = a.Q[2];
             kp.s = 1.0;
             kp.c = kDpmCoef[k];
+            kp.ip = 1.0 / kp.p;
             primitive_quartet<double, LA, LB, 0, 0>(bp, kp, bg, kg, a.rys, 1.0,
                                                     [&](auto ia, auto ib_, auto, auto, double v) {
                                                         acc[decltype(ia)::value * ncart(LB) + decltype(ib_)::value] += v;
@@ -146,6 +147,7 @@ extern "C" void dpm_ints(double *hmat)
                             for (int x = 0; x < 3; ++x) q.P[x] = (al * A[x] + be * B[x]) / q.p;
                             q.s = exp(-al * be * ab2 / q.p);
                             q.c = sh[a].co[u] * sh[b].co[v];
+                            q.ip = 1.0 / q.p;
                             pp.push_back(q);
                         }
                     refs.push_back(r);

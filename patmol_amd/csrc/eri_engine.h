@@ -116,6 +116,7 @@ struct PrimPair {       // one primitive pair of a shell pair
     double P[3];        // product centre
     double s;           // exp(-alpha beta/p |A-B|^2)
     double c;           // c_alpha c_beta (normalised contraction coefficients)
+    double ip;          // 1 / p
 };
 
 struct PairRef {        // shell pair in the sorted shell order, I >= J
@@ -329,7 +330,8 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
 {
     constexpr int L12 = LA + LB, L34 = LC + LD, NR = (L12 + L34) / 2 + 1;
     const double p = bp.p, q = kp.p, pq = p + q;
-    const double sr0 = SRTERM * bp.s * kp.s / (p * q * sqrt(pq));
+    const double ipq_d = 1.0 / pq;
+    const double sr0 = SRTERM * (bp.s * bp.ip) * (kp.s * kp.ip) * sqrt(ipq_d);
     // Screened primitive quartets still run (with a zero prefactor) so that
     // every lane reaches the cross-lane reductions inside f.  Bounds (SCREEN
     // false) must not use the screen: it can zero a diagonal (ab|ab) whose
@@ -337,7 +339,7 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
     const bool keep = !SCREEN || sr0 >= PRIM_SCREEN;
     const R sr = keep ? R(sr0 * bp.c * kp.c) * scale : R(0);
     const R PQ[3] = {R(bp.P[0] - kp.P[0]), R(bp.P[1] - kp.P[1]), R(bp.P[2] - kp.P[2])};
-    const R rp = R(p), rq = R(q), ipq = R(1.0 / pq);
+    const R rp = R(p), rq = R(q), ipq = R(ipq_d), irp = R(bp.ip), irq = R(kp.ip);
     const R T = (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]) * rp * rq * ipq;
     R x[NR], w[NR];
     rys_eval<NR>(T, x, w, tab);
@@ -348,8 +350,8 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
     for (int r = 0; r < NR; ++r) {
         const R fff = x[r] * ipq;
         const R B00 = R(0.5) * fff;
-        const R B1 = (R(0.5) - B00 * rq) / rp;
-        const R B1p = (R(0.5) - B00 * rp) / rq;
+        const R B1 = (R(0.5) - B00 * rq) * irp;
+        const R B1p = (R(0.5) - B00 * rp) * irq;
         rys_vrr<R, L12, L34>(Gx[r], B00, B1, B1p, PA[0] - rq * PQ[0] * fff, QC[0] + rp * PQ[0] * fff, R(1));
         rys_vrr<R, L12, L34>(Gy[r], B00, B1, B1p, PA[1] - rq * PQ[1] * fff, QC[1] + rp * PQ[1] * fff, R(1));
         rys_vrr<R, L12, L34>(Gz[r], B00, B1, B1p, PA[2] - rq * PQ[2] * fff, QC[2] + rp * PQ[2] * fff, w[r] * sr);
@@ -403,11 +405,12 @@ __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, c
 {
     constexpr int L12 = LA + LB, L34 = LC + LD, NR = (L12 + L34) / 2 + 1;
     const double p = bp.p, q = kp.p, pq = p + q;
-    const double sr0 = SRTERM * bp.s * kp.s / (p * q * sqrt(pq));
+    const double ipq_d = 1.0 / pq;
+    const double sr0 = SRTERM * (bp.s * bp.ip) * (kp.s * kp.ip) * sqrt(ipq_d);
     const bool keep = sr0 >= PRIM_SCREEN;
     const R sr = keep ? R(sr0 * bp.c * kp.c) * scale : R(0);
     const R PQ[3] = {R(bp.P[0] - kp.P[0]), R(bp.P[1] - kp.P[1]), R(bp.P[2] - kp.P[2])};
-    const R rp = R(p), rq = R(q), ipq = R(1.0 / pq);
+    const R rp = R(p), rq = R(q), ipq = R(ipq_d), irp = R(bp.ip), irq = R(kp.ip);
     const R T = (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]) * rp * rq * ipq;
     const R PA[3] = {R(bp.P[0]) - bra.A[0], R(bp.P[1]) - bra.A[1], R(bp.P[2]) - bra.A[2]};
     const R QC[3] = {R(kp.P[0]) - ket.A[0], R(kp.P[1]) - ket.A[1], R(kp.P[2]) - ket.A[2]};
@@ -431,8 +434,8 @@ __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, c
         R Gx[L12 + 1][L34 + 1], Gy[L12 + 1][L34 + 1], Gz[L12 + 1][L34 + 1];
         const R fff = xr * ipq;
         const R B00 = R(0.5) * fff;
-        const R B1 = (R(0.5) - B00 * rq) / rp;
-        const R B1p = (R(0.5) - B00 * rp) / rq;
+        const R B1 = (R(0.5) - B00 * rq) * irp;
+        const R B1p = (R(0.5) - B00 * rp) * irq;
         rys_vrr<R, L12, L34>(Gx, B00, B1, B1p, PA[0] - rq * PQ[0] * fff, QC[0] + rp * PQ[0] * fff, R(1));
         rys_vrr<R, L12, L34>(Gy, B00, B1, B1p, PA[1] - rq * PQ[1] * fff, QC[1] + rp * PQ[1] * fff, R(1));
         rys_vrr<R, L12, L34>(Gz, B00, B1, B1p, PA[2] - rq * PQ[2] * fff, QC[2] + rp * PQ[2] * fff, wr * sr);
@@ -497,11 +500,12 @@ struct OneDTable {
                                           bool screen = true)
     {
         const double p = bp.p, q = kp.p, pq = p + q;
-        const double sr0 = SRTERM * bp.s * kp.s / (p * q * sqrt(pq));
+        const double ipq_d = 1.0 / pq;
+        const double sr0 = SRTERM * (bp.s * bp.ip) * (kp.s * kp.ip) * sqrt(ipq_d);
         const bool keep = !screen || sr0 >= PRIM_SCREEN;
         const R sr = keep ? R(sr0 * bp.c * kp.c) * scale : R(0);
         const R PQ[3] = {R(bp.P[0] - kp.P[0]), R(bp.P[1] - kp.P[1]), R(bp.P[2] - kp.P[2])};
-        const R rp = R(p), rq = R(q), ipq = R(1.0 / pq);
+        const R rp = R(p), rq = R(q), ipq = R(ipq_d), irp = R(bp.ip), irq = R(kp.ip);
         const R T = (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]) * rp * rq * ipq;
         R x[NR], w[NR];
         rys_eval<NR>(T, x, w, tab);
@@ -521,8 +525,8 @@ struct OneDTable {
         for (int r = 0; r < NR; ++r) {
             const R fff = x[r] * ipq;
             const R B00 = R(0.5) * fff;
-            const R B1 = (R(0.5) - B00 * rq) / rp;
-            const R B1p = (R(0.5) - B00 * rp) / rq;
+            const R B1 = (R(0.5) - B00 * rq) * irp;
+            const R B1p = (R(0.5) - B00 * rp) * irq;
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
                 R G[L12 + 1][L34 + 1];

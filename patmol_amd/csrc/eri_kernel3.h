@@ -103,7 +103,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     __shared__ int sKok[KLP];
     __shared__ int sPPoff[KLP];
     __shared__ PrimPair sPP[S::PPCAP];
-    __shared__ int sStaged;
+    __shared__ int sStaged, sKmax;
     // block offsets inside sD / sO
     constexpr int oIJ = 0, oKL = oIJ + FI * FJ, oIK = oKL + FK * FL, oIL = oIK + FI * FK, oJK = oIL + FI * FL,
                   oJL = oJK + FJ * FK, oEND = oJL + FJ * FL;
@@ -190,12 +190,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     }
     __syncthreads();
     if (tid == 0) {
-        int tot = 0;
+        int tot = 0, mx = 1;
         for (int t = 0; t < KLP; ++t) {
             sPPoff[t] = tot;
             tot += sKref[t].npp;
+            mx = max(mx, sKref[t].npp);
         }
         sStaged = tot <= S::PPCAP ? 1 : 0;
+        sKmax = mx;
     }
     __syncthreads();
     const bool staged = sStaged != 0;
@@ -280,9 +282,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             const double *__restrict__ DIL = sD + oIL + (ii * NA) * FL + lq * ND;
             const double *__restrict__ DJK = sD + oJK + (jj * NB) * FK + kq * NC;
             const double *__restrict__ DIK = sD + oIK + (ii * NA) * FK + kq * NC;
-            int nkp = kref.npp;
-#pragma unroll
-            for (int b = 0; b < 6; ++b) nkp = max(nkp, __shfl_xor(nkp, 1 << b, 64));
+            const int nkp = sKmax;  // uniform bound of the ket primitive loop
             const PrimPair *kpp = staged ? (sPP + sPPoff[t]) : (a.pp + kref.pp0);
             auto digest = [&](auto ia, auto ib_, auto ic, auto id, R vr) {
                 constexpr int A_ = decltype(ia)::value, B_ = decltype(ib_)::value;

@@ -23,12 +23,43 @@
 
 namespace pm {
 
+// Lane i + 2^k of the same row of 16 (DPP row_shl), for 2^k <= 8.
+template <int K>
+__device__ __forceinline__ double dpp_row_shl(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x100 | (1 << K), 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x100 | (1 << K), 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Lane i + 16 for lanes in even rows (v_permlane16_swap), lane i + 32 for
+// lanes 0..31 (v_permlane32_swap).
+__device__ __forceinline__ double lane_plus16(double v)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+    return __hiloint2double(hi[1], lo[1]);
+}
+__device__ __forceinline__ double lane_plus32(double v)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+    return __hiloint2double(hi[1], lo[1]);
+}
+
+// Sum of v over the lanes that differ from this one only in the MASK bits,
+// valid in the leader lanes ((lane & MASK) == 0) only.  Every lane of the
+// wavefront must be active.  Stage k adds lane i + 2^k, which for a leader
+// is itself a leader of the lower stages, so the partial sums combine.
 template <int MASK>
 __device__ __forceinline__ double lane_reduce(double v)
 {
-#pragma unroll
-    for (int b = 0; b < 6; ++b)
-        if (MASK & (1 << b)) v += __shfl_xor(v, 1 << b, 64);
+    if constexpr ((MASK & 1) != 0) v += dpp_row_shl<0>(v);
+    if constexpr ((MASK & 2) != 0) v += dpp_row_shl<1>(v);
+    if constexpr ((MASK & 4) != 0) v += dpp_row_shl<2>(v);
+    if constexpr ((MASK & 8) != 0) v += dpp_row_shl<3>(v);
+    if constexpr ((MASK & 16) != 0) v += lane_plus16(v);
+    if constexpr ((MASK & 32) != 0) v += lane_plus32(v);
     return v;
 }
 

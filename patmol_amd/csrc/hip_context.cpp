@@ -259,6 +259,7 @@ static int build_tables(Ctx &c, int start)
                         q.s = exp(-al * be * ab2 / p);
                     }
                     q.c = a.co[i] * b.co[j];
+                    q.ip = 1.0 / p;
                     c.pp.push_back(q);
                 }
         }

@@ -398,7 +398,7 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
 // the contribution of each root separately (the integral is the sum over
 // roots, and every use of it here is linear), so only one root's 2-D tables
 // are live instead of all of them.
-template <typename R, int LA, int LB, int LC, int LD, typename F>
+template <typename R, int LA, int LB, int LC, int LD, bool BLOCK_FENCE, typename F>
 __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, const PrimPair &kp,
                                                            const PairGeom<R> &bra, const PairGeom<R> &ket,
                                                            const RysTables &tab, R scale, F &&f)
@@ -441,6 +441,9 @@ __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, c
         rys_vrr<R, L12, L34>(Gz, B00, B1, B1p, PA[2] - rq * PQ[2] * fff, QC[2] + rp * PQ[2] * fff, wr * sr);
         static_for<ncart(LA)>([&](auto ia) {
             static_for<ncart(LB)>([&](auto ib) {
+                // with BLOCK_FENCE the caller's density reads for this (a, b)
+                // block are issued here, not hoisted for all blocks at once
+                if constexpr (BLOCK_FENCE) __asm__ volatile("" ::: "memory");
                 static_for<ncart(LC)>([&](auto ic) {
                     static_for<ncart(LD)>([&](auto id) {
                         constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;

@@ -14,9 +14,11 @@
 //     element, LDS atomics where waves share it), and one pass of global
 //     atomics at the end of the item moves them to the per-XCD J'/K' copies.
 // Classes whose 2-D tables for all roots would not fit the register file
-// digest one Rys root at a time (primitive_quartet_rootwise); the choice and
-// the register budget per class are measured (jk3_shape).
+// digest one Rys root at a time (primitive_quartet_rootwise); that choice, the
+// density-read fence and the register budget per class are measured
+// (jk3_shape).
 #pragma once
+#include <type_traits>
 #include "eri_kernels.h"
 
 namespace pm {
@@ -32,7 +34,8 @@ struct K3 {
 };
 
 // Per-class launch shape, from timing every class on (H2O)8 with both
-// digestion orders at 1 and 2 waves per SIMD: code = waves * 2 + rootwise.
+// digestion orders, with and without the density-read fence, at 1 and 2
+// waves per SIMD: code = fence * 8 + waves * 2 + rootwise.
 constexpr int jk3_shape(int la, int lb, int lc, int ld)
 {
     const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
@@ -51,10 +54,12 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 66: return 1 * 2 + 0;   // (dp|ps)
     case 67: return 1 * 2 + 1;   // (dp|pp)
     case 69: return 1 * 2 + 1;   // (dp|ds)
-    case 70: return 1 * 2 + 1;   // (dp|dp)
+    case 70: return 8 + 1 * 2 + 1;   // (dp|dp)
     case 72: return 1 * 2 + 0;   // (dd|ss)
     case 75: return 1 * 2 + 1;   // (dd|ps)
-    case 78: return 1 * 2 + 1;   // (dd|ds)
+    case 76: return 8 + 1 * 2 + 1;   // (dd|pp)
+    case 78: return 8 + 1 * 2 + 1;   // (dd|ds)
+    case 79: return 8 + 1 * 2 + 1;   // (dd|dp)
     default: return 1 * 2 + 1;
     }
 }
@@ -70,12 +75,22 @@ constexpr bool jk3_rootwise()
 }
 
 template <int LA, int LB, int LC, int LD>
+constexpr bool jk3_block_fence()
+{
+#ifdef PM_K3_FENCE
+    return PM_K3_FENCE;
+#else
+    return (jk3_shape(LA, LB, LC, LD) >> 3) & 1;
+#endif
+}
+
+template <int LA, int LB, int LC, int LD>
 constexpr int jk3_waves_per_eu()
 {
 #ifdef PM_JK_WAVES
     return PM_JK_WAVES;
 #else
-    return jk3_shape(LA, LB, LC, LD) >> 1;
+    return (jk3_shape(LA, LB, LC, LD) >> 1) & 3;
 #endif
 }
 
@@ -94,8 +109,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     constexpr int NLSTEP = (TL + KSUB - 1) / KSUB;
     static_assert(IJN * KSUB == 64, "bra tile pair must divide a wavefront");
 
-    __shared__ double sD[FI * FJ + FK * FL + FI * FK + FI * FL + FJ * FK + FJ * FL];
-    __shared__ double sO[FI * FJ + FK * FL + FI * FK + FI * FL + FJ * FK + FJ * FL];
+    // block rows padded to an odd number of doubles so that rows of
+    // different bra shells fall into different LDS banks
+    constexpr int SJ = FJ | 1, SK = FK | 1, SL = FL | 1;
+    __shared__ double sD[FI * SJ + FK * SL + FI * SK + FI * SL + FJ * SK + FJ * SL];
+    __shared__ double sO[FI * SJ + FK * SL + FI * SK + FI * SL + FJ * SK + FJ * SL];
     __shared__ int sOff[TI + TJ + TK + TL];
     __shared__ double sXK[TK][3], sXL[TL][3];
     __shared__ PairRef sKref[KLP];
@@ -104,9 +122,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     __shared__ int sPPoff[KLP];
     __shared__ PrimPair sPP[S::PPCAP];
     __shared__ int sStaged, sKmax;
+    // Rys coefficient tables of the one- and two-root rules kept in LDS for
+    // the light classes (their lookups are otherwise global-memory latency
+    // on the critical path of every step)
+    constexpr bool RYS_LDS = std::is_same<R, double>::value && S::NR <= 2;
+    constexpr int RYS_LDS_N = RYS_LDS ? rys_off(S::NR) + rys_tmax(S::NR) * RYS_NC * 2 * S::NR : 2;
+    __shared__ __attribute__((aligned(16))) double sRys[RYS_LDS_N];
     // block offsets inside sD / sO
-    constexpr int oIJ = 0, oKL = oIJ + FI * FJ, oIK = oKL + FK * FL, oIL = oIK + FI * FK, oJK = oIL + FI * FL,
-                  oJL = oJK + FJ * FK, oEND = oJL + FJ * FL;
+    constexpr int oIJ = 0, oKL = oIJ + FI * SJ, oIK = oKL + FK * SL, oIL = oIK + FI * SK, oJK = oIL + FI * SL,
+                  oJL = oJK + FJ * SK, oEND = oJL + FJ * SL;
 
     const int item = blockIdx.x;
     if (item >= a.nitems) return;
@@ -146,47 +170,34 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         sKq[t] = a.pair_q[pk];
         sKok[t] = ok ? 1 : 0;
     }
+    if constexpr (RYS_LDS) {
+        const double2 *__restrict__ src = reinterpret_cast<const double2 *>(a.rys.cheb);
+        double2 *dst = reinterpret_cast<double2 *>(sRys);
+        for (int e = tid; e < RYS_LDS_N / 2; e += 256) dst[e] = src[e];
+    }
+    RysTables rt = a.rys;
+    if constexpr (RYS_LDS) rt.cheb = sRys;
     {
-        const double *__restrict__ DJ = a.DJ;
-        const double *__restrict__ DK = a.DK0;
-        for (int e = tid; e < oEND; e += 256) {
-            int r, c, ro, co;
-            const double *src;
-            bool ex;
-            if (e < oKL) {
-                r = (e - oIJ) / FJ; c = (e - oIJ) % FJ;
-                ex = r / NA < nI && c / NB < nJ;
-                ro = a.shell_off[fI + (ex ? r / NA : 0)] + r % NA; co = a.shell_off[fJ + (ex ? c / NB : 0)] + c % NB;
-                src = DJ;
-            } else if (e < oIK) {
-                r = (e - oKL) / FL; c = (e - oKL) % FL;
-                ex = r / NC < nK && c / ND < nL;
-                ro = a.shell_off[fK + (ex ? r / NC : 0)] + r % NC; co = a.shell_off[fL + (ex ? c / ND : 0)] + c % ND;
-                src = DJ;
-            } else if (e < oIL) {
-                r = (e - oIK) / FK; c = (e - oIK) % FK;
-                ex = r / NA < nI && c / NC < nK;
-                ro = a.shell_off[fI + (ex ? r / NA : 0)] + r % NA; co = a.shell_off[fK + (ex ? c / NC : 0)] + c % NC;
-                src = DK;
-            } else if (e < oJK) {
-                r = (e - oIL) / FL; c = (e - oIL) % FL;
-                ex = r / NA < nI && c / ND < nL;
-                ro = a.shell_off[fI + (ex ? r / NA : 0)] + r % NA; co = a.shell_off[fL + (ex ? c / ND : 0)] + c % ND;
-                src = DK;
-            } else if (e < oJL) {
-                r = (e - oJK) / FK; c = (e - oJK) % FK;
-                ex = r / NB < nJ && c / NC < nK;
-                ro = a.shell_off[fJ + (ex ? r / NB : 0)] + r % NB; co = a.shell_off[fK + (ex ? c / NC : 0)] + c % NC;
-                src = DK;
-            } else {
-                r = (e - oJL) / FL; c = (e - oJL) % FL;
-                ex = r / NB < nJ && c / ND < nL;
-                ro = a.shell_off[fJ + (ex ? r / NB : 0)] + r % NB; co = a.shell_off[fL + (ex ? c / ND : 0)] + c % ND;
-                src = DK;
+        // one block of the tile quartet: rows of tile (rf, rn) with NRc
+        // components per shell, columns of tile (cf, cn) with NCc components
+        auto gather = [&](int o, int rows, int stride, int rf, int rn, int nrc, int cf, int cn, int ncc,
+                          const double *__restrict__ src) {
+            for (int e = tid; e < rows * stride; e += 256) {
+                const int r = e / stride, c = e % stride;
+                const int rs = r / nrc, cs = c / ncc;
+                const bool ex = c < cn * ncc && rs < rn && cs < cn;
+                double v = 0.0;
+                if (ex) v = src[(size_t)(a.shell_off[rf + rs] + r % nrc) * N + a.shell_off[cf + cs] + c % ncc];
+                sD[o + e] = v;
+                sO[o + e] = 0.0;
             }
-            sD[e] = ex ? src[(size_t)ro * N + co] : 0.0;
-            sO[e] = 0.0;
-        }
+        };
+        gather(oIJ, FI, SJ, fI, nI, NA, fJ, nJ, NB, a.DJ);
+        gather(oKL, FK, SL, fK, nK, NC, fL, nL, ND, a.DJ);
+        gather(oIK, FI, SK, fI, nI, NA, fK, nK, NC, a.DK0);
+        gather(oIL, FI, SL, fI, nI, NA, fL, nL, ND, a.DK0);
+        gather(oJK, FJ, SK, fJ, nJ, NB, fK, nK, NC, a.DK0);
+        gather(oJL, FJ, SL, fJ, nJ, NB, fL, nL, ND, a.DK0);
     }
     __syncthreads();
     if (tid == 0) {
@@ -233,7 +244,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     double jab[NA * NB];
 #pragma unroll
     for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
-    const double *__restrict__ DIJ = sD + oIJ + (ii * NA) * FJ + jj * NB;
+    const double *__restrict__ DIJ = sD + oIJ + (ii * NA) * SJ + jj * NB;
 
     for (int kq = wave; kq < TK; kq += 4) {
         if (kq >= nK) break;  // uniform
@@ -277,23 +288,23 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             for (int i = 0; i < NA * ND; ++i) kad[i] = 0.0;
 #pragma unroll
             for (int i = 0; i < NB * ND; ++i) kbd[i] = 0.0;
-            const double *__restrict__ DKL = sD + oKL + (kq * NC) * FL + lq * ND;
-            const double *__restrict__ DJL = sD + oJL + (jj * NB) * FL + lq * ND;
-            const double *__restrict__ DIL = sD + oIL + (ii * NA) * FL + lq * ND;
-            const double *__restrict__ DJK = sD + oJK + (jj * NB) * FK + kq * NC;
-            const double *__restrict__ DIK = sD + oIK + (ii * NA) * FK + kq * NC;
+            const double *__restrict__ DKL = sD + oKL + (kq * NC) * SL + lq * ND;
+            const double *__restrict__ DJL = sD + oJL + (jj * NB) * SL + lq * ND;
+            const double *__restrict__ DIL = sD + oIL + (ii * NA) * SL + lq * ND;
+            const double *__restrict__ DJK = sD + oJK + (jj * NB) * SK + kq * NC;
+            const double *__restrict__ DIK = sD + oIK + (ii * NA) * SK + kq * NC;
             const int nkp = sKmax;  // uniform bound of the ket primitive loop
             const PrimPair *kpp = staged ? (sPP + sPPoff[t]) : (a.pp + kref.pp0);
             auto digest = [&](auto ia, auto ib_, auto ic, auto id, R vr) {
                 constexpr int A_ = decltype(ia)::value, B_ = decltype(ib_)::value;
                 constexpr int C_ = decltype(ic)::value, D_ = decltype(id)::value;
                 const double v = double(vr);
-                jab[A_ * NB + B_] += v * DKL[C_ * FL + D_];
-                jcd[C_ * ND + D_] += v * DIJ[A_ * FJ + B_];
-                kac[A_ * NC + C_] += v * DJL[B_ * FL + D_];
-                kad[A_ * ND + D_] += v * DJK[B_ * FK + C_];
-                kbc[B_ * NC + C_] += v * DIL[A_ * FL + D_];
-                kbd[B_ * ND + D_] += v * DIK[A_ * FK + C_];
+                jab[A_ * NB + B_] += v * DKL[C_ * SL + D_];
+                jcd[C_ * ND + D_] += v * DIJ[A_ * SJ + B_];
+                kac[A_ * NC + C_] += v * DJL[B_ * SL + D_];
+                kad[A_ * ND + D_] += v * DJK[B_ * SK + C_];
+                kbc[B_ * NC + C_] += v * DIL[A_ * SL + D_];
+                kbd[B_ * ND + D_] += v * DIK[A_ * SK + C_];
             };
             for (int ib = 0; ib < nbp; ++ib) {
                 const PrimPair bp = (ib == 0) ? bp0 : a.pp[bref.pp0 + min(ib, nbp_lane - 1)];
@@ -301,96 +312,89 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                     const PrimPair kp = kpp[min(ik, kref.npp - 1)];
                     const R sc = (ib < nbp_lane && ik < kref.npp) ? scale : R(0);
                     if constexpr (jk3_rootwise<LA, LB, LC, LD>())
-                        primitive_quartet_rootwise<R, LA, LB, LC, LD>(bp, kp, bg, kg, a.rys, sc, digest);
+                        primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD>()>(
+                            bp, kp, bg, kg, rt, sc, digest);
                     else
-                        primitive_quartet<R, LA, LB, LC, LD>(bp, kp, bg, kg, a.rys, sc, digest);
+                        primitive_quartet<R, LA, LB, LC, LD>(bp, kp, bg, kg, rt, sc, digest);
                 }
             }
             // per-step results: J'(KL) (one wave owns K), K'(IL), K'(JL) (shared by waves)
+            lane_reduce_n<MASK_I | MASK_J>(jcd);
+            lane_reduce_n<MASK_J>(kad);
+            lane_reduce_n<MASK_I>(kbd);
+            if ((lane & (MASK_I | MASK_J)) == 0 && lexist) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
+                for (int c = 0; c < NC; ++c)
 #pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    const double s = lane_reduce<MASK_I | MASK_J>(jcd[c * ND + d]);
-                    if ((lane & (MASK_I | MASK_J)) == 0 && lexist) sO[oKL + (kq * NC + c) * FL + lq * ND + d] += 2.0 * s;
-                }
+                    for (int d = 0; d < ND; ++d) sO[oKL + (kq * NC + c) * SL + lq * ND + d] += 2.0 * jcd[c * ND + d];
+            }
+            if ((lane & MASK_J) == 0 && iexist && lexist) {
+#pragma unroll
+                for (int x = 0; x < NA; ++x)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d)
+                        if (kad[x * ND + d] != 0.0)
+                            __hip_atomic_fetch_add(&sO[oIL + (ii * NA + x) * SL + lq * ND + d], kad[x * ND + d],
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            if ((lane & MASK_I) == 0 && jexist && lexist) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d)
+                        if (kbd[b * ND + d] != 0.0)
+                            __hip_atomic_fetch_add(&sO[oJL + (jj * NB + b) * SL + lq * ND + d], kbd[b * ND + d],
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        // per-K results: K'(IK), K'(JK) (one wave owns K)
+        lane_reduce_n<MASK_J | MASK_S>(kac);
+        lane_reduce_n<MASK_I | MASK_S>(kbc);
+        if ((lane & (MASK_J | MASK_S)) == 0 && iexist) {
 #pragma unroll
             for (int x = 0; x < NA; ++x)
 #pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    const double s = lane_reduce<MASK_J>(kad[x * ND + d]);
-                    if ((lane & MASK_J) == 0 && iexist && lexist && s != 0.0)
-                        __hip_atomic_fetch_add(&sO[oIL + (ii * NA + x) * FL + lq * ND + d], s, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
+                for (int c = 0; c < NC; ++c) sO[oIK + (ii * NA + x) * SK + kq * NC + c] += kac[x * NC + c];
+        }
+        if ((lane & (MASK_I | MASK_S)) == 0 && jexist) {
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    const double s = lane_reduce<MASK_I>(kbd[b * ND + d]);
-                    if ((lane & MASK_I) == 0 && jexist && lexist && s != 0.0)
-                        __hip_atomic_fetch_add(&sO[oJL + (jj * NB + b) * FL + lq * ND + d], s, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
+                for (int c = 0; c < NC; ++c) sO[oJK + (jj * NB + b) * SK + kq * NC + c] += kbc[b * NC + c];
         }
-        // per-K results: K'(IK), K'(JK) (one wave owns K)
+    }
+    // per-item result: J'(IJ) (shared by the four waves)
+    lane_reduce_n<MASK_S>(jab);
+    if ((lane & MASK_S) == 0 && iexist && jexist) {
 #pragma unroll
         for (int x = 0; x < NA; ++x)
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const double s = lane_reduce<MASK_J | MASK_S>(kac[x * NC + c]);
-                if ((lane & (MASK_J | MASK_S)) == 0 && iexist) sO[oIK + (ii * NA + x) * FK + kq * NC + c] += s;
-            }
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const double s = lane_reduce<MASK_I | MASK_S>(kbc[b * NC + c]);
-                if ((lane & (MASK_I | MASK_S)) == 0 && jexist) sO[oJK + (jj * NB + b) * FK + kq * NC + c] += s;
-            }
+            for (int b = 0; b < NB; ++b)
+                if (jab[x * NB + b] != 0.0)
+                    __hip_atomic_fetch_add(&sO[oIJ + (ii * NA + x) * SJ + jj * NB + b], 2.0 * jab[x * NB + b],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    // per-item result: J'(IJ) (shared by the four waves)
-#pragma unroll
-    for (int x = 0; x < NA; ++x)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const double s = lane_reduce<MASK_S>(jab[x * NB + b]);
-            if ((lane & MASK_S) == 0 && iexist && jexist && s != 0.0)
-                __hip_atomic_fetch_add(&sO[oIJ + (ii * NA + x) * FJ + jj * NB + b], 2.0 * s, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
     __syncthreads();
 
     // ---- flush the output blocks to this XCD's J'/K' copy ----
     const size_t xoff = (size_t)xcd_slot() * N * N;
     double *const Jx = a.J + xoff;
     double *const Kx = a.K0 + xoff;
-    for (int e = tid; e < oEND; e += 256) {
-        const double v = sO[e];
-        if (v == 0.0) continue;
-        int ro, co;
-        double *dst;
-        if (e < oKL) {
-            const int r = (e - oIJ) / FJ, c = (e - oIJ) % FJ;
-            ro = sOff[r / NA] + r % NA; co = sOff[TI + c / NB] + c % NB; dst = Jx;
-        } else if (e < oIK) {
-            const int r = (e - oKL) / FL, c = (e - oKL) % FL;
-            ro = sOff[TI + TJ + r / NC] + r % NC; co = sOff[TI + TJ + TK + c / ND] + c % ND; dst = Jx;
-        } else if (e < oIL) {
-            const int r = (e - oIK) / FK, c = (e - oIK) % FK;
-            ro = sOff[r / NA] + r % NA; co = sOff[TI + TJ + c / NC] + c % NC; dst = Kx;
-        } else if (e < oJK) {
-            const int r = (e - oIL) / FL, c = (e - oIL) % FL;
-            ro = sOff[r / NA] + r % NA; co = sOff[TI + TJ + TK + c / ND] + c % ND; dst = Kx;
-        } else if (e < oJL) {
-            const int r = (e - oJK) / FK, c = (e - oJK) % FK;
-            ro = sOff[TI + r / NB] + r % NB; co = sOff[TI + TJ + c / NC] + c % NC; dst = Kx;
-        } else {
-            const int r = (e - oJL) / FL, c = (e - oJL) % FL;
-            ro = sOff[TI + r / NB] + r % NB; co = sOff[TI + TJ + TK + c / ND] + c % ND; dst = Kx;
+    auto flush = [&](int o, int rows, int stride, int cols, int rsh, int nrc, int csh, int ncc, double *dst) {
+        for (int e = tid; e < rows * stride; e += 256) {
+            const int r = e / stride, c = e % stride;
+            if (c >= cols) continue;
+            const double v = sO[o + e];
+            if (v == 0.0) continue;
+            atomic_add(&dst[(size_t)(sOff[rsh + r / nrc] + r % nrc) * N + sOff[csh + c / ncc] + c % ncc], v);
         }
-        atomic_add(&dst[(size_t)ro * N + co], v);
-    }
+    };
+    flush(oIJ, FI, SJ, FJ, 0, NA, TI, NB, Jx);
+    flush(oKL, FK, SL, FL, TI + TJ, NC, TI + TJ + TK, ND, Jx);
+    flush(oIK, FI, SK, FK, 0, NA, TI + TJ, NC, Kx);
+    flush(oIL, FI, SL, FL, 0, NA, TI + TJ + TK, ND, Kx);
+    flush(oJK, FJ, SK, FK, TI, NB, TI + TJ, NC, Kx);
+    flush(oJL, FJ, SL, FL, TI, NB, TI + TJ + TK, ND, Kx);
 }
 
 }  // namespace pm

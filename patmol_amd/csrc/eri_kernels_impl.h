@@ -51,6 +51,37 @@ __device__ __forceinline__ double lane_plus32(double v)
 // valid in the leader lanes ((lane & MASK) == 0) only.  Every lane of the
 // wavefront must be active.  Stage k adds lane i + 2^k, which for a leader
 // is itself a leader of the lower stages, so the partial sums combine.
+// The same for N values at once, stage by stage, so that the cross-lane
+// moves of different values overlap.
+template <int MASK, int N>
+__device__ __forceinline__ void lane_reduce_n(double (&v)[N])
+{
+    if constexpr ((MASK & 1) != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += dpp_row_shl<0>(v[i]);
+    }
+    if constexpr ((MASK & 2) != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += dpp_row_shl<1>(v[i]);
+    }
+    if constexpr ((MASK & 4) != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += dpp_row_shl<2>(v[i]);
+    }
+    if constexpr ((MASK & 8) != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += dpp_row_shl<3>(v[i]);
+    }
+    if constexpr ((MASK & 16) != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += lane_plus16(v[i]);
+    }
+    if constexpr ((MASK & 32) != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += lane_plus32(v[i]);
+    }
+}
+
 template <int MASK>
 __device__ __forceinline__ double lane_reduce(double v)
 {

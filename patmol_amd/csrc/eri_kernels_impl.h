@@ -2,7 +2,7 @@
 //
 // Work decomposition (replaces the reference's stored integral list and its
 // per-integral scatter, twoints.c:228-404 + scfaux.c:428-552):
-//   shells are sorted by (l, centre) and cut into tiles of T(l) shells; a
+//   shells are sorted by (l, exponent, centre) and cut into tiles of T(l) shells; a
 //   work item is a canonical quartet of tiles (bra tile pair >= ket tile pair)
 //   and one workgroup (4 wavefronts) processes every shell quartet in it.
 //   Lanes of a wavefront hold fixed ket shell pairs (K,L) laid out as

@@ -319,11 +319,13 @@ struct PairGeom {
 
 // Integrals of one primitive quartet.  Calls f(ia, ib, ic, id, v) for every
 // Cartesian component (ia..id are std::integral_constant in the reference's
-// order); v already carries the primitive prefactor, contraction
+// order; ia restricted to [CA0, CA1) and ib to [CB0, CB1), by default all of
+// them); v already carries the primitive prefactor, contraction
 // coefficients and the component normalisation of basis.c:309-319.
 // Returns false when the primitive quartet falls under the prefactor screen
 // (its integrals are then passed as exact zeros).
-template <typename R, int LA, int LB, int LC, int LD, bool SCREEN = true, typename F>
+template <typename R, int LA, int LB, int LC, int LD, bool SCREEN = true, int CA0 = 0, int CA1 = ncart(LA),
+          int CB0 = 0, int CB1 = ncart(LB), typename F>
 __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const PrimPair &kp, const PairGeom<R> &bra,
                                                   const PairGeom<R> &ket, const RysTables &tab, R scale,
                                                   F &&f)
@@ -366,11 +368,13 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
 #pragma unroll
         for (int i = 1; i <= LD; ++i) cdp[a][i] = cdp[a][i - 1] * ket.AB[a];
     }
-    static_for<ncart(LA)>([&](auto ia) {
-        static_for<ncart(LB)>([&](auto ib) {
+    static_for<CA1 - CA0>([&](auto ia0) {
+        using ia_t = std::integral_constant<int, CA0 + decltype(ia0)::value>;
+        static_for<CB1 - CB0>([&](auto ib0) {
+            using ib_t = std::integral_constant<int, CB0 + decltype(ib0)::value>;
             static_for<ncart(LC)>([&](auto ic) {
                 static_for<ncart(LD)>([&](auto id) {
-                    constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+                    constexpr int IA = ia_t::value, IB = ib_t::value;
                     constexpr int IC = decltype(ic)::value, ID = decltype(id)::value;
                     constexpr int ax = cart_x(LA, IA), ay = cart_y(LA, IA), az = cart_z(LA, IA);
                     constexpr int bx = cart_x(LB, IB), by = cart_y(LB, IB), bz = cart_z(LB, IB);
@@ -386,7 +390,7 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
                     }
                     constexpr double nrm = 1.0 / csqrt(cart_dfprod(LA, IA) * cart_dfprod(LB, IB) *
                                                        cart_dfprod(LC, IC) * cart_dfprod(LD, ID));
-                    f(ia, ib, ic, id, v * R(nrm));
+                    f(ia_t{}, ib_t{}, ic, id, v * R(nrm));
                 });
             });
         });
@@ -398,7 +402,8 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
 // the contribution of each root separately (the integral is the sum over
 // roots, and every use of it here is linear), so only one root's 2-D tables
 // are live instead of all of them.
-template <typename R, int LA, int LB, int LC, int LD, bool BLOCK_FENCE, typename F>
+template <typename R, int LA, int LB, int LC, int LD, bool BLOCK_FENCE, int CA0 = 0, int CA1 = ncart(LA),
+          int CB0 = 0, int CB1 = ncart(LB), typename F>
 __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, const PrimPair &kp,
                                                            const PairGeom<R> &bra, const PairGeom<R> &ket,
                                                            const RysTables &tab, R scale, F &&f)
@@ -439,14 +444,16 @@ __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, c
         rys_vrr<R, L12, L34>(Gx, B00, B1, B1p, PA[0] - rq * PQ[0] * fff, QC[0] + rp * PQ[0] * fff, R(1));
         rys_vrr<R, L12, L34>(Gy, B00, B1, B1p, PA[1] - rq * PQ[1] * fff, QC[1] + rp * PQ[1] * fff, R(1));
         rys_vrr<R, L12, L34>(Gz, B00, B1, B1p, PA[2] - rq * PQ[2] * fff, QC[2] + rp * PQ[2] * fff, wr * sr);
-        static_for<ncart(LA)>([&](auto ia) {
-            static_for<ncart(LB)>([&](auto ib) {
+        static_for<CA1 - CA0>([&](auto ia0) {
+            using ia_t = std::integral_constant<int, CA0 + decltype(ia0)::value>;
+            static_for<CB1 - CB0>([&](auto ib0) {
+                using ib_t = std::integral_constant<int, CB0 + decltype(ib0)::value>;
                 // with BLOCK_FENCE the caller's density reads for this (a, b)
                 // block are issued here, not hoisted for all blocks at once
                 if constexpr (BLOCK_FENCE) __asm__ volatile("" ::: "memory");
                 static_for<ncart(LC)>([&](auto ic) {
                     static_for<ncart(LD)>([&](auto id) {
-                        constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+                        constexpr int IA = ia_t::value, IB = ib_t::value;
                         constexpr int IC = decltype(ic)::value, ID = decltype(id)::value;
                         constexpr int ax = cart_x(LA, IA), ay = cart_y(LA, IA), az = cart_z(LA, IA);
                         constexpr int bx = cart_x(LB, IB), by = cart_y(LB, IB), bz = cart_z(LB, IB);
@@ -457,7 +464,7 @@ __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, c
                         const R ix = rys_hrr<R, ax, bx, cx, dx, L12, L34, LB, LD>(Gx, abp[0], cdp[0]);
                         const R iy = rys_hrr<R, ay, by, cy, dy, L12, L34, LB, LD>(Gy, abp[1], cdp[1]);
                         const R iz = rys_hrr<R, az, bz, cz, dz, L12, L34, LB, LD>(Gz, abp[2], cdp[2]);
-                        f(ia, ib, ic, id, ix * iy * (iz * R(nrm)));
+                        f(ia_t{}, ib_t{}, ic, id, ix * iy * (iz * R(nrm)));
                     });
                 });
             });

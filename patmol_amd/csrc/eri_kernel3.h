@@ -35,7 +35,9 @@ struct K3 {
 
 // Per-class launch shape, from timing every class on (H2O)8 with both
 // digestion orders, with and without the density-read fence, at 1 and 2
-// waves per SIMD: code = fence * 8 + waves * 2 + rootwise.
+// waves per SIMD, with and without splitting the bra components over the
+// waves: code = (split_b - 1) * 32 + (split_a - 1) * 16 + fence * 8 +
+// waves * 2 + rootwise.
 constexpr int jk3_shape(int la, int lb, int lc, int ld)
 {
     const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
@@ -54,12 +56,12 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 66: return 1 * 2 + 0;   // (dp|ps)
     case 67: return 1 * 2 + 1;   // (dp|pp)
     case 69: return 1 * 2 + 1;   // (dp|ds)
-    case 70: return 8 + 1 * 2 + 1;   // (dp|dp)
+    case 70: return 16 + 8 + 1 * 2 + 1;   // (dp|dp)
     case 72: return 1 * 2 + 0;   // (dd|ss)
     case 75: return 1 * 2 + 1;   // (dd|ps)
-    case 76: return 8 + 1 * 2 + 1;   // (dd|pp)
-    case 78: return 8 + 1 * 2 + 1;   // (dd|ds)
-    case 79: return 8 + 1 * 2 + 1;   // (dd|dp)
+    case 76: return 16 + 8 + 1 * 2 + 1;   // (dd|pp)
+    case 78: return 32 + 8 + 1 * 2 + 1;   // (dd|ds)
+    case 79: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dp)
     default: return 1 * 2 + 1;
     }
 }
@@ -81,6 +83,26 @@ constexpr bool jk3_block_fence()
     return PM_K3_FENCE;
 #else
     return (jk3_shape(LA, LB, LC, LD) >> 3) & 1;
+#endif
+}
+
+template <int LA, int LB, int LC, int LD>
+constexpr int jk3_split_a()
+{
+#ifdef PM_K3_SPLIT_A
+    return (ncart(LA) % PM_K3_SPLIT_A == 0) ? PM_K3_SPLIT_A : 1;
+#else
+    return ((jk3_shape(LA, LB, LC, LD) >> 4) & 1) + 1;
+#endif
+}
+
+template <int LA, int LB, int LC, int LD>
+constexpr int jk3_split_b()
+{
+#ifdef PM_K3_SPLIT_B
+    return (ncart(LB) % PM_K3_SPLIT_B == 0) ? PM_K3_SPLIT_B : 1;
+#else
+    return ((jk3_shape(LA, LB, LC, LD) >> 5) & 1) + 1;
 #endif
 }
 
@@ -241,19 +263,31 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     // bra primitive pairs: contraction 1 is the common case, keep it in registers
     const PrimPair bp0 = a.pp[bref.pp0];
 
-    double jab[NA * NB];
+    // Bra-component split: SA x SB waves share every K, each taking a block
+    // of NAH bra-a components x NBH bra-b components of every quartet (and
+    // only that block's accumulators); without a split the four waves own
+    // different K.
+    constexpr int SA = jk3_split_a<LA, LB, LC, LD>(), SB = jk3_split_b<LA, LB, LC, LD>();
+    constexpr int SPLIT = SA * SB, KSTRIDE = 4 / SPLIT;
+    constexpr int NAH = NA / SA, NBH = NB / SB;
+    static_assert(NA % SA == 0 && NB % SB == 0 && 4 % SPLIT == 0, "bra component split");
+    const int sub = wave % SPLIT;
+    const int ha = sub % SA, hb = sub / SA;   // wave-uniform
+    const int a0 = ha * NAH, b0 = hb * NBH;
+
+    double jab[NAH * NBH];
 #pragma unroll
-    for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
+    for (int i = 0; i < NAH * NBH; ++i) jab[i] = 0.0;
     const double *__restrict__ DIJ = sD + oIJ + (ii * NA) * SJ + jj * NB;
 
-    for (int kq = wave; kq < TK; kq += 4) {
+    for (int kq = wave / SPLIT; kq < TK; kq += KSTRIDE) {
         if (kq >= nK) break;  // uniform
         const int K = fK + kq;
-        double kac[NA * NC], kbc[NB * NC];
+        double kac[NAH * NC], kbc[NBH * NC];
 #pragma unroll
-        for (int i = 0; i < NA * NC; ++i) kac[i] = 0.0;
+        for (int i = 0; i < NAH * NC; ++i) kac[i] = 0.0;
 #pragma unroll
-        for (int i = 0; i < NB * NC; ++i) kbc[i] = 0.0;
+        for (int i = 0; i < NBH * NC; ++i) kbc[i] = 0.0;
         for (int lc = 0; lc < NLSTEP; ++lc) {
             const int lq0 = lc * KSUB + ks;
             const bool lexist = lq0 < nL;
@@ -281,13 +315,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 kg.A[x] = R(sXK[kq][x]);
                 kg.AB[x] = R(sXK[kq][x] - sXL[lqs][x]);
             }
-            double jcd[NC * ND], kad[NA * ND], kbd[NB * ND];
+            double jcd[NC * ND], kad[NAH * ND], kbd[NBH * ND];
 #pragma unroll
             for (int i = 0; i < NC * ND; ++i) jcd[i] = 0.0;
 #pragma unroll
-            for (int i = 0; i < NA * ND; ++i) kad[i] = 0.0;
+            for (int i = 0; i < NAH * ND; ++i) kad[i] = 0.0;
 #pragma unroll
-            for (int i = 0; i < NB * ND; ++i) kbd[i] = 0.0;
+            for (int i = 0; i < NBH * ND; ++i) kbd[i] = 0.0;
             const double *__restrict__ DKL = sD + oKL + (kq * NC) * SL + lq * ND;
             const double *__restrict__ DJL = sD + oJL + (jj * NB) * SL + lq * ND;
             const double *__restrict__ DIL = sD + oIL + (ii * NA) * SL + lq * ND;
@@ -295,30 +329,53 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             const double *__restrict__ DIK = sD + oIK + (ii * NA) * SK + kq * NC;
             const int nkp = sKmax;  // uniform bound of the ket primitive loop
             const PrimPair *kpp = staged ? (sPP + sPPoff[t]) : (a.pp + kref.pp0);
-            auto digest = [&](auto ia, auto ib_, auto ic, auto id, R vr) {
-                constexpr int A_ = decltype(ia)::value, B_ = decltype(ib_)::value;
-                constexpr int C_ = decltype(ic)::value, D_ = decltype(id)::value;
-                const double v = double(vr);
-                jab[A_ * NB + B_] += v * DKL[C_ * SL + D_];
-                jcd[C_ * ND + D_] += v * DIJ[A_ * SJ + B_];
-                kac[A_ * NC + C_] += v * DJL[B_ * SL + D_];
-                kad[A_ * ND + D_] += v * DJK[B_ * SK + C_];
-                kbc[B_ * NC + C_] += v * DIL[A_ * SL + D_];
-                kbd[B_ * ND + D_] += v * DIK[A_ * SK + C_];
-            };
-            for (int ib = 0; ib < nbp; ++ib) {
-                const PrimPair bp = (ib == 0) ? bp0 : a.pp[bref.pp0 + min(ib, nbp_lane - 1)];
-                for (int ik = 0; ik < nkp; ++ik) {
-                    const PrimPair kp = kpp[min(ik, kref.npp - 1)];
-                    const R sc = (ib < nbp_lane && ik < kref.npp) ? scale : R(0);
-                    if constexpr (jk3_rootwise<LA, LB, LC, LD>())
-                        primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD>()>(
-                            bp, kp, bg, kg, rt, sc, digest);
-                    else
-                        primitive_quartet<R, LA, LB, LC, LD>(bp, kp, bg, kg, rt, sc, digest);
+            // the primitive loops for one (a, b) component block
+            auto run_block = [&](auto HA, auto HB) {
+                constexpr int A0 = decltype(HA)::value * NAH, B0 = decltype(HB)::value * NBH;
+                auto digest = [&](auto ia, auto ib_, auto ic, auto id, R vr) {
+                    constexpr int A_ = decltype(ia)::value, B_ = decltype(ib_)::value;
+                    constexpr int C_ = decltype(ic)::value, D_ = decltype(id)::value;
+                    constexpr int Al = A_ - A0, Bl = B_ - B0;
+                    const double v = double(vr);
+                    jab[Al * NBH + Bl] += v * DKL[C_ * SL + D_];
+                    jcd[C_ * ND + D_] += v * DIJ[A_ * SJ + B_];
+                    kac[Al * NC + C_] += v * DJL[B_ * SL + D_];
+                    kad[Al * ND + D_] += v * DJK[B_ * SK + C_];
+                    kbc[Bl * NC + C_] += v * DIL[A_ * SL + D_];
+                    kbd[Bl * ND + D_] += v * DIK[A_ * SK + C_];
+                };
+                for (int ib = 0; ib < nbp; ++ib) {
+                    const PrimPair bp = (ib == 0) ? bp0 : a.pp[bref.pp0 + min(ib, nbp_lane - 1)];
+                    for (int ik = 0; ik < nkp; ++ik) {
+                        const PrimPair kp = kpp[min(ik, kref.npp - 1)];
+                        const R sc = (ib < nbp_lane && ik < kref.npp) ? scale : R(0);
+                        if constexpr (jk3_rootwise<LA, LB, LC, LD>())
+                            primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD>(), A0,
+                                                       A0 + NAH, B0, B0 + NBH>(bp, kp, bg, kg, rt, sc, digest);
+                        else
+                            primitive_quartet<R, LA, LB, LC, LD, true, A0, A0 + NAH, B0, B0 + NBH>(bp, kp, bg, kg, rt,
+                                                                                               sc, digest);
+                    }
                 }
+            };
+            using Z = std::integral_constant<int, 0>;
+            using O = std::integral_constant<int, 1>;
+            if constexpr (SPLIT == 1) {
+                run_block(Z{}, Z{});
+            } else if constexpr (SA == 2 && SB == 1) {
+                if (ha == 0) run_block(Z{}, Z{});
+                else run_block(O{}, Z{});
+            } else if constexpr (SA == 1 && SB == 2) {
+                if (hb == 0) run_block(Z{}, Z{});
+                else run_block(Z{}, O{});
+            } else {
+                if (ha == 0 && hb == 0) run_block(Z{}, Z{});
+                else if (ha == 1 && hb == 0) run_block(O{}, Z{});
+                else if (ha == 0) run_block(Z{}, O{});
+                else run_block(O{}, O{});
             }
-            // per-step results: J'(KL) (one wave owns K), K'(IL), K'(JL) (shared by waves)
+            // per-step results: J'(KL) (owned by the waves of this K),
+            // K'(IL), K'(JL) (shared by waves)
             lane_reduce_n<MASK_I | MASK_J>(jcd);
             lane_reduce_n<MASK_J>(kad);
             lane_reduce_n<MASK_I>(kbd);
@@ -326,52 +383,68 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) sO[oKL + (kq * NC + c) * SL + lq * ND + d] += 2.0 * jcd[c * ND + d];
+                    for (int d = 0; d < ND; ++d) {
+                        double *o = &sO[oKL + (kq * NC + c) * SL + lq * ND + d];
+                        if constexpr (SPLIT == 1) *o += 2.0 * jcd[c * ND + d];
+                        else if (jcd[c * ND + d] != 0.0)
+                            __hip_atomic_fetch_add(o, 2.0 * jcd[c * ND + d], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
             }
             if ((lane & MASK_J) == 0 && iexist && lexist) {
 #pragma unroll
-                for (int x = 0; x < NA; ++x)
+                for (int x = 0; x < NAH; ++x)
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
                         if (kad[x * ND + d] != 0.0)
-                            __hip_atomic_fetch_add(&sO[oIL + (ii * NA + x) * SL + lq * ND + d], kad[x * ND + d],
+                            __hip_atomic_fetch_add(&sO[oIL + (ii * NA + a0 + x) * SL + lq * ND + d], kad[x * ND + d],
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             if ((lane & MASK_I) == 0 && jexist && lexist) {
 #pragma unroll
-                for (int b = 0; b < NB; ++b)
+                for (int b = 0; b < NBH; ++b)
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
                         if (kbd[b * ND + d] != 0.0)
-                            __hip_atomic_fetch_add(&sO[oJL + (jj * NB + b) * SL + lq * ND + d], kbd[b * ND + d],
+                            __hip_atomic_fetch_add(&sO[oJL + (jj * NB + b0 + b) * SL + lq * ND + d], kbd[b * ND + d],
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        // per-K results: K'(IK), K'(JK) (one wave owns K)
+        // per-K results: K'(IK), K'(JK) (owned by the waves of this K)
         lane_reduce_n<MASK_J | MASK_S>(kac);
         lane_reduce_n<MASK_I | MASK_S>(kbc);
         if ((lane & (MASK_J | MASK_S)) == 0 && iexist) {
 #pragma unroll
-            for (int x = 0; x < NA; ++x)
+            for (int x = 0; x < NAH; ++x)
 #pragma unroll
-                for (int c = 0; c < NC; ++c) sO[oIK + (ii * NA + x) * SK + kq * NC + c] += kac[x * NC + c];
+                for (int c = 0; c < NC; ++c) {
+                    double *o = &sO[oIK + (ii * NA + a0 + x) * SK + kq * NC + c];
+                    if constexpr (SB == 1) *o += kac[x * NC + c];
+                    else if (kac[x * NC + c] != 0.0)
+                        __hip_atomic_fetch_add(o, kac[x * NC + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
         }
         if ((lane & (MASK_I | MASK_S)) == 0 && jexist) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b)
+            for (int b = 0; b < NBH; ++b)
 #pragma unroll
-                for (int c = 0; c < NC; ++c) sO[oJK + (jj * NB + b) * SK + kq * NC + c] += kbc[b * NC + c];
+                for (int c = 0; c < NC; ++c) {
+                    double *o = &sO[oJK + (jj * NB + b0 + b) * SK + kq * NC + c];
+                    if constexpr (SA == 1) *o += kbc[b * NC + c];
+                    else if (kbc[b * NC + c] != 0.0)
+                        __hip_atomic_fetch_add(o, kbc[b * NC + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
         }
     }
-    // per-item result: J'(IJ) (shared by the four waves)
+    // per-item result: J'(IJ) (shared by the waves of different K)
     lane_reduce_n<MASK_S>(jab);
     if ((lane & MASK_S) == 0 && iexist && jexist) {
 #pragma unroll
-        for (int x = 0; x < NA; ++x)
+        for (int x = 0; x < NAH; ++x)
 #pragma unroll
-            for (int b = 0; b < NB; ++b)
-                if (jab[x * NB + b] != 0.0)
-                    __hip_atomic_fetch_add(&sO[oIJ + (ii * NA + x) * SJ + jj * NB + b], 2.0 * jab[x * NB + b],
+            for (int b = 0; b < NBH; ++b)
+                if (jab[x * NBH + b] != 0.0)
+                    __hip_atomic_fetch_add(&sO[oIJ + (ii * NA + a0 + x) * SJ + jj * NB + b0 + b], 2.0 * jab[x * NBH + b],
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();

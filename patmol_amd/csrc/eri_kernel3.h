@@ -167,7 +167,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     const bool sameBra = it.x == it.y, sameKet = it.z == it.w, samePair = it.x == it.z && it.y == it.w;
     const int N = a.N;
 
-    // ---- stage: function offsets, ket pairs, density blocks, zero outputs ----
+    // ---- stage A: function offsets, centres, ket pairs, Rys table ----
     if (tid < TI) sOff[tid] = a.shell_off[fI + (tid < nI ? tid : 0)];
     else if (tid < TI + TJ) sOff[tid] = a.shell_off[fJ + (tid - TI < nJ ? tid - TI : 0)];
     else if (tid < TI + TJ + TK) sOff[tid] = a.shell_off[fK + (tid - TI - TJ < nK ? tid - TI - TJ : 0)];
@@ -179,18 +179,45 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         const int l = (tid - 64) / 3, x = (tid - 64) % 3;
         sXL[l][x] = a.shell_xyz[3 * (fL + (l < nL ? l : 0)) + x];
     }
-    if (tid >= 128 && tid < 128 + KLP) {
-        const int t = tid - 128, kq = t / TL, lq = t % TL;
+    if (wave == 2) {
+        // one lane per ket pair: reference, bound, validity, and (when the
+        // primitive pairs of the whole ket tile pair fit) the primitive pairs
+        const int t = lane, kq = t / TL, lq = t % TL;
+        const bool here = t < KLP;
         const bool kex = kq < nK, lex = lq < nL;
         const int K = fK + (kex ? kq : 0), L = fL + (lex ? lq : 0);
-        const bool ok = kex && lex && (!sameKet || K >= L);
+        const bool ok = here && kex && lex && (!sameKet || K >= L);
         // a structurally valid stand-in pair for lanes whose (K, L) does not exist
         const int Ls = ok ? L : (sameKet ? K : fL);
         const int Kp = K >= Ls ? K : Ls, Lp = K >= Ls ? Ls : K;
         const int pk = Kp * (Kp + 1) / 2 + Lp;
-        sKref[t] = a.pairs[pk];
-        sKq[t] = a.pair_q[pk];
-        sKok[t] = ok ? 1 : 0;
+        PairRef r = {0, 0};
+        if (here) {
+            r = a.pairs[pk];
+            sKref[t] = r;
+            sKq[t] = a.pair_q[pk];
+            sKok[t] = ok ? 1 : 0;
+        }
+        // exclusive prefix sum and maximum of the primitive counts
+        int incl = r.npp, mx = r.npp;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += u;
+            mx = max(mx, __shfl_xor(mx, d, 64));
+        }
+        const int tot = __shfl(incl, 63, 64);
+        const int off = incl - r.npp;
+        const bool staged_w = tot <= S::PPCAP;
+        if (here) {
+            sPPoff[t] = off;
+            if (staged_w)
+                for (int p = 0; p < r.npp; ++p) sPP[off + p] = a.pp[r.pp0 + p];
+        }
+        if (lane == 0) {
+            sStaged = staged_w ? 1 : 0;
+            sKmax = max(mx, 1);
+        }
     }
     if constexpr (RYS_LDS) {
         const double2 *__restrict__ src = reinterpret_cast<const double2 *>(a.rys.cheb);
@@ -199,46 +226,32 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     }
     RysTables rt = a.rys;
     if constexpr (RYS_LDS) rt.cheb = sRys;
+    __syncthreads();
+    // ---- stage B: density blocks of the tile quartet, zeroed output blocks ----
     {
-        // one block of the tile quartet: rows of tile (rf, rn) with NRc
-        // components per shell, columns of tile (cf, cn) with NCc components
-        auto gather = [&](int o, int rows, int stride, int rf, int rn, int nrc, int cf, int cn, int ncc,
+        // one block: rows of tile (rsh: first entry in sOff, rn shells, nrc
+        // components each), columns of tile (csh, cn, ncc)
+        auto gather = [&](int o, int rows, int stride, int rsh, int rn, int nrc, int csh, int cn, int ncc,
                           const double *__restrict__ src) {
             for (int e = tid; e < rows * stride; e += 256) {
                 const int r = e / stride, c = e % stride;
                 const int rs = r / nrc, cs = c / ncc;
                 const bool ex = c < cn * ncc && rs < rn && cs < cn;
                 double v = 0.0;
-                if (ex) v = src[(size_t)(a.shell_off[rf + rs] + r % nrc) * N + a.shell_off[cf + cs] + c % ncc];
+                if (ex) v = src[(size_t)(sOff[rsh + rs] + r % nrc) * N + sOff[csh + cs] + c % ncc];
                 sD[o + e] = v;
                 sO[o + e] = 0.0;
             }
         };
-        gather(oIJ, FI, SJ, fI, nI, NA, fJ, nJ, NB, a.DJ);
-        gather(oKL, FK, SL, fK, nK, NC, fL, nL, ND, a.DJ);
-        gather(oIK, FI, SK, fI, nI, NA, fK, nK, NC, a.DK0);
-        gather(oIL, FI, SL, fI, nI, NA, fL, nL, ND, a.DK0);
-        gather(oJK, FJ, SK, fJ, nJ, NB, fK, nK, NC, a.DK0);
-        gather(oJL, FJ, SL, fJ, nJ, NB, fL, nL, ND, a.DK0);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int tot = 0, mx = 1;
-        for (int t = 0; t < KLP; ++t) {
-            sPPoff[t] = tot;
-            tot += sKref[t].npp;
-            mx = max(mx, sKref[t].npp);
-        }
-        sStaged = tot <= S::PPCAP ? 1 : 0;
-        sKmax = mx;
+        gather(oIJ, FI, SJ, 0, nI, NA, TI, nJ, NB, a.DJ);
+        gather(oKL, FK, SL, TI + TJ, nK, NC, TI + TJ + TK, nL, ND, a.DJ);
+        gather(oIK, FI, SK, 0, nI, NA, TI + TJ, nK, NC, a.DK0);
+        gather(oIL, FI, SL, 0, nI, NA, TI + TJ + TK, nL, ND, a.DK0);
+        gather(oJK, FJ, SK, TI, nJ, NB, TI + TJ, nK, NC, a.DK0);
+        gather(oJL, FJ, SL, TI, nJ, NB, TI + TJ + TK, nL, ND, a.DK0);
     }
     __syncthreads();
     const bool staged = sStaged != 0;
-    if (staged && tid < KLP) {
-        const PairRef r = sKref[tid];
-        for (int p = 0; p < r.npp; ++p) sPP[sPPoff[tid] + p] = a.pp[r.pp0 + p];
-    }
-    __syncthreads();
 
     // ---- bra pair (fixed per lane) ----
     const bool iexist = ii < nI, jexist = jj < nJ;

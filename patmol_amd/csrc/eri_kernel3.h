@@ -116,6 +116,19 @@ constexpr int jk3_waves_per_eu()
 #endif
 }
 
+// Cross-lane sums: lane bits 0-3 by DPP inside rows of 16; bits 4-5 (other
+// rows) are left to LDS atomics from one leader per row, which costs less
+// than the permlane swaps for the few values a step produces.
+constexpr int red_lo(int mask)
+{
+#ifdef PM_K3_FULLREDUCE
+    return mask;
+#else
+    return mask & 15;
+#endif
+}
+constexpr bool red_hi(int mask) { return (mask & ~red_lo(mask)) != 0; }
+
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(jk3_waves_per_eu<LA, LB, LC, LD>()))) void jk_kernel3(JKArgs a)
 {
@@ -388,23 +401,24 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 else run_block(O{}, O{});
             }
             // per-step results: J'(KL) (owned by the waves of this K),
-            // K'(IL), K'(JL) (shared by waves)
-            lane_reduce_n<MASK_I | MASK_J>(jcd);
-            lane_reduce_n<MASK_J>(kad);
-            lane_reduce_n<MASK_I>(kbd);
-            if ((lane & (MASK_I | MASK_J)) == 0 && lexist) {
+            // K'(IL), K'(JL) (shared by waves).  Lane bits 0-3 are summed
+            // with DPP; lanes that differ in bits 4-5 add into LDS.
+            lane_reduce_n<red_lo(MASK_I | MASK_J)>(jcd);
+            lane_reduce_n<red_lo(MASK_J)>(kad);
+            lane_reduce_n<red_lo(MASK_I)>(kbd);
+            if ((lane & red_lo(MASK_I | MASK_J)) == 0 && lexist) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
 #pragma unroll
                     for (int d = 0; d < ND; ++d) {
                         double *o = &sO[oKL + (kq * NC + c) * SL + lq * ND + d];
-                        if constexpr (SPLIT == 1) *o += 2.0 * jcd[c * ND + d];
+                        if constexpr (SPLIT == 1 && !red_hi(MASK_I | MASK_J)) *o += 2.0 * jcd[c * ND + d];
                         else if (jcd[c * ND + d] != 0.0)
                             __hip_atomic_fetch_add(o, 2.0 * jcd[c * ND + d], __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
             }
-            if ((lane & MASK_J) == 0 && iexist && lexist) {
+            if ((lane & red_lo(MASK_J)) == 0 && iexist && lexist) {
 #pragma unroll
                 for (int x = 0; x < NAH; ++x)
 #pragma unroll
@@ -413,7 +427,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                             __hip_atomic_fetch_add(&sO[oIL + (ii * NA + a0 + x) * SL + lq * ND + d], kad[x * ND + d],
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            if ((lane & MASK_I) == 0 && jexist && lexist) {
+            if ((lane & red_lo(MASK_I)) == 0 && jexist && lexist) {
 #pragma unroll
                 for (int b = 0; b < NBH; ++b)
 #pragma unroll
@@ -424,34 +438,34 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             }
         }
         // per-K results: K'(IK), K'(JK) (owned by the waves of this K)
-        lane_reduce_n<MASK_J | MASK_S>(kac);
-        lane_reduce_n<MASK_I | MASK_S>(kbc);
-        if ((lane & (MASK_J | MASK_S)) == 0 && iexist) {
+        lane_reduce_n<red_lo(MASK_J | MASK_S)>(kac);
+        lane_reduce_n<red_lo(MASK_I | MASK_S)>(kbc);
+        if ((lane & red_lo(MASK_J | MASK_S)) == 0 && iexist) {
 #pragma unroll
             for (int x = 0; x < NAH; ++x)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
                     double *o = &sO[oIK + (ii * NA + a0 + x) * SK + kq * NC + c];
-                    if constexpr (SB == 1) *o += kac[x * NC + c];
+                    if constexpr (SB == 1 && !red_hi(MASK_J | MASK_S)) *o += kac[x * NC + c];
                     else if (kac[x * NC + c] != 0.0)
                         __hip_atomic_fetch_add(o, kac[x * NC + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
         }
-        if ((lane & (MASK_I | MASK_S)) == 0 && jexist) {
+        if ((lane & red_lo(MASK_I | MASK_S)) == 0 && jexist) {
 #pragma unroll
             for (int b = 0; b < NBH; ++b)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
                     double *o = &sO[oJK + (jj * NB + b0 + b) * SK + kq * NC + c];
-                    if constexpr (SA == 1) *o += kbc[b * NC + c];
+                    if constexpr (SA == 1 && !red_hi(MASK_I | MASK_S)) *o += kbc[b * NC + c];
                     else if (kbc[b * NC + c] != 0.0)
                         __hip_atomic_fetch_add(o, kbc[b * NC + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
         }
     }
     // per-item result: J'(IJ) (shared by the waves of different K)
-    lane_reduce_n<MASK_S>(jab);
-    if ((lane & MASK_S) == 0 && iexist && jexist) {
+    lane_reduce_n<red_lo(MASK_S)>(jab);
+    if ((lane & red_lo(MASK_S)) == 0 && iexist && jexist) {
 #pragma unroll
         for (int x = 0; x < NAH; ++x)
 #pragma unroll

@@ -10,7 +10,8 @@ library's device entry point.  The step is timed in FP64 and in mixed
 precision (work items whose Schwarz bound times max|P| is below 1e-6 run in
 FP32, accumulated in FP64); |dE| of the mixed build against the FP64 build
 is reported.  value = surviving shell quartets per second of the mixed
-build (of the FP64 build if the mixed one misses |dE| <= 1e-6).
+build (of the FP64 build if the mixed one misses |dE| <= 1e-6, runs no FP32
+work item or is not faster).
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
        python bench.py --calibrate-copy BYTES   (plain device copy of BYTES,
@@ -216,7 +217,7 @@ def main():
     ms_host = (time.perf_counter() - t0) * 1e3 / nh
     log(f"host-array entry (rhf_form_gmatrix, PCIe included): {ms_host:.3f} ms/build")
 
-    use_mixed = dE <= 1e-6 and stm["work_items_fp32"] > 0
+    use_mixed = dE <= 1e-6 and stm["work_items_fp32"] > 0 and msmx < ms64
     ms = msmx if use_mixed else ms64
     kept = st["shell_quartets_kept"]
     value = kept / (ms * 1e-3)

@@ -27,8 +27,8 @@ namespace pm {
 template <int K>
 __device__ __forceinline__ double dpp_row_shl(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x100 | (1 << K), 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x100 | (1 << K), 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x100 | (1 << K), 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x100 | (1 << K), 0xf, 0xf, false);
     return __hiloint2double(hi, lo);
 }
 

@@ -1,7 +1,8 @@
 // eri_engine.h -- Rys-quadrature electron-repulsion integrals for gfx950.
 //
 // Device building blocks shared by the J/K, ERI-export and Schwarz kernels:
-//   * Rys nodes/weights from piecewise-Chebyshev tables (rys_table.inc),
+//   * Rys nodes/weights from piecewise-Chebyshev tables (rys_table.inc,
+//     converted to monomials in u when uploaded and evaluated by Horner),
 //     evaluated in registers per primitive quartet (rys.c:2217-2243 computes
 //     the same quadrature with its own fits);
 //   * the 2-D recursion G[i][j] (rys.c:2269-2292) and the binomial transfer to
@@ -32,8 +33,8 @@ constexpr int rys_off(int n)
 }
 
 struct RysTables {
-    const double *cheb;    // double coefficients
-    const float *cheb32;   // the same, rounded to float
+    const double *cheb;    // monomial coefficients in u of the Chebyshev fits
+    const float *cheb32;   // the same from the first 9 Chebyshev terms, in float
     const double *herm_r;  // large-T nodes r_i (x = r_i / T), offset n(n-1)/2
     const double *herm_h;  // large-T weights h_i (w = h_i / sqrt(T))
 };
@@ -136,27 +137,20 @@ __device__ __forceinline__ void rys_eval(double T, double (&x)[N], double (&w)[N
 {
     if (T < (double)rys_tmax(N)) {
         int k = (int)T;
-        double u = 2.0 * (T - k) - 1.0, u2 = 2.0 * u;
+        const double u = 2.0 * (T - k) - 1.0;
         const double *__restrict__ c = tb.cheb + rys_off(N) + k * (RYS_NC * 2 * N);
-        double b1[2 * N], b2[2 * N];
+        double y[2 * N];
 #pragma unroll
-        for (int v = 0; v < 2 * N; ++v) {
-            b1[v] = 0.0;
-            b2[v] = 0.0;
-        }
+        for (int v = 0; v < 2 * N; ++v) y[v] = c[(RYS_NC - 1) * 2 * N + v];
 #pragma unroll
-        for (int j = RYS_NC - 1; j >= 1; --j) {
+        for (int j = RYS_NC - 2; j >= 0; --j) {
 #pragma unroll
-            for (int v = 0; v < 2 * N; ++v) {
-                double t = fma(u2, b1[v], c[j * 2 * N + v] - b2[v]);
-                b2[v] = b1[v];
-                b1[v] = t;
-            }
+            for (int v = 0; v < 2 * N; ++v) y[v] = fma(y[v], u, c[j * 2 * N + v]);
         }
 #pragma unroll
         for (int v = 0; v < N; ++v) {
-            x[v] = fma(u, b1[v], c[v] - b2[v]);
-            w[v] = fma(u, b1[N + v], c[N + v] - b2[N + v]);
+            x[v] = y[v];
+            w[v] = y[N + v];
         }
     } else {
         double it = 1.0 / T, rs = 1.0 / sqrt(T);
@@ -176,27 +170,20 @@ __device__ __forceinline__ void rys_eval(float T, float (&x)[N], float (&w)[N], 
     constexpr int NC32 = 9;
     if (T < (float)rys_tmax(N)) {
         int k = (int)T;
-        float u = 2.0f * (T - (float)k) - 1.0f, u2 = 2.0f * u;
+        const float u = 2.0f * (T - (float)k) - 1.0f;
         const float *__restrict__ c = tb.cheb32 + rys_off(N) + k * (RYS_NC * 2 * N);
-        float b1[2 * N], b2[2 * N];
+        float y[2 * N];
 #pragma unroll
-        for (int v = 0; v < 2 * N; ++v) {
-            b1[v] = 0.0f;
-            b2[v] = 0.0f;
-        }
+        for (int v = 0; v < 2 * N; ++v) y[v] = c[(NC32 - 1) * 2 * N + v];
 #pragma unroll
-        for (int j = NC32 - 1; j >= 1; --j) {
+        for (int j = NC32 - 2; j >= 0; --j) {
 #pragma unroll
-            for (int v = 0; v < 2 * N; ++v) {
-                float t = fmaf(u2, b1[v], c[j * 2 * N + v] - b2[v]);
-                b2[v] = b1[v];
-                b1[v] = t;
-            }
+            for (int v = 0; v < 2 * N; ++v) y[v] = fmaf(y[v], u, c[j * 2 * N + v]);
         }
 #pragma unroll
         for (int v = 0; v < N; ++v) {
-            x[v] = fmaf(u, b1[v], c[v] - b2[v]);
-            w[v] = fmaf(u, b1[N + v], c[N + v] - b2[N + v]);
+            x[v] = y[v];
+            w[v] = y[N + v];
         }
     } else {
         float it = 1.0f / T, rs = 1.0f / sqrtf(T);
@@ -215,20 +202,16 @@ __device__ __forceinline__ void rys_eval_one(double T, int r, double &x, double 
 {
     if (T < (double)rys_tmax(N)) {
         int k = (int)T;
-        double u = 2.0 * (T - k) - 1.0, u2 = 2.0 * u;
+        const double u = 2.0 * (T - k) - 1.0;
         const double *__restrict__ c = tb.cheb + rys_off(N) + k * (RYS_NC * 2 * N) + r;
-        double bx1 = 0.0, bx2 = 0.0, bw1 = 0.0, bw2 = 0.0;
+        double yx = c[(RYS_NC - 1) * 2 * N], yw = c[(RYS_NC - 1) * 2 * N + N];
 #pragma unroll
-        for (int j = RYS_NC - 1; j >= 1; --j) {
-            double tx = fma(u2, bx1, c[j * 2 * N] - bx2);
-            double tw = fma(u2, bw1, c[j * 2 * N + N] - bw2);
-            bx2 = bx1;
-            bx1 = tx;
-            bw2 = bw1;
-            bw1 = tw;
+        for (int j = RYS_NC - 2; j >= 0; --j) {
+            yx = fma(yx, u, c[j * 2 * N]);
+            yw = fma(yw, u, c[j * 2 * N + N]);
         }
-        x = fma(u, bx1, c[0] - bx2);
-        w = fma(u, bw1, c[N] - bw2);
+        x = yx;
+        w = yw;
     } else {
         x = tb.herm_r[N * (N - 1) / 2 + r] / T;
         w = tb.herm_h[N * (N - 1) / 2 + r] / sqrt(T);
@@ -241,20 +224,16 @@ __device__ __forceinline__ void rys_eval_one(float T, int r, float &x, float &w,
     constexpr int NC32 = 9;
     if (T < (float)rys_tmax(N)) {
         int k = (int)T;
-        float u = 2.0f * (T - (float)k) - 1.0f, u2 = 2.0f * u;
+        const float u = 2.0f * (T - (float)k) - 1.0f;
         const float *__restrict__ c = tb.cheb32 + rys_off(N) + k * (RYS_NC * 2 * N) + r;
-        float bx1 = 0.0f, bx2 = 0.0f, bw1 = 0.0f, bw2 = 0.0f;
+        float yx = c[(NC32 - 1) * 2 * N], yw = c[(NC32 - 1) * 2 * N + N];
 #pragma unroll
-        for (int j = NC32 - 1; j >= 1; --j) {
-            float tx = fmaf(u2, bx1, c[j * 2 * N] - bx2);
-            float tw = fmaf(u2, bw1, c[j * 2 * N + N] - bw2);
-            bx2 = bx1;
-            bx1 = tx;
-            bw2 = bw1;
-            bw1 = tw;
+        for (int j = NC32 - 2; j >= 0; --j) {
+            yx = fmaf(yx, u, c[j * 2 * N]);
+            yw = fmaf(yw, u, c[j * 2 * N + N]);
         }
-        x = fmaf(u, bx1, c[0] - bx2);
-        w = fmaf(u, bw1, c[N] - bw2);
+        x = yx;
+        w = yw;
     } else {
         x = (float)tb.herm_r[N * (N - 1) / 2 + r] / T;
         w = (float)tb.herm_h[N * (N - 1) / 2 + r] / sqrtf(T);

@@ -9,6 +9,7 @@
  *   rhf_form_gmatrix   replaces scfaux.h:22-23  (scfaux.c:428-552)
  *   uhf_form_gmatrix   replaces scfaux.h:24-26  (scfaux.c:554-714)
  *   dpm_ints           replaces dpmints.h:12    (dpmints.c:120-199)
+ *   one_electron_ints  replaces oneints.h      (oneints.c:137-234)
  * Instead of writing ERINT files, two_electron_ints builds the device-side
  * shell-pair and screened work tables for the basis current at the call;
  * rhf_form_gmatrix/uhf_form_gmatrix then run a direct (recompute) J/K build
@@ -42,6 +43,9 @@ void rhf_form_gmatrix(const ERIFS *tfiles, const double *Pmat, double *Gmat);
 void uhf_form_gmatrix(const ERIFS *tfiles, const double *PmatA, const double *PmatB,
                       double *GmatA, double *GmatB);
 void dpm_ints(double *hmat);
+/* S, T and H = T + V of the current basis into caller-owned packed no2
+ * arrays (assigned, not accumulated); V leaves out skip_center(). */
+void one_electron_ints(double *Smat, double *Tmat, double *Hmat);
 
 /* ---- status ---- */
 #define PATMOL_HIP_OK 0

@@ -83,7 +83,7 @@ def lib():
         H.number_of_shells.restype = C.c_int
         H.nuclear_repulsion.restype = C.c_double
         H.set_center_position.argtypes = [C.c_int, _dp]
-        H.one_electron_ints.argtypes = [_dp, _dp, _dp]
+        H.patmol_host_one_electron_ints.argtypes = [_dp, _dp, _dp]
         H.patmol_rhf_scf.argtypes = [_dp, _dp, C.POINTER(ScfResult)]
         H.patmol_uhf_scf.argtypes = [_dp, _dp, C.POINTER(ScfResult)]
         G.init_ERIFS.restype = C.c_void_p
@@ -93,6 +93,7 @@ def lib():
         G.rhf_form_gmatrix.argtypes = [C.c_void_p, _dp, _dp]
         G.uhf_form_gmatrix.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         G.dpm_ints.argtypes = [_dp]
+        G.one_electron_ints.argtypes = [_dp, _dp, _dp]
         G.patmol_hip_last_error.restype = C.c_int
         G.patmol_hip_error_string.restype = C.c_char_p
         G.patmol_hip_set_strict.argtypes = [C.c_int]
@@ -127,7 +128,7 @@ def _check(what: str = ""):
 def exported_symbols():
     """Names the C ABI declares (include/patmol_hip.h)."""
     return ["init_ERIFS", "clean_ERIFS", "two_electron_ints", "rhf_form_gmatrix", "uhf_form_gmatrix",
-            "dpm_ints", "patmol_hip_last_error", "patmol_hip_error_string", "patmol_hip_set_strict",
+            "dpm_ints", "one_electron_ints", "patmol_hip_last_error", "patmol_hip_error_string", "patmol_hip_set_strict",
             "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision",
             "patmol_hip_rhf_gmatrix_device", "patmol_hip_stats", "patmol_hip_class_stats",
             "patmol_hip_pair_schwarz",
@@ -159,11 +160,23 @@ def set_options(tau: Optional[float] = None, precision: Optional[int] = None,
 
 
 def one_electron_ints():
+    """S, T, H (packed) of the current basis, computed on the GPU."""
+    H, G = lib()
+    n = H.number_of_orbitals()
+    no2 = n * (n + 1) // 2
+    S, T, Hm = np.zeros(no2), np.zeros(no2), np.zeros(no2)
+    G.one_electron_ints(S, T, Hm)
+    _check("one_electron_ints")
+    return S, T, Hm
+
+
+def host_one_electron_ints():
+    """The same integrals from the host library's CPU code."""
     H, _ = lib()
     n = H.number_of_orbitals()
     no2 = n * (n + 1) // 2
     S, T, Hm = np.zeros(no2), np.zeros(no2), np.zeros(no2)
-    H.one_electron_ints(S, T, Hm)
+    H.patmol_host_one_electron_ints(S, T, Hm)
     return S, T, Hm
 
 

@@ -97,8 +97,8 @@ int launch_dpm(int la, int lb, const DpmArgs &args, hipStream_t s)
 }
 
 // defined in hip_context.cpp
-int dpm_fail(int code, const char *msg);
-RysTables dpm_tables();
+int aux_fail(int code, const char *msg);
+RysTables aux_tables();
 
 }  // namespace pm
 
@@ -109,16 +109,16 @@ extern "C" void dpm_ints(double *hmat)
     const Center *ce = centers();
     const int ns = number_of_shells(), sc = skip_center();
     if (!sh || !ce || ns <= 0 || !hmat) {
-        dpm_fail(PATMOL_HIP_EBADARG, "dpm_ints: no basis or null hmat");
+        aux_fail(PATMOL_HIP_EBADARG, "dpm_ints: no basis or null hmat");
         return;
     }
-    RysTables rys = dpm_tables();
+    RysTables rys = aux_tables();
     if (!rys.cheb) return;
     const double *Q = ce[sc].r;
     std::vector<double> xyz(3 * ns);
     for (int s = 0; s < ns; ++s) {
         if (sh[s].lsh > 2) {
-            dpm_fail(PATMOL_HIP_EUNSUPPORTED, "dpm_ints: l > 2");
+            aux_fail(PATMOL_HIP_EUNSUPPORTED, "dpm_ints: l > 2");
             return;
         }
         for (int x = 0; x < 3; ++x) xyz[3 * s + x] = ce[sh[s].cen].r[x];
@@ -182,7 +182,7 @@ extern "C" void dpm_ints(double *hmat)
             (void)hipFree(dref);
             (void)hipFree(dpp);
             if (!ok) {
-                dpm_fail(PATMOL_HIP_EDEVICE, "dpm_ints: device failure");
+                aux_fail(PATMOL_HIP_EDEVICE, "dpm_ints: device failure");
                 return;
             }
             const int nb = (lb + 1) * (lb + 2) / 2, na = (la + 1) * (la + 2) / 2;

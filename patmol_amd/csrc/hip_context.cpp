@@ -607,9 +607,9 @@ static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
     return 0;
 }
 
-int dpm_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+int aux_fail(int code, const char *msg) { return fail(code, "%s", msg); }
 
-RysTables dpm_tables()
+RysTables aux_tables()
 {
     g_err = PATMOL_HIP_OK;
     std::lock_guard<std::mutex> lk(g_mu);

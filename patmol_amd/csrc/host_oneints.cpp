@@ -1,6 +1,8 @@
-// host_oneints.cpp -- one-electron integrals for the SCF driver
-// (oneints.c:12-234): overlap, kinetic and nuclear attraction over
-// contracted Cartesian shells by McMurchie-Davidson Hermite expansion.
+// host_oneints.cpp -- CPU one-electron integrals (oneints.c:12-234):
+// overlap, kinetic and nuclear attraction over contracted Cartesian shells
+// by McMurchie-Davidson Hermite expansion.  The SCF drivers use the device
+// one_electron_ints of libpatmol_hip.so; this CPU version serves callers
+// without a GPU.
 // Centre skip_center() (the positron centre) is left out of the nuclear
 // attraction, as oneints.c:51-53 does.  O(N^2) host work.
 #include <cmath>
@@ -101,7 +103,7 @@ double kin1d(const Herm1D &d, int l1, int l2, double a, double b)
 
 }  // namespace
 
-extern "C" void one_electron_ints(double *S, double *T, double *H)
+extern "C" void patmol_host_one_electron_ints(double *S, double *T, double *H)
 {
     const Shell *sh = shells();
     const Center *ce = centers();

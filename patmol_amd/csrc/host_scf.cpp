@@ -7,8 +7,9 @@
 // or on the density change below max(eps, 10 eps N^2), and damping only when
 // the energy rose (simple averaging, or the two-point Anderson step of
 // scfaux.c:63-95).  The first update of a run never tests convergence and
-// resets the iteration counter (rhfscf.c:85-88).  G builds go through the
-// library's own rhf_form_gmatrix / uhf_form_gmatrix.
+// resets the iteration counter (rhfscf.c:85-88).  The one-electron
+// integrals and the G builds go through the library's own device
+// one_electron_ints and rhf_form_gmatrix / uhf_form_gmatrix.
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -266,6 +267,7 @@ extern "C" int patmol_rhf_scf(double *P_out, double *evals_out, PatmolScfResult 
     std::vector<double> S(no2), T(no2), H(no2), G(no2), F(no2), P(no2), P1(no2, 0.0), P2(no2, 0.0);
     std::vector<double> X, C, ev;
     one_electron_ints(S.data(), T.data(), H.data());
+    if (patmol_hip_last_error()) return 2;
     ERIFS *mints = init_ERIFS(0, n, "MINTS.DAT");
     two_electron_ints(mints, 0);
     if (patmol_hip_last_error()) {
@@ -350,6 +352,7 @@ extern "C" int patmol_uhf_scf(double *PA_out, double *PB_out, PatmolScfResult *r
     std::vector<double> PA(no2), PA1(no2, 0.0), PA2(no2, 0.0), PB(no2), PB1(no2, 0.0), PB2(no2, 0.0);
     std::vector<double> X, C, ev;
     one_electron_ints(S.data(), T.data(), H.data());
+    if (patmol_hip_last_error()) return 2;
     ERIFS *mints = init_ERIFS(0, n, "MINTS.DAT");
     two_electron_ints(mints, 0);
     if (patmol_hip_last_error()) {

@@ -1,5 +1,5 @@
 // patmol_host.h -- libpatmol_host.so: the caller side of the hot path
-// (basis model with the reference's basis.h accessors, one-electron
+// (basis model with the reference's basis.h accessors, CPU one-electron
 // integrals, RHF/UHF SCF drivers).
 #pragma once
 #include "patmol_types.h"
@@ -42,8 +42,9 @@ int patmol_basis_load(int ncen, const double *q, const double *xyz, int nshell, 
                       int mult, int maxits, double eps, int scf_a);
 int patmol_basis_add_dpm_center(void);
 
-/* one-electron integrals, packed (oneints.c:139-234) */
-void one_electron_ints(double *S, double *T, double *H);
+/* CPU one-electron integrals, packed (oneints.c:139-234); the device
+ * one_electron_ints is declared in patmol_hip.h */
+void patmol_host_one_electron_ints(double *S, double *T, double *H);
 
 /* SCF drivers (rhfscf.c:21-174,432-471; uhfscf.c:22-168,485-527) */
 typedef struct {

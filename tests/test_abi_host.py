@@ -53,7 +53,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_drop_in_signatures_match_reference_headers():
-    """The six drop-in prototypes keep the reference's parameter lists."""
+    """The drop-in prototypes keep the reference's parameter lists."""
     text = open(HEADER).read()
     expect = [
         r"ERIFS \*init_ERIFS\(int nst, int nend, const char \*sname\);",
@@ -63,6 +63,7 @@ def test_drop_in_signatures_match_reference_headers():
         r"void uhf_form_gmatrix\(const ERIFS \*tfiles, const double \*PmatA, const double \*PmatB,\s+"
         r"double \*GmatA, double \*GmatB\);",
         r"void dpm_ints\(double \*hmat\);",
+        r"void one_electron_ints\(double \*Smat, double \*Tmat, double \*Hmat\);",
     ]
     for e in expect:
         assert re.search(e, text), e
@@ -71,7 +72,7 @@ def test_drop_in_signatures_match_reference_headers():
 def test_host_basis_accessors_exported():
     exp = _exported(pm.LIB_HOST)
     for n in ("shells", "centers", "number_of_shells", "number_of_orbitals", "number_of_centers",
-              "skip_center", "set_center_position", "nuclear_repulsion", "one_electron_ints",
+              "skip_center", "set_center_position", "nuclear_repulsion", "patmol_host_one_electron_ints",
               "init_Basis", "dpm_augment", "patmol_basis_load", "patmol_rhf_scf", "patmol_uhf_scf"):
         assert n in exp, n
 
@@ -82,7 +83,7 @@ def test_host_one_electron_vs_oracle(factory):
     pm.load_system(s)
     H, _ = pm.lib()
     assert H.number_of_orbitals() == s.norb and H.number_of_shells() == s.nshell
-    S, T, Hm = pm.one_electron_ints()
+    S, T, Hm = pm.host_one_electron_ints()
     So, To, Ho = oracle.Basis(s).one_electron_ints()
     assert np.abs(S - So).max() < 1e-13
     assert np.abs(T - To).max() < 1e-11 * np.abs(To).max()
@@ -96,7 +97,7 @@ def test_host_sf6_patin_reader(golden_dir):
     pm.load_system(s)
     H, _ = pm.lib()
     assert H.number_of_orbitals() == 214 and H.number_of_shells() == 86
-    S, T, Hm = pm.one_electron_ints()
+    S, T, Hm = pm.host_one_electron_ints()
     So, To, Ho = oracle.Basis(s).one_electron_ints()
     assert np.abs(Hm - Ho).max() < 1e-10 * np.abs(Ho).max()
 

@@ -70,6 +70,46 @@ void patmol_hip_set_precision(int mode, double threshold);
  * Asynchronous with respect to the host. */
 int patmol_hip_rhf_gmatrix_device(const ERIFS *tfiles, const double *d_P, double *d_G, void *stream);
 
+/* ---- device SCF (the whole iteration in HBM) ---- */
+/* The reference's RHF / UHF iteration (rhfscf.c:21-95, uhfscf.c:22-100)
+ * with every matrix on the GPU: G builds of `mints` (and `xints`, may be
+ * NULL) as rhf_form_gmatrix / uhf_form_gmatrix, F' = X^T (H + G) X, the
+ * eigenproblem (rocSOLVER dsyevd with the reference's check_phase sign
+ * convention, spack_math.c:274-319), C = X V, P = C_occ C_occ^T
+ * (scfaux.c:5-41), calc_energy / pdiffNorm (scfaux.c:97-136) and the
+ * damping of rhfscf.c:60-69 / scfAccelerator (scfaux.c:64-95).  X = U s^-1/2
+ * as rhf_form_xmatrix (rhfscf.c:432-452) fails with PATMOL_HIP_EDEVICE on an
+ * overlap eigenvalue <= DBL_EPSILON.  The guess is the core Hamiltonian
+ * (rhfscf.c:454-471, uhfscf.c:505-527) unless one is passed.  scf_eps gets
+ * the floor 10 DBL_EPSILON N^2 (rhfscf.c:161-164).  S, H, guesses and
+ * outputs are host packed arrays (Cmat row-major N x N, MO j in column j);
+ * any output may be NULL.  Returns 0 or an error code.  Set
+ * PATMOL_HIP_SCF_LOG=1 for the reference's per-iteration log lines. */
+typedef struct {
+    double energy, eelec, enuc, ediff, pdiff;
+    int iterations, converged;
+    double gbuild_ms;   /* device time in the G builds */
+    double other_ms;    /* the rest of the loop (eigensolver, density, sums, host logic) */
+    double total_ms;
+} PatmolHipScfResult;
+int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Smat, const double *Hmat,
+                       int nocc, int maxits, double scf_eps, int scf_accel, double enuc,
+                       const double *Pguess, double *Pout, double *evals, double *Cmat,
+                       PatmolHipScfResult *res);
+int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Smat, const double *Hmat,
+                       int nocca, int noccb, int maxits, double scf_eps, int scf_accel, double enuc,
+                       const double *PAguess, const double *PBguess, double *PAout, double *PBout,
+                       double *evalsA, double *evalsB, PatmolHipScfResult *res);
+/* One diagonalisation (spack_trans + spack_diag + form_cmatrix +
+ * form_pmatrix) of host packed F in the basis orthogonalised by S: ascending
+ * eigenvalues, C (row-major, phase-fixed) and P for nocc occupied orbitals. */
+int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int nocc, double *evals, double *Cmat,
+                         double *Pmat);
+/* UHF G build on device arrays: dGA += J[PA+PB] - K[PA], dGB likewise
+ * (scfaux.c:554-714), asynchronous on `stream`. */
+int patmol_hip_uhf_gmatrix_device(const ERIFS *tfiles, const double *d_PA, const double *d_PB, double *d_GA,
+                                  double *d_GB, void *stream);
+
 /* ---- introspection ---- */
 typedef struct {
     int nshell, norb, start;

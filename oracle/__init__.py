@@ -83,6 +83,9 @@ def lib():
         L.or_uhf_scf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.c_void_p, C.c_int64, _dp, _dp, C.POINTER(ScfResult), C.c_int]
         L.or_eigh.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.or_check_phase.argtypes = [C.c_int, _dp]
+        L.or_diag_fock.restype = C.c_int
+        L.or_diag_fock.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.or_quartet_tasks.restype = C.c_int64
         L.or_quartet_tasks.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
                                        C.c_double, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
@@ -110,6 +113,26 @@ def eigh(A: np.ndarray):
     V = np.zeros((n, n))
     assert lib().or_eigh(n, np.ascontiguousarray(A, dtype=np.float64), ev, V) == 0
     return ev, V
+
+
+def check_phase(V: np.ndarray) -> np.ndarray:
+    """Copy of V (eigenvectors in columns) with the reference's sign
+    convention applied (spack_math.c:274-297)."""
+    Z = np.ascontiguousarray(V, dtype=np.float64).copy()
+    lib().or_check_phase(Z.shape[0], Z.reshape(-1))
+    return Z
+
+
+def diag_fock(S: np.ndarray, F: np.ndarray, nocc: int):
+    """(eigenvalues, C row-major, packed P) of one reference SCF
+    diagonalisation of packed F in the basis orthogonalised by packed S."""
+    no2 = len(S)
+    n = int(round((np.sqrt(8 * no2 + 1) - 1) / 2))
+    ev, Cm, P = np.zeros(n), np.zeros((n, n)), np.zeros(no2)
+    rc = lib().or_diag_fock(n, np.ascontiguousarray(S, dtype=np.float64), np.ascontiguousarray(F, dtype=np.float64),
+                            nocc, ev, Cm.reshape(-1), P)
+    assert rc == 0, "overlap matrix has an eigenvalue <= DBL_EPSILON"
+    return ev, Cm, P
 
 
 def quartet_tasks(qbra, qket, same, tau, tau32=0.0, chunk=64):

@@ -19,11 +19,16 @@
  *   or_two_electron_ints twoints.c:188-418                integer recount of the canonical filter;
  *                                                         SF6 golden E
  *   or_*_form_gmatrix   scfaux.c:428-714                  einsum over the full 4-index tensor
- *   or_one_electron_ints oneints.c:12-234                 reference one_electron_ints (_ref); SF6
+ *   or_one_electron_ints oneints.c:12-234                 reference one_electron_ints (_ref); SF6;
+ *                                                         closed forms for s/p primitives
  *   or_dpm_ints         dpmints.c:18-199                  reference dpm_ints (_ref); s-type closed form
  *   or_rhf_scf          rhfscf.c:21-174,432-471           SF6 golden: E, 43 iterations, E_nuc
  *   or_uhf_scf          uhfscf.c:22-168,485-527           closed-shell UHF == RHF (parity unpinned
  *                                                         for open shells: no reference answer)
+ *   or_check_phase      spack_math.c:274-297              reference spack_diag eigenvectors (_ref),
+ *                                                         incl. a dominant negative first component
+ *   or_diag_fock        rhfscf.c:432-471, scfaux.c:5-41   reference spack_trans/spack_diag/
+ *                                                         form_cmatrix/form_pmatrix (_ref)
  *   or_quartet_tasks    (no reference counterpart: the reference has no Schwarz list, K4)
  *                                                         integer brute force over all pairs
  */
@@ -1342,6 +1347,28 @@ int or_eigh(int n, const double *Ain, double *d, double *z)
     return 0;
 }
 
+/* check_phase (spack_math.c:274-297) on the columns of row-major z
+ * (column j = eigenvector j).  The reference scans components 1..n-1 for
+ * |z_k| strictly above a running maximum that starts at the SIGNED first
+ * component, and negates the vector when the component it ends on is
+ * negative. */
+void or_check_phase(int n, double *z)
+{
+    for (int j = 0; j < n; ++j) {
+        double maxx = z[j];
+        int maxi = 0;
+        for (int k = 1; k < n; ++k) {
+            double t = fabs(z[(size_t)k * n + j]);
+            if (t > maxx) {
+                maxx = t;
+                maxi = k;
+            }
+        }
+        if (z[(size_t)maxi * n + j] < 0.0)
+            for (int k = 0; k < n; ++k) z[(size_t)k * n + j] = -z[(size_t)k * n + j];
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* SCF drivers                                                          */
 /* ------------------------------------------------------------------ */
@@ -1358,6 +1385,7 @@ static int form_x(int n, const double *S, double *X)
     double *U = (double *)malloc(sizeof(double) * n * n);
     unpack(n, S, Sf);
     or_eigh(n, Sf, ev, U);
+    or_check_phase(n, U);
     int bad = 0;
     for (int j = 0; j < n; ++j) {
         if (ev[j] <= 2.220446049250313e-16) bad = 1;
@@ -1396,6 +1424,7 @@ static void diag_fock(int n, const double *Fp, const double *X, double *C, doubl
             Fp2[(size_t)i * n + j] = Fp2[(size_t)j * n + i] = a;
         }
     or_eigh(n, Fp2, ev, V);
+    or_check_phase(n, V);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             double s = 0.0;
@@ -1417,6 +1446,24 @@ static void form_p(int n, int nocc, const double *C, double *P)
             for (int k = 0; k < nocc; ++k) s += C[(size_t)i * n + k] * C[(size_t)j * n + k];
             P[ij] = s;
         }
+}
+
+/* One diagonalisation of the reference's SCF step: X = U s^-1/2
+ * (rhfscf.c:432-452), F' = X^T F X, eigenvectors with check_phase
+ * (spack_math.c:299-319), C = X V (scfaux.c:5-21), P = C_occ C_occ^T
+ * (scfaux.c:23-41).  C row-major, MO j in column j.  -1 when S has an
+ * eigenvalue <= DBL_EPSILON. */
+int or_diag_fock(int n, const double *S, const double *F, int nocc, double *ev, double *C, double *P)
+{
+    double *X = (double *)malloc(sizeof(double) * n * n);
+    if (form_x(n, S, X)) {
+        free(X);
+        return -1;
+    }
+    diag_fock(n, F, X, C, ev);
+    form_p(n, nocc, C, P);
+    free(X);
+    return 0;
 }
 
 /* scfaux.c:99-118 */

@@ -103,6 +103,8 @@ int or_uhf_scf(const OBasis *b, int nelec, int mult, int maxits, double eps, int
                const OERINT *ints, int64_t nints, double *PA_out, double *PB_out,
                OScfResult *res, int nthreads);
 int or_eigh(int n, const double *A, double *evals, double *evecs);
+void or_check_phase(int n, double *z);
+int or_diag_fock(int n, const double *S, const double *F, int nocc, double *ev, double *C, double *P);
 
 /* ---- screened quartet list mirror ---- */
 int64_t or_quartet_tasks(int nbra, const double *qbra, int nket, const double *qket,

@@ -53,6 +53,8 @@ def lib():
         L.ref_formMDR.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
         L.ref_eri_block.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.ref_spack_diag.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.ref_diag_fock.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.ref_check_phase.argtypes = [C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -103,3 +105,12 @@ def formMDR(L, sr, t, w, pq):
     out = np.zeros((L + 1) ** 3)
     lib().ref_formMDR(out, L, sr, t, w, np.ascontiguousarray(pq, dtype=np.float64))
     return out.reshape(L + 1, L + 1, L + 1)
+
+
+def diag_fock(S, F, nocc):
+    """(eigenvalues, C row-major, packed P) of the reference's diagonalisation step."""
+    no2 = len(S)
+    n = int(round((np.sqrt(8 * no2 + 1) - 1) / 2))
+    ev, Cm, P = np.zeros(n), np.zeros(n * n), np.zeros(no2)
+    lib().ref_diag_fock(n, np.ascontiguousarray(S), np.ascontiguousarray(F), nocc, ev, Cm, P)
+    return ev, Cm.reshape(n, n), P

@@ -223,3 +223,33 @@ void ref_spack_diag(int n, double *packed, double *z, double *evals)
     spack_diag(n, packed, z, evals, tmp);
     free(tmp);
 }
+
+/* One diagonalisation step of the reference's RHF driver from its own
+ * functions: X as rhf_form_xmatrix (rhfscf.c:432-452, restated here because
+ * it works on rhfscf.c's static globals), then spack_trans, spack_diag,
+ * form_cmatrix and form_pmatrix as rhf_form_guess does (rhfscf.c:465-470). */
+void ref_diag_fock(int n, const double *S, const double *F, int nocc, double *ev, double *C, double *P)
+{
+    const size_t no2 = (size_t)n * (n + 1) / 2;
+    double *a = (double *)malloc(sizeof(double) * no2), *w = (double *)malloc(sizeof(double) * n * n);
+    double *x = (double *)malloc(sizeof(double) * n * n), *tmp = (double *)malloc(sizeof(double) * n * n);
+    memcpy(a, S, sizeof(double) * no2);
+    spack_diag(n, a, w, ev, tmp);
+    for (int j = 0; j < n; ++j) ev[j] = 1. / sqrt(ev[j]);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) x[i * n + j] = w[i * n + j] * ev[j];
+    memcpy(a, F, sizeof(double) * no2);
+    spack_trans(n, a, x, tmp);
+    spack_diag(n, a, w, ev, tmp);
+    form_cmatrix(n, C, x, w);
+    form_pmatrix(n, nocc, C, P);
+    free(a);
+    free(w);
+    free(x);
+    free(tmp);
+}
+
+/* The reference's check_phase (spack_math.c:274-297) on the rows of z (row
+ * i = vector i), declared here because spack_math.h does not export it. */
+extern void check_phase(int n, double *z);
+void ref_check_phase(int n, double *z) { check_phase(n, z); }

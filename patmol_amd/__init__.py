@@ -57,6 +57,15 @@ class ScfResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class HipScfResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("eelec", C.c_double), ("enuc", C.c_double), ("ediff", C.c_double),
+                ("pdiff", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
+                ("gbuild_ms", C.c_double), ("other_ms", C.c_double), ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def build(jobs: int = 8) -> None:
     """Compile both libraries in-tree (hipcc cross-compiles gfx950 without a GPU)."""
     subprocess.run(["make", "-s", f"-j{jobs}", "-C", os.path.join(HERE, "csrc")], check=True)
@@ -114,6 +123,19 @@ def lib():
         G.patmol_hip_last_timing.restype = C.c_int
         G.patmol_hip_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), _dp]
         G.patmol_hip_set_timing.argtypes = [C.c_int]
+        _odp = C.c_void_p  # optional double arrays (None or array.ctypes.data)
+        G.patmol_hip_rhf_scf.restype = C.c_int
+        G.patmol_hip_rhf_scf.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
+                                         C.c_double, _odp, _odp, _odp, _odp, C.POINTER(HipScfResult)]
+        G.patmol_hip_uhf_scf.restype = C.c_int
+        G.patmol_hip_uhf_scf.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                         C.c_int, C.c_double, _odp, _odp, _odp, _odp, _odp, _odp,
+                                         C.POINTER(HipScfResult)]
+        G.patmol_hip_diag_fock.restype = C.c_int
+        G.patmol_hip_diag_fock.argtypes = [_dp, _dp, C.c_int, _odp, _odp, _odp]
+        G.patmol_hip_uhf_gmatrix_device.restype = C.c_int
+        G.patmol_hip_uhf_gmatrix_device.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        H.scf_convergence.restype = C.c_double
         _host, _hip = H, G
     return _host, _hip
 
@@ -133,7 +155,8 @@ def exported_symbols():
             "patmol_hip_rhf_gmatrix_device", "patmol_hip_stats", "patmol_hip_class_stats",
             "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
-            "patmol_hip_set_timing"]
+            "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
+            "patmol_hip_uhf_gmatrix_device"]
 
 
 def load_system(sys_: "systems.System") -> None:
@@ -322,3 +345,76 @@ def uhf_scf():
         _check("patmol_uhf_scf")
         raise PatmolError(f"patmol_uhf_scf failed ({rc})")
     return res.as_dict(), PA, PB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _scf_inputs():
+    H, _ = lib()
+    return dict(n=H.number_of_orbitals(), ne=H.number_of_electrons(), mult=H.spin_multiplicity(),
+                maxits=H.max_scf_iterations(), eps=H.scf_convergence(), accel=H.scf_acceleration(),
+                enuc=H.nuclear_repulsion())
+
+
+def rhf_scf_device(store: Optional["EriStore"] = None, guess: Optional[np.ndarray] = None,
+                   want_c: bool = False):
+    """The whole RHF iteration on the GPU (patmol_hip_rhf_scf) for the
+    current basis; returns (result dict, P, orbital energies[, C])."""
+    _, G = lib()
+    a = _scf_inputs()
+    n = a["n"]
+    S, T, Hm = one_electron_ints()
+    own = store is None
+    st = EriStore() if own else store
+    P = np.zeros(n * (n + 1) // 2)
+    ev = np.zeros(n)
+    Cm = np.zeros((n, n)) if want_c else None
+    g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+    res = HipScfResult()
+    rc = G.patmol_hip_rhf_scf(st.h, None, S, Hm, a["ne"] // 2, a["maxits"], a["eps"], a["accel"], a["enuc"],
+                              _ptr(g), _ptr(P), _ptr(ev), _ptr(Cm), C.byref(res))
+    if own:
+        st.close()
+    if rc:
+        _check("patmol_hip_rhf_scf")
+        raise PatmolError(f"patmol_hip_rhf_scf failed ({rc})")
+    out = (res.as_dict(), P, ev)
+    return out + (Cm,) if want_c else out
+
+
+def uhf_scf_device(store: Optional["EriStore"] = None):
+    """The whole UHF iteration on the GPU (patmol_hip_uhf_scf); returns
+    (result dict, PA, PB, alpha and beta orbital energies)."""
+    _, G = lib()
+    a = _scf_inputs()
+    n = a["n"]
+    noccb = (a["ne"] - (a["mult"] - 1)) // 2
+    nocca = a["ne"] - noccb
+    S, T, Hm = one_electron_ints()
+    own = store is None
+    st = EriStore() if own else store
+    PA, PB = np.zeros(n * (n + 1) // 2), np.zeros(n * (n + 1) // 2)
+    eA, eB = np.zeros(n), np.zeros(n)
+    res = HipScfResult()
+    rc = G.patmol_hip_uhf_scf(st.h, None, S, Hm, nocca, noccb, a["maxits"], a["eps"], a["accel"], a["enuc"],
+                              None, None, _ptr(PA), _ptr(PB), _ptr(eA), _ptr(eB), C.byref(res))
+    if own:
+        st.close()
+    if rc:
+        _check("patmol_hip_uhf_scf")
+        raise PatmolError(f"patmol_hip_uhf_scf failed ({rc})")
+    return res.as_dict(), PA, PB, eA, eB
+
+
+def diag_fock(S: np.ndarray, F: np.ndarray, nocc: int):
+    """One device diagonalisation of packed F in the basis orthogonalised by
+    packed S: (eigenvalues, C row-major, packed P)."""
+    _, G = lib()
+    n = int(round((np.sqrt(8 * len(S) + 1) - 1) / 2))
+    ev, Cm, P = np.zeros(n), np.zeros((n, n)), np.zeros(len(S))
+    rc = G.patmol_hip_diag_fock(np.ascontiguousarray(S), np.ascontiguousarray(F), nocc, _ptr(ev), _ptr(Cm), _ptr(P))
+    if rc:
+        _check("patmol_hip_diag_fock")
+    return ev, Cm, P

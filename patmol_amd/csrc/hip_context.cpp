@@ -199,7 +199,7 @@ struct Ctx {
     DevBuf<PrimPair> d_pp;
     DevBuf<int4> d_items;
     // work buffers
-    DevBuf<double> d_P, d_D, d_D2, d_J, d_K0, d_K1, d_G, d_G2, d_dmax;
+    DevBuf<double> d_P, d_PB, d_D, d_D2, d_DJ, d_J, d_K0, d_K1, d_G, d_G2, d_dmax;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> cls_ev;
     // side streams for concurrent class launches (default priority)
@@ -581,7 +581,7 @@ static int alloc_work(Ctx &c, int nk)
     if (c.d_D.alloc(NN) || c.d_J.alloc(NN * PM_XCD_SLOTS) || c.d_K0.alloc(NN * PM_XCD_SLOTS) || c.d_P.alloc(no2) ||
         c.d_G.alloc(no2) || c.d_dmax.alloc(1))
         return g_err;
-    if (nk == 2 && (c.d_D2.alloc(NN) || c.d_K1.alloc(NN * PM_XCD_SLOTS) || c.d_G2.alloc(no2))) return g_err;
+    if (nk == 2 && (c.d_D2.alloc(NN) || c.d_DJ.alloc(NN) || c.d_PB.alloc(no2) || c.d_G2.alloc(no2))) return g_err;
     return 0;
 }
 
@@ -603,6 +603,26 @@ static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
     if (g_prec_mode == 1) dmax = device_absmax(c, dP, no2, s);
     if (run_jk(c, c.d_D.p, c.d_D.p, nullptr, 1, dmax, s)) return g_err;
     hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, c.d_J.p, c.d_K0.p, dG, c.N, 2.0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// GA += J[PA+PB] - K[PA], GB += J[PA+PB] - K[PB] (scfaux.c:554-714), all on
+// stream s: two single-density passes (J[PA+PB], K[PA]) and (J[PA+PB], K[PB])
+// with J = J' + J'^T.
+static int uhf_device(Ctx &x, const double *dPA, const double *dPB, double *dGA, double *dGB, hipStream_t s)
+{
+    if (alloc_work(x, 2)) return g_err;
+    const int64_t NN = (int64_t)x.N * x.N, no2 = (int64_t)x.N * (x.N + 1) / 2;
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPA, x.d_D.p, x.N);
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPB, x.d_D2.p, x.N);
+    hipLaunchKernelGGL(add_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, x.d_D.p, x.d_D2.p, x.d_DJ.p, NN);
+    double dmax = 0.0;
+    if (g_prec_mode == 1) dmax = std::max(device_absmax(x, dPA, no2, s), device_absmax(x, dPB, no2, s));
+    if (run_jk(x, x.d_DJ.p, x.d_D.p, nullptr, 1, dmax, s)) return g_err;
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGA, x.N, 1.0);
+    if (run_jk(x, x.d_DJ.p, x.d_D2.p, nullptr, 1, dmax, s)) return g_err;
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGB, x.N, 1.0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -674,8 +694,8 @@ void clean_ERIFS(ERIFS *e)
         }
     }
     if (c) {
-        for (auto *b : {&c->d_xyz, &c->d_q, &c->d_bound, &c->d_P, &c->d_D, &c->d_D2, &c->d_J, &c->d_K0, &c->d_K1,
-                        &c->d_G, &c->d_G2, &c->d_dmax})
+        for (auto *b : {&c->d_xyz, &c->d_q, &c->d_bound, &c->d_P, &c->d_PB, &c->d_D, &c->d_D2, &c->d_DJ, &c->d_J,
+                        &c->d_K0, &c->d_K1, &c->d_G, &c->d_G2, &c->d_dmax})
             b->release();
         for (auto *b : {&c->d_off, &c->d_orig, &c->d_tile_first, &c->d_tile_size}) b->release();
         c->d_pairs.release();
@@ -753,28 +773,16 @@ void uhf_form_gmatrix(const ERIFS *e, const double *PA, const double *PB, double
         return;
     }
     Ctx &x = *c;
-    const int64_t NN = (int64_t)x.N * x.N, no2 = (int64_t)x.N * (x.N + 1) / 2;
+    const int64_t no2 = (int64_t)x.N * (x.N + 1) / 2;
     if (alloc_work(x, 2)) return;
-    DevBuf<double> dPB, dDJ;
-    if (dPB.alloc(no2) || dDJ.alloc(NN)) return;
     if (hipMemcpy(x.d_P.p, PA, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dPB.p, PB, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(x.d_PB.p, PB, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(x.d_G.p, 0, sizeof(double) * no2) != hipSuccess ||
         hipMemset(x.d_G2.p, 0, sizeof(double) * no2) != hipSuccess) {
         fail(PATMOL_HIP_EDEVICE, "uhf_form_gmatrix: transfer failed");
         return;
     }
-    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, x.d_P.p, x.d_D.p, x.N);
-    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, dPB.p, x.d_D2.p, x.N);
-    hipLaunchKernelGGL(add_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, x.d_D.p, x.d_D2.p, dDJ.p, NN);
-    double dmax = 0.0;
-    if (g_prec_mode == 1) dmax = std::max(device_absmax(x, x.d_P.p, no2, 0), device_absmax(x, dPB.p, no2, 0));
-    // Two single-density passes: (J[PA+PB], K[PA]) then (J[PA+PB], K[PB]);
-    // GA += J - K[PA], GB += J - K[PB] with J = J' + J'^T.
-    if (run_jk(x, dDJ.p, x.d_D.p, nullptr, 1, dmax, 0)) return;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, 0, x.d_J.p, x.d_K0.p, x.d_G.p, x.N, 1.0);
-    if (run_jk(x, dDJ.p, x.d_D2.p, nullptr, 1, dmax, 0)) return;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, 0, x.d_J.p, x.d_K0.p, x.d_G2.p, x.N, 1.0);
+    if (uhf_device(x, x.d_P.p, x.d_PB.p, x.d_G.p, x.d_G2.p, 0)) return;
     std::vector<double> ga(no2), gb(no2);
     if (hipMemcpy(ga.data(), x.d_G.p, sizeof(double) * no2, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(gb.data(), x.d_G2.p, sizeof(double) * no2, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -782,8 +790,6 @@ void uhf_form_gmatrix(const ERIFS *e, const double *PA, const double *PB, double
         return;
     }
     collect_timing(x);
-    dPB.release();
-    dDJ.release();
     for (int64_t i = 0; i < no2; ++i) {
         GA[i] += ga[i];
         GB[i] += gb[i];
@@ -797,6 +803,16 @@ int patmol_hip_rhf_gmatrix_device(const ERIFS *e, const double *dP, double *dG, 
     if (!c || !dP || !dG) return fail(PATMOL_HIP_EBADARG, "bad handle or null pointer");
     if (!c->built) return fail(PATMOL_HIP_ESTATE, "two_electron_ints was not run for this handle");
     return rhf_device(*c, dP, dG, (hipStream_t)stream);
+}
+
+int patmol_hip_uhf_gmatrix_device(const ERIFS *e, const double *dPA, const double *dPB, double *dGA, double *dGB,
+                                  void *stream)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *c = ctx_of(e);
+    if (!c || !dPA || !dPB || !dGA || !dGB) return fail(PATMOL_HIP_EBADARG, "bad handle or null pointer");
+    if (!c->built) return fail(PATMOL_HIP_ESTATE, "two_electron_ints was not run for this handle");
+    return uhf_device(*c, dPA, dPB, dGA, dGB, (hipStream_t)stream);
 }
 
 int patmol_hip_last_timing(const ERIFS *e, double *total_ms, double *class_ms)

@@ -1,0 +1,638 @@
+// scf_device.hip -- the whole SCF iteration on the GPU.
+//
+// Restates the reference's RHF and UHF updates (rhfscf.c:21-71,
+// uhfscf.c:22-80), their drivers (rhfscf.c:73-95, uhfscf.c:82-100), the
+// orthogonaliser X = U s^-1/2 (rhfscf.c:432-452) and the core-Hamiltonian
+// guess (rhfscf.c:454-471) with every matrix resident in HBM:
+//   G       the library's own device G build (G zeroed, then G += 2J - K, or
+//           the UHF pair);
+//   F' = X^T (H + G) X    two rocBLAS dgemm (spack_trans, spack_math.c:321-371);
+//   eigen   rocSOLVER dsyevd (the role of spack_diag, spack_math.c:299-319),
+//           then the reference's eigenvector sign convention check_phase
+//           (spack_math.c:274-297, below);
+//   C = X V dgemm (form_cmatrix, scfaux.c:5-21);
+//   P = C_occ C_occ^T     dsyrk + pack (form_pmatrix, scfaux.c:23-41);
+//   sums    calc_energy and pdiffNorm (scfaux.c:97-136), and the three dot
+//           products of scfAccelerator (scfaux.c:64-95), each one
+//           fixed-order block reduction;
+//   damping the averaging / two-point Anderson updates (rhfscf.c:60-69).
+// The host keeps only the iteration logic: per iteration it reads back the
+// energy and density change (two doubles) to take the reference's
+// convergence and damping decisions.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "patmol_hip.h"
+
+extern "C" {
+int number_of_orbitals(void);
+}
+
+namespace pm {
+
+int aux_fail(int code, const char *msg);
+
+namespace {
+
+constexpr int RED_THREADS = 1024;
+
+// full symmetric F = a + b from packed lower triangles (b may be null)
+__global__ void sym_from_packed_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                       double *__restrict__ F, int n)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)n * n) return;
+    const int i = (int)(k / n), j = (int)(k % n);
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const int64_t ij = (int64_t)hi * (hi + 1) / 2 + lo;
+    F[k] = b ? a[ij] + b[ij] : a[ij];
+}
+
+// check_phase (spack_math.c:274-297) on every eigenvector (column j of V,
+// contiguous): the reference scans v_1..v_{n-1} for |v_k| strictly above a
+// running maximum that starts at the SIGNED v_0, and flips the vector when the
+// component it lands on is negative.  Equivalently, with M = max_{k>=1} |v_k|
+// at its first index k*: flip iff M > v_0 and v_{k*} < 0.
+__global__ __launch_bounds__(256) void phase_kernel(double *__restrict__ V, int n)
+{
+    __shared__ double sm[256];
+    __shared__ int si[256];
+    __shared__ int flip;
+    double *v = V + (size_t)blockIdx.x * n;
+    double m = -1.0;
+    int mi = n;
+    for (int k = 1 + threadIdx.x; k < n; k += blockDim.x) {
+        const double t = fabs(v[k]);
+        if (t > m) {
+            m = t;
+            mi = k;
+        }
+    }
+    sm[threadIdx.x] = m;
+    si[threadIdx.x] = mi;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = sm[threadIdx.x + s];
+            const int oi = si[threadIdx.x + s];
+            if (o > sm[threadIdx.x] || (o == sm[threadIdx.x] && oi < si[threadIdx.x])) {
+                sm[threadIdx.x] = o;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) flip = (si[0] < n && sm[0] > v[0] && v[si[0]] < 0.0) ? 1 : 0;
+    __syncthreads();
+    if (flip)
+        for (int k = threadIdx.x; k < n; k += blockDim.x) v[k] = -v[k];
+}
+
+// X(:, j) = U(:, j) / sqrt(s_j)  (rhfscf.c:447-451)
+__global__ void scale_columns_kernel(double *__restrict__ X, const double *__restrict__ U,
+                                     const double *__restrict__ s, int n)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)n * n) return;
+    X[k] = U[k] * (1.0 / sqrt(s[k / n]));
+}
+
+// packed P(i,j), i >= j, from the lower triangle of column-major D
+__global__ void pack_lower_kernel(const double *__restrict__ D, double *__restrict__ P, int n)
+{
+    const int i = blockIdx.x;
+    for (int j = threadIdx.x; j <= i; j += blockDim.x) P[(size_t)i * (i + 1) / 2 + j] = D[(size_t)j * n + i];
+}
+
+// Weighted sums over packed triangles, off-diagonal weight 1 and diagonal
+// 1/2 (calc_energy and pdiffNorm, scfaux.c:97-136), one block, fixed order:
+//   out[0] = sum w H P1, out[1] = sum w G P1, out[2] = sum w (P - P1)^2.
+__global__ __launch_bounds__(RED_THREADS) void scf_sums_kernel(const double *__restrict__ P,
+                                                              const double *__restrict__ P1,
+                                                              const double *__restrict__ H,
+                                                              const double *__restrict__ G, int n,
+                                                              double *__restrict__ out)
+{
+    __shared__ double red[3][RED_THREADS];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const size_t r0 = (size_t)i * (i + 1) / 2;
+        for (int j = threadIdx.x; j <= i; j += RED_THREADS) {
+            const size_t ij = r0 + j;
+            const double w = j == i ? 0.5 : 1.0, p1 = P1[ij], d = P[ij] - p1;
+            a += w * H[ij] * p1;
+            b += w * G[ij] * p1;
+            c += w * d * d;
+        }
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) out[q] = red[q][0];
+}
+
+// dot products of scfAccelerator (scfaux.c:70-77)
+__global__ __launch_bounds__(RED_THREADS) void accel_dots_kernel(const double *__restrict__ p0,
+                                                                const double *__restrict__ p1,
+                                                                const double *__restrict__ p2, int64_t no2,
+                                                                double *__restrict__ out)
+{
+    __shared__ double red[3][RED_THREADS];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int64_t i = threadIdx.x; i < no2; i += RED_THREADS) {
+        const double s0 = p0[i] - p1[i], s1 = p1[i] - p2[i];
+        a += s0 * s0;
+        b += s0 * s1;
+        c += s1 * s1;
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) out[q] = red[q][0];
+}
+
+// p2 <- p1, p1 <- p0, p0 <- beta p1 + (1 - beta) p2   (scfaux.c:84-94)
+__global__ void accel_update_kernel(double *__restrict__ p0, double *__restrict__ p1, double *__restrict__ p2,
+                                    int64_t no2, double beta)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= no2) return;
+    const double a = p0[i], b = p1[i];
+    p2[i] = b;
+    p1[i] = a;
+    p0[i] = a * beta + b * (1.0 - beta);
+}
+
+// p <- (p1 + p2) / 2   (rhfscf.c:61-65)
+__global__ void average_kernel(double *__restrict__ p, const double *__restrict__ p1, const double *__restrict__ p2,
+                               int64_t no2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < no2) p[i] = 0.5 * (p1[i] + p2[i]);
+}
+
+struct DevMem {
+    std::vector<void *> ptrs;
+    ~DevMem()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    T *get(size_t n)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T *)p;
+    }
+};
+
+// Device state of one SCF run: X, the Fock/eigen work matrices and the
+// rocBLAS handle on the run's stream.
+struct Scf {
+    int n = 0;
+    size_t no2 = 0;
+    hipStream_t s = nullptr;
+    rocblas_handle h = nullptr;
+    DevMem mem;
+    double *X = nullptr, *F = nullptr, *T = nullptr, *C = nullptr, *ev = nullptr, *off = nullptr, *sums = nullptr;
+    rocblas_int *info = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const char *err = nullptr;
+
+    ~Scf()
+    {
+        if (h) rocblas_destroy_handle(h);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+    }
+
+    bool init(int n_)
+    {
+        n = n_;
+        no2 = (size_t)n * (n + 1) / 2;
+        const size_t nn = (size_t)n * n;
+        if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            return fail("stream/event creation failed");
+        if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, s) != rocblas_status_success)
+            return fail("rocblas handle creation failed");
+        X = mem.get<double>(nn);
+        F = mem.get<double>(nn);
+        T = mem.get<double>(nn);
+        C = mem.get<double>(nn);
+        ev = mem.get<double>(n);
+        off = mem.get<double>(n);
+        sums = mem.get<double>(8);
+        info = mem.get<rocblas_int>(1);
+        if (!X || !F || !T || !C || !ev || !off || !sums || !info) return fail("device allocation failed");
+        return true;
+    }
+
+    bool fail(const char *m)
+    {
+        err = m;
+        return false;
+    }
+
+    // eigenvectors of the symmetric F (overwritten, column-major V) with
+    // the reference's sign convention; eigenvalues ascending into ev
+    bool eigen()
+    {
+        if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, F, n, ev, off, info) !=
+            rocblas_status_success)
+            return fail("rocsolver_dsyevd failed");
+        hipLaunchKernelGGL(phase_kernel, dim3(n), dim3(256), 0, s, F, n);
+        rocblas_int inf = 0;
+        if (hipMemcpyAsync(&inf, info, sizeof(inf), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail("device failure in the eigensolver");
+        if (inf != 0) return fail("eigensolver did not converge");
+        return true;
+    }
+
+    // X = U s^-1/2 from packed S (rhfscf.c:432-452)
+    bool form_x(const double *dS)
+    {
+        const int64_t nn = (int64_t)n * n;
+        hipLaunchKernelGGL(sym_from_packed_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, dS, nullptr, F, n);
+        if (!eigen()) return false;
+        std::vector<double> sv(n);
+        if (hipMemcpyAsync(sv.data(), ev, sizeof(double) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail("device failure in the orthogonaliser");
+        for (int i = 0; i < n; ++i)
+            if (sv[i] <= DBL_EPSILON) return fail("Negative or Zero Eigenvalues in Overlap Matrix!");
+        hipLaunchKernelGGL(scale_columns_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, X, F, ev, n);
+        return hipGetLastError() == hipSuccess || fail("launch failure");
+    }
+
+    // C = X V with V the eigenvectors of X^T (a + b) X; P = C_occ C_occ^T
+    // packed (rhfscf.c:37-46).  Leaves the eigenvalues in ev.
+    bool diag_density(const double *a, const double *b, int nocc, double *P)
+    {
+        const int64_t nn = (int64_t)n * n;
+        const double one = 1.0, zero = 0.0;
+        hipLaunchKernelGGL(sym_from_packed_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, a, b, F, n);
+        // T = F X, F' = X^T T
+        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, F, n, X, n, &zero, T, n) !=
+                rocblas_status_success ||
+            rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, X, n, T, n, &zero, F,
+                          n) != rocblas_status_success)
+            return fail("rocblas_dgemm failed");
+        if (!eigen()) return false;
+        // C = X V, D = C_occ C_occ^T (lower), P = pack(D)
+        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, F, n, &zero, C, n) !=
+            rocblas_status_success)
+            return fail("rocblas_dgemm failed");
+        if (nocc > 0) {
+            if (rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n, nocc, &one, C, n, &zero, T, n) !=
+                rocblas_status_success)
+                return fail("rocblas_dsyrk failed");
+        } else {
+            if (hipMemsetAsync(T, 0, sizeof(double) * nn, s) != hipSuccess) return fail("memset failed");
+        }
+        hipLaunchKernelGGL(pack_lower_kernel, dim3(n), dim3(256), 0, s, T, P, n);
+        return hipGetLastError() == hipSuccess || fail("launch failure");
+    }
+
+    // sums over (P, P1, H, G) -> host
+    bool sums3(const double *P, const double *P1, const double *H, const double *G, double *out)
+    {
+        hipLaunchKernelGGL(scf_sums_kernel, dim3(1), dim3(RED_THREADS), 0, s, P, P1, H, G, n, sums);
+        if (hipMemcpyAsync(out, sums, sizeof(double) * 3, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail("device failure in the energy sums");
+        return true;
+    }
+
+    // scfAccelerator (scfaux.c:64-95) on device densities
+    bool accelerate(int iter, double *p0, double *p1, double *p2)
+    {
+        double beta = 0.5;
+        if (iter > 2) {
+            double d[3];
+            hipLaunchKernelGGL(accel_dots_kernel, dim3(1), dim3(RED_THREADS), 0, s, p0, p1, p2, (int64_t)no2, sums);
+            if (hipMemcpyAsync(d, sums, sizeof(double) * 3, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return fail("device failure in the accelerator");
+            beta = (d[0] - d[1]) / (d[0] - 2.0 * d[1] + d[2]);
+            if (beta < 1e-3) beta = 1e-3;
+            if (beta > 1.5) beta = 1.5;
+        }
+        hipLaunchKernelGGL(accel_update_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, p0, p1, p2, (int64_t)no2,
+                           beta);
+        return hipGetLastError() == hipSuccess || fail("launch failure");
+    }
+
+    void average(double *p, const double *p1, const double *p2)
+    {
+        hipLaunchKernelGGL(average_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, p, p1, p2, (int64_t)no2);
+    }
+
+    float elapsed_ms()
+    {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    }
+};
+
+bool scf_log()
+{
+    const char *e = getenv("PATMOL_HIP_SCF_LOG");
+    return e && *e && *e != '0';
+}
+
+// copy a column-major device matrix to a row-major host one
+bool download_rowmajor(const Scf &c, const double *dM, double *host)
+{
+    std::vector<double> tmp((size_t)c.n * c.n);
+    if (hipMemcpyAsync(tmp.data(), dM, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, c.s) != hipSuccess ||
+        hipStreamSynchronize(c.s) != hipSuccess)
+        return false;
+    for (int i = 0; i < c.n; ++i)
+        for (int j = 0; j < c.n; ++j) host[(size_t)i * c.n + j] = tmp[(size_t)j * c.n + i];
+    return true;
+}
+
+int scf_fail(Scf &c, int code, const char *where)
+{
+    char msg[256];
+    snprintf(msg, sizeof msg, "%s: %s", where, c.err ? c.err : "failure");
+    return aux_fail(code, msg);
+}
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Smat, const double *Hmat,
+                                  int nocc, int maxits, double scf_eps, int scf_accel, double enuc,
+                                  const double *Pguess, double *Pout, double *evals, double *Cmat,
+                                  PatmolHipScfResult *res)
+{
+    const int n = number_of_orbitals();
+    if (!mints || !Smat || !Hmat || !res || n <= 0 || nocc < 0 || nocc > n)
+        return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_rhf_scf: bad argument");
+    const double EPS10 = 10 * DBL_EPSILON;
+    Scf c;
+    if (!c.init(n)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+    const size_t no2 = c.no2;
+    double *dS = c.mem.get<double>(no2), *dH = c.mem.get<double>(no2), *dG = c.mem.get<double>(no2);
+    double *Pb[3] = {c.mem.get<double>(no2), c.mem.get<double>(no2), c.mem.get<double>(no2)};
+    if (!dS || !dH || !dG || !Pb[0] || !Pb[1] || !Pb[2])
+        return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: device allocation failed");
+    double *P = Pb[0], *P1 = Pb[1], *P2 = Pb[2];
+    if (hipMemcpyAsync(dS, Smat, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
+        hipMemcpyAsync(dH, Hmat, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
+        hipMemsetAsync(P1, 0, sizeof(double) * no2, c.s) != hipSuccess ||
+        hipMemsetAsync(P2, 0, sizeof(double) * no2, c.s) != hipSuccess)
+        return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: upload failed");
+    if (!c.form_x(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+    if (Pguess) {
+        if (hipMemcpyAsync(P, Pguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
+            return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: upload failed");
+    } else if (!c.diag_density(dH, nullptr, nocc, P)) {
+        return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+    }
+    double eps = scf_eps;
+    if (eps < EPS10 * n * n) eps = EPS10 * n * n;
+    const bool log = scf_log();
+    double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0, dms = 0.0;
+    int iteration = 0, done = 0;
+    bool first = true;
+    auto t_start = std::chrono::steady_clock::now();
+    for (;;) {
+        ++iteration;
+        (void)hipEventRecord(c.e0, c.s);
+        if (hipMemsetAsync(dG, 0, sizeof(double) * no2, c.s) != hipSuccess)
+            return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: memset failed");
+        if (patmol_hip_rhf_gmatrix_device(mints, P, dG, c.s)) return patmol_hip_last_error();
+        if (xints && patmol_hip_rhf_gmatrix_device(xints, P, dG, c.s)) return patmol_hip_last_error();
+        (void)hipEventRecord(c.e1, c.s);
+        // P2 <- P1 <- P, new P into the freed buffer (rhfscf.c:43-46)
+        double *fresh = P2;
+        P2 = P1;
+        P1 = P;
+        P = fresh;
+        if (!c.diag_density(dH, dG, nocc, P)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        double sm[3];
+        if (!c.sums3(P, P1, dH, dG, sm)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        const float g_ms = c.elapsed_ms();
+        gms += g_ms;
+        pdiff = sqrt(2.0 * sm[2]) / n;
+        enrg = 2.0 * (sm[0] + sm[0] + sm[1]);
+        ediff = enrg - eold;
+        eold = enrg;
+        int ret = 0;
+        if (iteration == 1) {
+        } else if (fabs(ediff) < EPS10 || pdiff < eps) {
+            ret = 1;
+        } else if (ediff > 0.0) {
+            if (scf_accel == 1) c.average(P, P1, P2);
+            if (scf_accel == 2 && !c.accelerate(iteration, P, P1, P2))
+                return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        }
+        if (first) {
+            first = false;
+            iteration = 0;
+        } else {
+            done = ret;
+        }
+        if (log) fprintf(stderr, "ITERATION = %12d EDIFF= %lg\n  energy = %.12f\n", iteration, ediff, enrg + enuc);
+        if (iteration == 0) continue;
+        if (done || iteration >= maxits) break;
+    }
+    if (hipStreamSynchronize(c.s) != hipSuccess) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: device failure");
+    const double tot = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    dms = tot - gms;
+    res->energy = enrg + enuc;
+    res->eelec = enrg;
+    res->enuc = enuc;
+    res->ediff = ediff;
+    res->pdiff = pdiff;
+    res->iterations = iteration;
+    res->converged = done;
+    res->gbuild_ms = gms;
+    res->other_ms = dms;
+    res->total_ms = tot;
+    bool ok = true;
+    if (Pout) ok = ok && hipMemcpy(Pout, P, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
+    if (evals) ok = ok && hipMemcpy(evals, c.ev, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess;
+    if (Cmat) ok = ok && download_rowmajor(c, c.C, Cmat);
+    if (!ok) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: download failed");
+    return 0;
+}
+
+extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Smat, const double *Hmat,
+                                  int nocca, int noccb, int maxits, double scf_eps, int scf_accel, double enuc,
+                                  const double *PAguess, const double *PBguess, double *PAout, double *PBout,
+                                  double *evalsA, double *evalsB, PatmolHipScfResult *res)
+{
+    const int n = number_of_orbitals();
+    if (!mints || !Smat || !Hmat || !res || n <= 0 || nocca < 0 || noccb < 0 || nocca > n || noccb > n)
+        return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_uhf_scf: bad argument");
+    const double EPS10 = 10 * DBL_EPSILON;
+    Scf c;
+    if (!c.init(n)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
+    const size_t no2 = c.no2;
+    double *dS = c.mem.get<double>(no2), *dH = c.mem.get<double>(no2);
+    double *GA = c.mem.get<double>(no2), *GB = c.mem.get<double>(no2);
+    double *A[3], *B[3];
+    for (int k = 0; k < 3; ++k) {
+        A[k] = c.mem.get<double>(no2);
+        B[k] = c.mem.get<double>(no2);
+    }
+    double *evA = c.mem.get<double>(n);
+    if (!dS || !dH || !GA || !GB || !A[0] || !A[1] || !A[2] || !B[0] || !B[1] || !B[2] || !evA)
+        return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: device allocation failed");
+    double *PA = A[0], *PA1 = A[1], *PA2 = A[2], *PB = B[0], *PB1 = B[1], *PB2 = B[2];
+    if (hipMemcpyAsync(dS, Smat, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
+        hipMemcpyAsync(dH, Hmat, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
+        hipMemsetAsync(PA1, 0, sizeof(double) * no2, c.s) != hipSuccess ||
+        hipMemsetAsync(PA2, 0, sizeof(double) * no2, c.s) != hipSuccess ||
+        hipMemsetAsync(PB1, 0, sizeof(double) * no2, c.s) != hipSuccess ||
+        hipMemsetAsync(PB2, 0, sizeof(double) * no2, c.s) != hipSuccess)
+        return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: upload failed");
+    if (!c.form_x(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
+    // core-Hamiltonian guess: both spins from the same orbitals (uhfscf.c:505-527)
+    if (PAguess && PBguess) {
+        if (hipMemcpyAsync(PA, PAguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
+            hipMemcpyAsync(PB, PBguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
+            return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: upload failed");
+    } else if (!c.diag_density(dH, nullptr, nocca, PA)) {
+        return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
+    } else if (noccb > 0) {
+        const double one = 1.0, zero = 0.0;
+        if (rocblas_dsyrk(c.h, rocblas_fill_lower, rocblas_operation_none, n, noccb, &one, c.C, n, &zero, c.T, n) !=
+            rocblas_status_success)
+            return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: rocblas_dsyrk failed");
+        hipLaunchKernelGGL(pack_lower_kernel, dim3(n), dim3(256), 0, c.s, c.T, PB, n);
+    } else if (hipMemsetAsync(PB, 0, sizeof(double) * no2, c.s) != hipSuccess) {
+        return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: memset failed");
+    }
+    double eps = scf_eps;
+    if (eps < EPS10 * n * n) eps = EPS10 * n * n;
+    const bool log = scf_log();
+    double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0;
+    int iteration = 0, done = 0;
+    bool first = true;
+    auto t_start = std::chrono::steady_clock::now();
+    for (;;) {
+        ++iteration;
+        (void)hipEventRecord(c.e0, c.s);
+        if (hipMemsetAsync(GA, 0, sizeof(double) * no2, c.s) != hipSuccess ||
+            hipMemsetAsync(GB, 0, sizeof(double) * no2, c.s) != hipSuccess)
+            return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: memset failed");
+        if (patmol_hip_uhf_gmatrix_device(mints, PA, PB, GA, GB, c.s)) return patmol_hip_last_error();
+        if (xints && patmol_hip_uhf_gmatrix_device(xints, PA, PB, GA, GB, c.s)) return patmol_hip_last_error();
+        (void)hipEventRecord(c.e1, c.s);
+        double *fa = PA2, *fb = PB2;
+        PA2 = PA1;
+        PA1 = PA;
+        PA = fa;
+        PB2 = PB1;
+        PB1 = PB;
+        PB = fb;
+        double sa[3], sb[3];
+        if (!c.diag_density(dH, GA, nocca, PA) ||
+            hipMemcpyAsync(evA, c.ev, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s) != hipSuccess ||
+            !c.sums3(PA, PA1, dH, GA, sa) || !c.diag_density(dH, GB, noccb, PB) || !c.sums3(PB, PB1, dH, GB, sb))
+            return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
+        gms += c.elapsed_ms();
+        pdiff = sqrt(2.0 * sa[2]) / n + sqrt(2.0 * sb[2]) / n;
+        enrg = (sa[0] + sa[0] + sa[1]) + (sb[0] + sb[0] + sb[1]);
+        ediff = enrg - eold;
+        eold = enrg;
+        int ret = 0;
+        if (iteration == 1) {
+        } else if (fabs(ediff) < EPS10 || pdiff < eps) {
+            ret = 1;
+        } else if (ediff > 0.0) {
+            if (scf_accel == 1) {
+                c.average(PA, PA1, PA2);
+                c.average(PB, PB1, PB2);
+            }
+            if (scf_accel == 2 && (!c.accelerate(iteration, PA, PA1, PA2) || !c.accelerate(iteration, PB, PB1, PB2)))
+                return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
+        }
+        if (first) {
+            first = false;
+            iteration = 0;
+        } else {
+            done = ret;
+        }
+        if (log) fprintf(stderr, "ITERATION = %12d EDIFF= %lg\n  energy = %.12f\n", iteration, ediff, enrg + enuc);
+        if (iteration == 0) continue;
+        if (done || iteration >= maxits) break;
+    }
+    if (hipStreamSynchronize(c.s) != hipSuccess) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: device failure");
+    const double tot = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    res->energy = enrg + enuc;
+    res->eelec = enrg;
+    res->enuc = enuc;
+    res->ediff = ediff;
+    res->pdiff = pdiff;
+    res->iterations = iteration;
+    res->converged = done;
+    res->gbuild_ms = gms;
+    res->other_ms = tot - gms;
+    res->total_ms = tot;
+    bool ok = true;
+    if (PAout) ok = ok && hipMemcpy(PAout, PA, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
+    if (PBout) ok = ok && hipMemcpy(PBout, PB, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
+    if (evalsA) ok = ok && hipMemcpy(evalsA, evA, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess;
+    if (evalsB) ok = ok && hipMemcpy(evalsB, c.ev, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: download failed");
+    return 0;
+}
+
+// Building block for callers with their own loop: eigenvalues and
+// phase-fixed eigenvectors (row-major, eigenvector j in column j) of X^T F X,
+// and C = X V, for host packed S and F.
+extern "C" int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int nocc, double *evals, double *Cmat,
+                                    double *Pmat)
+{
+    const int n = number_of_orbitals();
+    if (!Smat || !Fmat || n <= 0 || nocc < 0 || nocc > n)
+        return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_diag_fock: bad argument");
+    Scf c;
+    if (!c.init(n)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock");
+    double *dS = c.mem.get<double>(c.no2), *dF = c.mem.get<double>(c.no2), *dP = c.mem.get<double>(c.no2);
+    if (!dS || !dF || !dP) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock: device allocation failed");
+    if (hipMemcpyAsync(dS, Smat, sizeof(double) * c.no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
+        hipMemcpyAsync(dF, Fmat, sizeof(double) * c.no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
+        return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock: upload failed");
+    if (!c.form_x(dS) || !c.diag_density(dF, nullptr, nocc, dP)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock");
+    bool ok = true;
+    if (evals) ok = ok && hipMemcpy(evals, c.ev, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess;
+    if (Cmat) ok = ok && download_rowmajor(c, c.C, Cmat);
+    if (Pmat) ok = ok && hipMemcpy(Pmat, dP, sizeof(double) * c.no2, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock: download failed");
+    return 0;
+}

@@ -10,6 +10,7 @@
  *   uhf_form_gmatrix   replaces scfaux.h:24-26  (scfaux.c:554-714)
  *   dpm_ints           replaces dpmints.h:12    (dpmints.c:120-199)
  *   one_electron_ints  replaces oneints.h      (oneints.c:137-234)
+ *   momints            replaces momints.h      (momints.c:94-190)
  * Instead of writing ERINT files, two_electron_ints builds the device-side
  * shell-pair and screened work tables for the basis current at the call;
  * rhf_form_gmatrix/uhf_form_gmatrix then run a direct (recompute) J/K build
@@ -46,6 +47,9 @@ void dpm_ints(double *hmat);
 /* S, T and H = T + V of the current basis into caller-owned packed no2
  * arrays (assigned, not accumulated); V leaves out skip_center(). */
 void one_electron_ints(double *Smat, double *Tmat, double *Hmat);
+/* MOMINTS.DAT in the working directory: the reference's records
+ * (int ir, int jr, double dip[3], double quad[6]) in its order. */
+void momints(void);
 
 /* ---- status ---- */
 #define PATMOL_HIP_OK 0
@@ -70,6 +74,14 @@ void patmol_hip_set_precision(int mode, double threshold);
  * Asynchronous with respect to the host. */
 int patmol_hip_rhf_gmatrix_device(const ERIFS *tfiles, const double *d_P, double *d_G, void *stream);
 
+/* ---- multipole integrals (momints.c:14-92) ---- */
+/* Dipole (3 x no2: x, y, z) and quadrupole (6 x no2: xx, xy, xz, yy, yz, zz)
+ * integrals about the coordinate origin, packed lower triangles,
+ * component-major, host arrays. */
+int patmol_hip_moment_ints(double *dip, double *quad);
+/* momints() with an explicit output path. */
+int patmol_hip_write_momints(const char *path);
+
 /* ---- device SCF (the whole iteration in HBM) ---- */
 /* The reference's RHF / UHF iteration (rhfscf.c:21-95, uhfscf.c:22-100)
  * with every matrix on the GPU: G builds of `mints` (and `xints`, may be
@@ -91,6 +103,7 @@ typedef struct {
     double gbuild_ms;   /* device time in the G builds */
     double other_ms;    /* the rest of the loop (eigensolver, density, sums, host logic) */
     double total_ms;
+    double energy0;     /* energy after the first update (rhfscf.c:86, 191) */
 } PatmolHipScfResult;
 int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Smat, const double *Hmat,
                        int nocc, int maxits, double scf_eps, int scf_accel, double enuc,

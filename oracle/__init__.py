@@ -78,6 +78,10 @@ def lib():
         L.or_g_rows.argtypes = [C.c_void_p, _dp, C.c_int, _ip, _dp, C.c_int]
         L.or_one_electron_ints.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.or_dpm_ints.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_moment_ints.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_rhf_scf_ex.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int64,
+                                    C.c_void_p, C.c_void_p, _dp, _dp, C.POINTER(ScfResult), C.c_int,
+                                    C.POINTER(C.c_double)]
         L.or_rhf_scf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p,
                                  C.c_int64, _dp, _dp, C.POINTER(ScfResult), C.c_int]
         L.or_uhf_scf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -219,6 +223,13 @@ class Basis:
         S, T, H = np.zeros(no2), np.zeros(no2), np.zeros(no2)
         lib().or_one_electron_ints(self.h, S, T, H)
         return S, T, H
+
+    def moment_ints(self):
+        """Dipole (3, no2) and quadrupole (6, no2) integrals about the origin."""
+        no2 = self.norb * (self.norb + 1) // 2
+        d, q = np.zeros(3 * no2), np.zeros(6 * no2)
+        lib().or_moment_ints(self.h, d, q)
+        return d.reshape(3, no2), q.reshape(6, no2)
 
     def dpm_ints(self, Q, hmat):
         lib().or_dpm_ints(self.h, np.ascontiguousarray(Q, dtype=np.float64), hmat)
@@ -391,3 +402,56 @@ def work_items_mirror(system, q_orig, tau, start=0):
     bounds = np.array([it[3] for it in items])
     return dict(tiles=tiles, classes=classes, bounds=bounds, sorted_orig=np.array(order, dtype=np.int32),
                 tile_first=np.array(tf, dtype=np.int32), tile_size=np.array(tsz, dtype=np.int32))
+
+
+def rhf_scf_ex(basis: Basis, ints: EriList, nelec, H=None, guess=None, maxits=200, eps=1e-10, scf_a=2,
+               nthreads=1):
+    """or_rhf_scf with an optional one-electron H and guess density; returns
+    (result, P, orbital energies, energy after the first update)."""
+    no2 = basis.norb * (basis.norb + 1) // 2
+    P = np.zeros(no2)
+    ev = np.zeros(basis.norb)
+    res = ScfResult()
+    e0 = C.c_double()
+    h = None if H is None else np.ascontiguousarray(H, dtype=np.float64)
+    g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+    rc = lib().or_rhf_scf_ex(basis.h, nelec, maxits, eps, scf_a, ints.ptr, ints.n,
+                             None if h is None else h.ctypes.data, None if g is None else g.ctypes.data,
+                             P, ev, C.byref(res), nthreads, C.byref(e0))
+    assert rc == 0
+    return res, P, ev, e0.value
+
+
+def rhf_moments(P, dip, quad, charges, coords):
+    """rhf_analyzeMoments (scfaux.c:140-214): electronic moments
+    sum over packed ij of f P_ij M_ij with f = 4 off the diagonal and 2 on it
+    (P is the half density), nuclear sum q_c r_c / q_c r_c r_c, total =
+    nuclear - electronic; 9 values each (x, y, z, xx, xy, xz, yy, yz, zz)."""
+    n = int(round((np.sqrt(8 * len(P) + 1) - 1) / 2))
+    f = np.full(len(P), 4.0)
+    f[[i * (i + 1) // 2 + i for i in range(n)]] = 2.0
+    M = np.vstack([dip, quad])
+    elec = (M * (f * P)).sum(axis=1)
+    r = np.asarray(coords, dtype=np.float64)
+    q = np.asarray(charges, dtype=np.float64)
+    nuc = np.array([(q * r[:, 0]).sum(), (q * r[:, 1]).sum(), (q * r[:, 2]).sum(),
+                    (q * r[:, 0] * r[:, 0]).sum(), (q * r[:, 0] * r[:, 1]).sum(), (q * r[:, 0] * r[:, 2]).sum(),
+                    (q * r[:, 1] * r[:, 1]).sum(), (q * r[:, 1] * r[:, 2]).sum(), (q * r[:, 2] * r[:, 2]).sum()])
+    return elec, nuc, nuc - elec
+
+
+def rhf_finite_field(basis: Basis, ints: EriList, nelec, efx, maxits=200, eps=1e-10, scf_a=2):
+    """rhf_finite_field_calculation (rhfscf.c:176-215): from the ground-state
+    density, per direction H = H_gs - efx d_dir (rhf_getDipoleHam,
+    rhfscf.c:474-489), SCF, alpha = -2 (E - E_first) / efx^2; then
+    alpha0 = mean and alpha2 = (2 a_zz - a_yy - a_zz) / 3 as rhfscf.c:211
+    writes it.  Returns (alpha[3], alpha0, alpha2)."""
+    _, _, H = basis.one_electron_ints()
+    gs, Pgs, _, _ = rhf_scf_ex(basis, ints, nelec, maxits=maxits, eps=eps, scf_a=scf_a)
+    dip, _ = basis.moment_ints()
+    alfa = np.zeros(3)
+    for d in range(3):
+        res, _, _, e0 = rhf_scf_ex(basis, ints, nelec, H=H - efx * dip[d], guess=Pgs, maxits=maxits, eps=eps,
+                                   scf_a=scf_a)
+        alfa[d] = -2.0 * (res.energy - e0) / efx / efx
+    return alfa, alfa.mean(), (2.0 * alfa[2] - alfa[1] - alfa[2]) / 3.0

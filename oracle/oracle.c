@@ -21,6 +21,8 @@
  *   or_*_form_gmatrix   scfaux.c:428-714                  einsum over the full 4-index tensor
  *   or_one_electron_ints oneints.c:12-234                 reference one_electron_ints (_ref); SF6;
  *                                                         closed forms for s/p primitives
+ *   or_moment_ints      momints.c:14-190                  reference momints (_ref, MOMINTS.DAT);
+ *                                                         s-type closed forms
  *   or_dpm_ints         dpmints.c:18-199                  reference dpm_ints (_ref); s-type closed form
  *   or_rhf_scf          rhfscf.c:21-174,432-471           SF6 golden: E, 43 iterations, E_nuc
  *   or_uhf_scf          uhfscf.c:22-168,485-527           closed-shell UHF == RHF (parity unpinned
@@ -1130,6 +1132,81 @@ void or_one_electron_ints(const OBasis *bs, double *S, double *Tm, double *H)
     free(rterm);
 }
 
+/* Dipole and quadrupole integrals about the origin (momints.c:14-92) from
+ * Hermite E coefficients: per axis <x> = E_0 P + E_1 and
+ * <x^2> = E_0 (P^2 + 1/(2p)) + 2 P E_1 + 2 E_2 (times sqrt(pi/p), carried
+ * by s12), products over axes for xy, xz, yz.  dip: 3 x no2, quad: 6 x no2
+ * (xx, xy, xz, yy, yz, zz), packed, component-major. */
+void or_moment_ints(const OBasis *bs, double *dip, double *quad)
+{
+    const double piterm = 5.5683279968317079;
+    const size_t no2 = (size_t)bs->norb * (bs->norb + 1) / 2;
+    static double dx[EMAX * EMAX * (2 * EMAX + 1)], dy[EMAX * EMAX * (2 * EMAX + 1)],
+        dz[EMAX * EMAX * (2 * EMAX + 1)];
+    for (int ish = 0; ish < bs->nshell; ++ish) {
+        const OShell *s1 = &bs->sh[ish];
+        const double *a = bs->cen[s1->cen].r;
+        int lv1 = s1->lsh;
+        for (int jsh = 0; jsh <= ish; ++jsh) {
+            const OShell *s2 = &bs->sh[jsh];
+            const double *b = bs->cen[s2->cen].r;
+            int lv2 = s2->lsh;
+            double absq = 0;
+            for (int x = 0; x < 3; ++x) absq += (a[x] - b[x]) * (a[x] - b[x]);
+            double mv[9][OR_MAXLST * OR_MAXLST];
+            memset(mv, 0, sizeof mv);
+            for (int ip = 0; ip < s1->npr; ip++) {
+                double axp = s1->al[ip], c1 = s1->co[ip];
+                for (int jp = 0; jp < s2->npr; jp++) {
+                    double c12 = c1 * s2->co[jp], bxp = s2->al[jp];
+                    double pxp = axp + bxp, abi = 1.0 / pxp;
+                    double s12 = piterm * exp(-abi * axp * bxp * absq) * abi * sqrt(abi);
+                    double p[3];
+                    for (int x = 0; x < 3; ++x) p[x] = (axp * a[x] + bxp * b[x]) * abi;
+                    abi *= 0.5;
+                    or_hermite_e(dx, lv1 + 2, lv2 + 2, abi, p[0] - a[0], p[0] - b[0]);
+                    or_hermite_e(dy, lv1 + 2, lv2 + 2, abi, p[1] - a[1], p[1] - b[1]);
+                    or_hermite_e(dz, lv1 + 2, lv2 + 2, abi, p[2] - a[2], p[2] - b[2]);
+                    for (int ia = 0; ia < s1->nst; ++ia)
+                        for (int ib = 0; ib < s2->nst; ++ib) {
+                            int kc = ia * s2->nst + ib;
+                            double ovl = s12 * c12 * bs->nf[lv1][ia] * bs->nf[lv2][ib];
+                            int l1 = bs->lxyz[lv1][ia][0], m1 = bs->lxyz[lv1][ia][1], n1 = bs->lxyz[lv1][ia][2];
+                            int l2 = bs->lxyz[lv2][ib][0], m2 = bs->lxyz[lv2][ib][1], n2 = bs->lxyz[lv2][ib][2];
+                            const double *ex = &dx[EIDX(l1, l2, 0)], *ey = &dy[EIDX(m1, m2, 0)],
+                                         *ez = &dz[EIDX(n1, n2, 0)];
+                            /* first and second moments per axis */
+                            double m1x = ex[0] * p[0] + ex[1], m1y = ey[0] * p[1] + ey[1], m1z = ez[0] * p[2] + ez[1];
+                            double m2x = ex[0] * (p[0] * p[0] + abi) + 2.0 * (p[0] * ex[1] + ex[2]);
+                            double m2y = ey[0] * (p[1] * p[1] + abi) + 2.0 * (p[1] * ey[1] + ey[2]);
+                            double m2z = ez[0] * (p[2] * p[2] + abi) + 2.0 * (p[2] * ez[1] + ez[2]);
+                            mv[0][kc] += ovl * m1x * ey[0] * ez[0];
+                            mv[1][kc] += ovl * ex[0] * m1y * ez[0];
+                            mv[2][kc] += ovl * ex[0] * ey[0] * m1z;
+                            mv[3][kc] += ovl * m2x * ey[0] * ez[0];
+                            mv[4][kc] += ovl * m1x * m1y * ez[0];
+                            mv[5][kc] += ovl * m1x * ey[0] * m1z;
+                            mv[6][kc] += ovl * ex[0] * m2y * ez[0];
+                            mv[7][kc] += ovl * ex[0] * m1y * m1z;
+                            mv[8][kc] += ovl * ex[0] * ey[0] * m2z;
+                        }
+                }
+            }
+            for (int ia = 0; ia < s1->nst; ++ia) {
+                int ir = s1->off + ia;
+                for (int ib = 0; ib < s2->nst; ++ib) {
+                    int jr = s2->off + ib;
+                    if (jr > ir) break;
+                    size_t ij = (size_t)ir * (ir + 1) / 2 + jr;
+                    int kc = ia * s2->nst + ib;
+                    for (int m = 0; m < 3; ++m) dip[m * no2 + ij] = mv[m][kc];
+                    for (int m = 0; m < 6; ++m) quad[m * no2 + ij] = mv[3 + m][kc];
+                }
+            }
+        }
+    }
+}
+
 /* DPM potential, restating dpm_ints/calcPDPM (dpmints.c:18-199):
  * hmat[ij] -= sum_k c_k (ij | e^{-a_k |r-Q|^2}) with the fixed six-term fit
  * of a unit charge (dpmints.c:31-36). */
@@ -1526,8 +1603,14 @@ static void scf_accelerator(int iter, int no2, double *p0, double *p1, double *p
     }
 }
 
-int or_rhf_scf(const OBasis *b, int nelec, int maxits, double eps, int scf_a, const OERINT *ints,
-               int64_t nints, double *P_out, double *evals_out, OScfResult *res, int nthreads)
+/* Hin (packed, may be NULL) replaces the one-electron H, Pguess (may be
+ * NULL) the core-Hamiltonian guess; energy0 (may be NULL) receives the
+ * energy after the first update, the reference's enrg0 (rhfscf.c:86,191).
+ * With Hin = H - F.d and Pguess = the ground-state density this is one
+ * direction of rhf_finite_field_calculation (rhfscf.c:176-202). */
+int or_rhf_scf_ex(const OBasis *b, int nelec, int maxits, double eps, int scf_a, const OERINT *ints,
+                  int64_t nints, const double *Hin, const double *Pguess, double *P_out, double *evals_out,
+                  OScfResult *res, int nthreads, double *energy0)
 {
     const double EPS10 = 10 * 2.220446049250313e-16;
     int n = b->norb, nocc = nelec / 2;
@@ -1536,10 +1619,15 @@ int or_rhf_scf(const OBasis *b, int nelec, int maxits, double eps, int scf_a, co
     double *F = calloc(no2, 8), *P = calloc(no2, 8), *P1 = calloc(no2, 8), *P2 = calloc(no2, 8);
     double *X = calloc((size_t)n * n, 8), *C = calloc((size_t)n * n, 8), *ev = calloc(n, 8);
     or_one_electron_ints(b, S, Tm, H);
+    if (Hin) memcpy(H, Hin, no2 * 8);
     if (eps < EPS10 * n * n) eps = EPS10 * n * n; /* rhfscf.c:161-164 */
     if (form_x(n, S, X)) return -1;
-    diag_fock(n, H, X, C, ev); /* core guess, rhfscf.c:454-471 */
-    form_p(n, nocc, C, P);
+    if (Pguess) {
+        memcpy(P, Pguess, no2 * 8);
+    } else {
+        diag_fock(n, H, X, C, ev); /* core guess, rhfscf.c:454-471 */
+        form_p(n, nocc, C, P);
+    }
     double enuc = or_nuclear_repulsion(b), eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0;
     int iteration = 0, done = 0, first = 1;
     for (;;) {
@@ -1570,6 +1658,7 @@ int or_rhf_scf(const OBasis *b, int nelec, int maxits, double eps, int scf_a, co
         if (first) { /* rhf_energy, rhfscf.c:80-95 */
             first = 0;
             iteration = 0;
+            if (energy0) *energy0 = enrg + enuc;
             continue;
         }
         done = ret;
@@ -1587,6 +1676,13 @@ int or_rhf_scf(const OBasis *b, int nelec, int maxits, double eps, int scf_a, co
     free(S); free(Tm); free(H); free(G); free(F); free(P); free(P1); free(P2);
     free(X); free(C); free(ev);
     return 0;
+}
+
+int or_rhf_scf(const OBasis *b, int nelec, int maxits, double eps, int scf_a, const OERINT *ints,
+               int64_t nints, double *P_out, double *evals_out, OScfResult *res, int nthreads)
+{
+    return or_rhf_scf_ex(b, nelec, maxits, eps, scf_a, ints, nints, NULL, NULL, P_out, evals_out, res, nthreads,
+                         NULL);
 }
 
 int or_uhf_scf(const OBasis *b, int nelec, int mult, int maxits, double eps, int scf_a,

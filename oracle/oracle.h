@@ -88,6 +88,7 @@ void or_g_rows(const OBasis *b, const double *Pfull, int nrows, const int *rows,
 
 /* ---- one-electron / DPM ---- */
 void or_one_electron_ints(const OBasis *b, double *S, double *T, double *H);
+void or_moment_ints(const OBasis *b, double *dip, double *quad);
 void or_dpm_ints(const OBasis *b, const double *q, double *hmat);
 
 /* ---- SCF (rhfscf.c:21-174,432-471; uhfscf.c:22-168,485-527) ---- */
@@ -96,6 +97,9 @@ typedef struct {
     int iterations, converged;
 } OScfResult;
 
+int or_rhf_scf_ex(const OBasis *b, int nelec, int maxits, double eps, int scf_a, const OERINT *ints,
+                  int64_t nints, const double *Hin, const double *Pguess, double *P_out, double *evals_out,
+                  OScfResult *res, int nthreads, double *energy0);
 int or_rhf_scf(const OBasis *b, int nelec, int maxits, double eps, int scf_a,
                const OERINT *ints, int64_t nints, double *P_out, double *evals_out,
                OScfResult *res, int nthreads);

@@ -55,6 +55,7 @@ def lib():
         L.ref_spack_diag.argtypes = [C.c_int, _dp, _dp, _dp]
         L.ref_diag_fock.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.ref_check_phase.argtypes = [C.c_int, _dp]
+        L.momints.argtypes = []
         _lib = L
     return _lib
 
@@ -114,3 +115,18 @@ def diag_fock(S, F, nocc):
     ev, Cm, P = np.zeros(n), np.zeros(n * n), np.zeros(no2)
     lib().ref_diag_fock(n, np.ascontiguousarray(S), np.ascontiguousarray(F), nocc, ev, Cm, P)
     return ev, Cm.reshape(n, n), P
+
+
+MOMINTS_DTYPE = np.dtype([("ir", "<i4"), ("jr", "<i4"), ("dip", "<f8", 3), ("quad", "<f8", 6)])
+
+
+def momints(workdir: str):
+    """The reference's momints() (momints.c:94-190) run in workdir; returns
+    the records of the MOMINTS.DAT it writes there."""
+    cwd = os.getcwd()
+    os.chdir(workdir)
+    try:
+        lib().momints()
+    finally:
+        os.chdir(cwd)
+    return np.fromfile(os.path.join(workdir, "MOMINTS.DAT"), dtype=MOMINTS_DTYPE)

@@ -60,7 +60,8 @@ class ScfResult(C.Structure):
 class HipScfResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("eelec", C.c_double), ("enuc", C.c_double), ("ediff", C.c_double),
                 ("pdiff", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
-                ("gbuild_ms", C.c_double), ("other_ms", C.c_double), ("total_ms", C.c_double)]
+                ("gbuild_ms", C.c_double), ("other_ms", C.c_double), ("total_ms", C.c_double),
+                ("energy0", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,8 +135,15 @@ def lib():
         G.patmol_hip_diag_fock.restype = C.c_int
         G.patmol_hip_diag_fock.argtypes = [_dp, _dp, C.c_int, _odp, _odp, _odp]
         G.patmol_hip_uhf_gmatrix_device.restype = C.c_int
+        G.patmol_hip_moment_ints.restype = C.c_int
+        G.patmol_hip_moment_ints.argtypes = [_dp, _dp]
+        G.patmol_hip_write_momints.restype = C.c_int
+        G.patmol_hip_write_momints.argtypes = [C.c_char_p]
         G.patmol_hip_uhf_gmatrix_device.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         H.scf_convergence.restype = C.c_double
+        H.patmol_rhf_moments.argtypes = [_dp, _dp, _dp, _dp]
+        H.patmol_uhf_moments.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        H.patmol_rhf_finite_field.argtypes = [C.c_double, _dp, C.POINTER(ScfResult)]
         _host, _hip = H, G
     return _host, _hip
 
@@ -156,7 +164,7 @@ def exported_symbols():
             "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
-            "patmol_hip_uhf_gmatrix_device"]
+            "patmol_hip_uhf_gmatrix_device", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints"]
 
 
 def load_system(sys_: "systems.System") -> None:
@@ -418,3 +426,58 @@ def diag_fock(S: np.ndarray, F: np.ndarray, nocc: int):
     if rc:
         _check("patmol_hip_diag_fock")
     return ev, Cm, P
+
+
+def moment_ints():
+    """Dipole (3, no2) and quadrupole (6, no2) integrals of the current basis
+    about the origin, computed on the GPU."""
+    H, G = lib()
+    n = H.number_of_orbitals()
+    no2 = n * (n + 1) // 2
+    d, q = np.zeros(3 * no2), np.zeros(6 * no2)
+    if G.patmol_hip_moment_ints(d, q):
+        _check("patmol_hip_moment_ints")
+    return d.reshape(3, no2), q.reshape(6, no2)
+
+
+MOMINTS_DTYPE = np.dtype([("ir", "<i4"), ("jr", "<i4"), ("dip", "<f8", 3), ("quad", "<f8", 6)])
+
+
+def write_momints(path: str) -> None:
+    _, G = lib()
+    if G.patmol_hip_write_momints(path.encode()):
+        _check("patmol_hip_write_momints")
+
+
+def rhf_moments(P: np.ndarray):
+    """rhf_analyzeMoments: (electronic, nuclear, total) moments, 9 values each
+    (x, y, z, xx, xy, xz, yy, yz, zz), for half density P."""
+    H, _ = lib()
+    e, n, t = np.zeros(9), np.zeros(9), np.zeros(9)
+    if H.patmol_rhf_moments(np.ascontiguousarray(P, dtype=np.float64), e, n, t):
+        _check("patmol_rhf_moments")
+        raise PatmolError("patmol_rhf_moments failed")
+    return e, n, t
+
+
+def uhf_moments(PA: np.ndarray, PB: np.ndarray):
+    H, _ = lib()
+    e, n, t = np.zeros(9), np.zeros(9), np.zeros(9)
+    if H.patmol_uhf_moments(np.ascontiguousarray(PA, dtype=np.float64), np.ascontiguousarray(PB, dtype=np.float64),
+                            e, n, t):
+        _check("patmol_uhf_moments")
+        raise PatmolError("patmol_uhf_moments failed")
+    return e, n, t
+
+
+def rhf_finite_field(efx: float):
+    """rhf_finite_field_calculation on the device SCF: (alfa[5], ground-state
+    result); alfa = xx, yy, zz, mean, and the reference's alfa2."""
+    H, _ = lib()
+    a = np.zeros(5)
+    gs = ScfResult()
+    rc = H.patmol_rhf_finite_field(efx, a, C.byref(gs))
+    if rc:
+        _check("patmol_rhf_finite_field")
+        raise PatmolError(f"patmol_rhf_finite_field failed ({rc})")
+    return a, gs.as_dict()

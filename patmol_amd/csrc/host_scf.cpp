@@ -439,3 +439,125 @@ extern "C" int patmol_uhf_scf(double *PA_out, double *PB_out, PatmolScfResult *r
     if (PB_out) memcpy(PB_out, PB.data(), no2 * sizeof(double));
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Multipole analysis and finite field
+// ---------------------------------------------------------------------------
+namespace {
+
+// nuclear moments sum_c q_c r_c and q_c r_c r_c (scfaux.c:194-215)
+void nuclear_moments(double *nuc)
+{
+    const Center *ce = centers();
+    const int nc = number_of_centers();
+    for (int m = 0; m < 9; ++m) nuc[m] = 0.0;
+    for (int c = 0; c < nc; ++c) {
+        const double q = ce[c].q, *r = ce[c].r;
+        nuc[0] += q * r[0];
+        nuc[1] += q * r[1];
+        nuc[2] += q * r[2];
+        nuc[3] += q * r[0] * r[0];
+        nuc[4] += q * r[0] * r[1];
+        nuc[5] += q * r[0] * r[2];
+        nuc[6] += q * r[1] * r[1];
+        nuc[7] += q * r[1] * r[2];
+        nuc[8] += q * r[2] * r[2];
+    }
+}
+
+// electronic moments sum_ij f_ij D_ij M_ij over the packed triangle
+int moments_common(const double *D, double off, double diag, double *elec, double *nuc, double *total)
+{
+    const int n = number_of_orbitals();
+    const size_t no2 = (size_t)n * (n + 1) / 2;
+    std::vector<double> dip(3 * no2), quad(6 * no2);
+    if (patmol_hip_moment_ints(dip.data(), quad.data())) return 2;
+    for (int m = 0; m < 9; ++m) elec[m] = 0.0;
+    for (int i = 0, ij = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j, ++ij) {
+            const double p = (i == j ? diag : off) * D[ij];
+            for (int m = 0; m < 3; ++m) elec[m] += p * dip[m * no2 + ij];
+            for (int m = 0; m < 6; ++m) elec[3 + m] += p * quad[m * no2 + ij];
+        }
+    nuclear_moments(nuc);
+    for (int m = 0; m < 9; ++m) total[m] = nuc[m] - elec[m];
+    return 0;
+}
+
+}  // namespace
+
+// rhf_analyzeMoments (scfaux.c:140-214) without the MOMINTS.DAT round trip:
+// P is the half density, electronic moments weigh off-diagonal elements 4 and
+// diagonal ones 2; 9 values (x, y, z, xx, xy, xz, yy, yz, zz) each.
+extern "C" int patmol_rhf_moments(const double *P, double *elec, double *nuc, double *total)
+{
+    if (!P || !elec || !nuc || !total) return 1;
+    return moments_common(P, 4.0, 2.0, elec, nuc, total);
+}
+
+// uhf_analyzeMoments (scfaux.c:267-340): weights 2 and 1 on PA + PB.
+extern "C" int patmol_uhf_moments(const double *PA, const double *PB, double *elec, double *nuc, double *total)
+{
+    if (!PA || !PB || !elec || !nuc || !total) return 1;
+    const int n = number_of_orbitals();
+    const size_t no2 = (size_t)n * (n + 1) / 2;
+    std::vector<double> D(no2);
+    for (size_t i = 0; i < no2; ++i) D[i] = PA[i] + PB[i];
+    return moments_common(D.data(), 2.0, 1.0, elec, nuc, total);
+}
+
+// rhf_finite_field_calculation (rhfscf.c:176-215) through the device SCF:
+// ground state, then per direction H = H_gs - efx d_dir (rhf_getDipoleHam,
+// rhfscf.c:474-489) from the ground-state density, and
+// alpha_dir = -2 (E - E_first) / efx^2 with E_first the energy after the first
+// update (enrg0, rhfscf.c:191).  alfa[0..2] = xx, yy, zz, alfa[3] = their
+// mean, alfa[4] = (2 a_zz - a_yy - a_zz) / 3 exactly as rhfscf.c:211 writes it.
+extern "C" int patmol_rhf_finite_field(double efx, double *alfa, PatmolScfResult *gs)
+{
+    const int n = number_of_orbitals(), ne = number_of_electrons();
+    if (!alfa || efx == 0.0 || ne % 2 || spin_multiplicity() != 1) return 1;
+    const size_t no2 = (size_t)n * (n + 1) / 2;
+    std::vector<double> S(no2), T(no2), H(no2), Hf(no2), Pgs(no2), dip(3 * no2), quad(6 * no2);
+    one_electron_ints(S.data(), T.data(), H.data());
+    if (patmol_hip_last_error()) return 2;
+    if (patmol_hip_moment_ints(dip.data(), quad.data())) return 2;
+    ERIFS *mints = init_ERIFS(0, n, "MINTS.DAT");
+    two_electron_ints(mints, 0);
+    if (patmol_hip_last_error()) {
+        clean_ERIFS(mints);
+        return 2;
+    }
+    const int maxits = max_scf_iterations(), accel = scf_acceleration();
+    const double eps = scf_convergence(), enuc = nuclear_repulsion();
+    PatmolHipScfResult r{};
+    if (patmol_hip_rhf_scf(mints, nullptr, S.data(), H.data(), ne / 2, maxits, eps, accel, enuc, nullptr,
+                           Pgs.data(), nullptr, nullptr, &r)) {
+        clean_ERIFS(mints);
+        return 2;
+    }
+    if (gs) {
+        gs->energy = r.energy;
+        gs->eelec = r.eelec;
+        gs->enuc = r.enuc;
+        gs->ediff = r.ediff;
+        gs->pdiff = r.pdiff;
+        gs->iterations = r.iterations;
+        gs->converged = r.converged;
+        gs->gbuild_seconds = r.gbuild_ms * 1e-3;
+        gs->diag_seconds = r.other_ms * 1e-3;
+    }
+    for (int d = 0; d < 3; ++d) {
+        for (size_t i = 0; i < no2; ++i) Hf[i] = H[i] - efx * dip[d * no2 + i];
+        PatmolHipScfResult rf{};
+        if (patmol_hip_rhf_scf(mints, nullptr, S.data(), Hf.data(), ne / 2, maxits, eps, accel, enuc, Pgs.data(),
+                               nullptr, nullptr, nullptr, &rf)) {
+            clean_ERIFS(mints);
+            return 2;
+        }
+        alfa[d] = -2.0 * (rf.energy - rf.energy0) / efx / efx;
+    }
+    clean_ERIFS(mints);
+    alfa[3] = (alfa[0] + alfa[1] + alfa[2]) / 3.0;
+    alfa[4] = (2.0 * alfa[2] - alfa[1] - alfa[2]) / 3.0;
+    return 0;
+}

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -57,6 +58,8 @@ struct OneArgs {
     const double4 *nuc;       // point nuclei (q, x, y, z), skip_center() left out
     int npairs, nnuc;
     double *S, *T, *H;        // packed lower triangles
+    double *M;                // moment integrals: 9 packed triangles
+    size_t no2;
     RysTables rys;
 };
 
@@ -208,6 +211,98 @@ __global__ __launch_bounds__(64) void onee_kernel(OneArgs a)
     });
 }
 
+// Dipole and quadrupole integrals about the coordinate origin (momints.c:14-92):
+// <a|x|b>, <a|y|b>, <a|z|b>, then <a|xx|b>, xy, xz, yy, yz, zz.  Per axis the
+// 1-D factors are the overlap S(i,j), the first moment
+// M1(i,j) = S(i+1,j) + A S(i,j) and the second M2(i,j) = S(i+2,j)
+// + 2A S(i+1,j) + A^2 S(i,j) (x = (x - A) + A), from the same Obara-Saika
+// overlaps; the reference forms the same quantities from Hermite E
+// coefficients (E_0 P + E_1 and E_0 (P^2 + 1/2p) + 2P E_1 + 2 E_2).  One
+// moment component per pass over the primitive pairs keeps NA*NB
+// accumulators live instead of 9 NA*NB.
+// exponent of axis `ax` in moment component m (x, y, z, xx, xy, xz, yy, yz, zz)
+constexpr int mom_pow(int m, int ax)
+{
+    constexpr int t[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0},
+                             {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}};
+    return t[m][ax];
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void moment_kernel(OneArgs a)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.npairs) return;
+    const int2 sh = a.pairs[t];
+    double A[3], AB[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        A[x] = a.xyz[3 * sh.x + x];
+        AB[x] = A[x] - a.xyz[3 * sh.y + x];
+    }
+    const PairRef ref = a.refs[t];
+    const int oI = a.off[sh.x], oJ = a.off[sh.y];
+    const bool same = sh.x == sh.y;
+    static_for<9>([&](auto mc) {
+        constexpr int M = decltype(mc)::value;
+        double acc[NAB];
+#pragma unroll
+        for (int i = 0; i < NAB; ++i) acc[i] = 0.0;
+        for (int k = 0; k < ref.npp; ++k) {
+            const PrimPair q = a.pp[ref.pp0 + k];
+            const double hip = 0.5 * q.ip;
+            double o[3][LA + 3][LB + 1];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) overlap_1d<LA + 2, LB>(o[x], q.P[x] - A[x], AB[x], hip);
+            const double s12 = PI32 * q.s * q.ip * sqrt(q.ip) * q.c;
+            auto f1 = [&](int x, int i, int j, int pwr) {
+                const double *r = &o[x][0][0];
+                const double s0 = r[i * (LB + 1) + j];
+                if (pwr == 0) return s0;
+                const double s1 = r[(i + 1) * (LB + 1) + j];
+                if (pwr == 1) return s1 + A[x] * s0;
+                return r[(i + 2) * (LB + 1) + j] + 2.0 * A[x] * s1 + A[x] * A[x] * s0;
+            };
+            static_for<NA>([&](auto ia) {
+                static_for<NB>([&](auto ib) {
+                    constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+                    constexpr int ax = cart_x(LA, IA), ay = cart_y(LA, IA), az = cart_z(LA, IA);
+                    constexpr int bx = cart_x(LB, IB), by = cart_y(LB, IB), bz = cart_z(LB, IB);
+                    acc[IA * NB + IB] += s12 * f1(0, ax, bx, mom_pow(M, 0)) * f1(1, ay, by, mom_pow(M, 1)) *
+                                         f1(2, az, bz, mom_pow(M, 2));
+                });
+            });
+        }
+        double *out = a.M + (size_t)M * a.no2;
+        static_for<NA>([&](auto ia) {
+            static_for<NB>([&](auto ib) {
+                constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+                constexpr double nrm = 1.0 / csqrt(cart_dfprod(LA, IA) * cart_dfprod(LB, IB));
+                const int r = oI + IA, c = oJ + IB;
+                if (!(same && c > r)) {
+                    const int hi = r > c ? r : c, lo = r > c ? c : r;
+                    out[(size_t)hi * (hi + 1) / 2 + lo] = nrm * acc[IA * NB + IB];
+                }
+            });
+        });
+    });
+}
+
+int launch_moment(int la, int lb, const OneArgs &args, hipStream_t s)
+{
+    dim3 g((args.npairs + 63) / 64), b(64);
+    switch (la * 3 + lb) {
+    case 0: hipLaunchKernelGGL((moment_kernel<0, 0>), g, b, 0, s, args); return 0;
+    case 3: hipLaunchKernelGGL((moment_kernel<1, 0>), g, b, 0, s, args); return 0;
+    case 4: hipLaunchKernelGGL((moment_kernel<1, 1>), g, b, 0, s, args); return 0;
+    case 6: hipLaunchKernelGGL((moment_kernel<2, 0>), g, b, 0, s, args); return 0;
+    case 7: hipLaunchKernelGGL((moment_kernel<2, 1>), g, b, 0, s, args); return 0;
+    case 8: hipLaunchKernelGGL((moment_kernel<2, 2>), g, b, 0, s, args); return 0;
+    default: return -1;
+    }
+}
+
 int launch_onee(int la, int lb, const OneArgs &args, hipStream_t s)
 {
     dim3 g((args.npairs + 63) / 64), b(64);
@@ -230,44 +325,67 @@ namespace {
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-}  // namespace
+constexpr int kClassL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 
-}  // namespace pm
+// Class-grouped shell-pair tables of the current basis in one device block,
+// followed by `nout` doubles of output space.
+struct PairStage {
+    char *d = nullptr;
+    size_t o_pairs = 0, o_refs = 0, o_pp = 0, o_ab = 0, o_xyz = 0, o_off = 0, o_nuc = 0, o_out = 0, no2 = 0;
+    int cls_first[7] = {0};
+    int nnuc = 0;
+    ~PairStage()
+    {
+        if (d) (void)hipFree(d);
+    }
+    double *out() const { return (double *)(d + o_out); }
+    OneArgs args(int c, const RysTables &rys) const
+    {
+        OneArgs a{};
+        a.pairs = (const int2 *)(d + o_pairs) + cls_first[c];
+        a.refs = (const PairRef *)(d + o_refs) + cls_first[c];
+        a.pp = (const PrimPair *)(d + o_pp);
+        a.ab = (const double2 *)(d + o_ab);
+        a.xyz = (const double *)(d + o_xyz);
+        a.off = (const int *)(d + o_off);
+        a.nuc = (const double4 *)(d + o_nuc);
+        a.npairs = cls_first[c + 1] - cls_first[c];
+        a.nnuc = nnuc;
+        a.no2 = no2;
+        a.rys = rys;
+        return a;
+    }
+};
 
-// oneints.h: void one_electron_ints(double *Smat, double *Tmat, double *Hmat)
-extern "C" void one_electron_ints(double *Smat, double *Tmat, double *Hmat)
+// Every shell pair I >= J (oneints.c:147-155), first shell of the higher l,
+// grouped by class; their primitive pairs; shell centres and offsets; the
+// point nuclei except skip_center() (oneints.c:51-53).
+int stage_pairs(PairStage &st, size_t nout_per_no2, const char *who)
 {
-    using namespace pm;
     const Shell *sh = shells();
     const Center *ce = centers();
     const int ns = number_of_shells(), nc = number_of_centers(), skip = skip_center(), N = number_of_orbitals();
-    if (!sh || !ce || ns <= 0 || N <= 0 || !Smat || !Tmat || !Hmat) {
-        aux_fail(PATMOL_HIP_EBADARG, "one_electron_ints: no basis or null output");
-        return;
+    char msg[160];
+    if (!sh || !ce || ns <= 0 || N <= 0) {
+        snprintf(msg, sizeof msg, "%s: no basis", who);
+        return aux_fail(PATMOL_HIP_EBADARG, msg);
     }
-    RysTables rys = aux_tables();
-    if (!rys.cheb) return;
     for (int s = 0; s < ns; ++s)
         if (sh[s].lsh < 0 || sh[s].lsh > 2) {
-            aux_fail(PATMOL_HIP_EUNSUPPORTED, "one_electron_ints: l > 2");
-            return;
+            snprintf(msg, sizeof msg, "%s: l > 2", who);
+            return aux_fail(PATMOL_HIP_EUNSUPPORTED, msg);
         }
-
-    // Host staging: pairs grouped by class (la >= lb), their primitive pairs,
-    // shell centres and offsets, and the nuclei.
     std::vector<int2> pairs;
     std::vector<PairRef> refs;
     std::vector<PrimPair> pp;
     std::vector<double2> ab;
-    int cls_first[7] = {0};
-    const int cls_l[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
     for (int c = 0; c < 6; ++c) {
-        cls_first[c] = (int)pairs.size();
+        st.cls_first[c] = (int)pairs.size();
         for (int i = 0; i < ns; ++i)
             for (int j = 0; j <= i; ++j) {
                 int I = i, J = j;
                 if (sh[I].lsh < sh[J].lsh) std::swap(I, J);
-                if (sh[I].lsh != cls_l[c][0] || sh[J].lsh != cls_l[c][1]) continue;
+                if (sh[I].lsh != kClassL[c][0] || sh[J].lsh != kClassL[c][1]) continue;
                 const double *Ar = ce[sh[I].cen].r, *Br = ce[sh[J].cen].r;
                 double ab2 = 0.0;
                 for (int x = 0; x < 3; ++x) ab2 += (Ar[x] - Br[x]) * (Ar[x] - Br[x]);
@@ -287,7 +405,7 @@ extern "C" void one_electron_ints(double *Smat, double *Tmat, double *Hmat)
                     }
             }
     }
-    cls_first[6] = (int)pairs.size();
+    st.cls_first[6] = (int)pairs.size();
     std::vector<double> xyz(3 * (size_t)ns);
     std::vector<int> off(ns);
     for (int s = 0; s < ns; ++s) {
@@ -297,52 +415,140 @@ extern "C" void one_electron_ints(double *Smat, double *Tmat, double *Hmat)
     std::vector<double4> nuc;
     for (int c = 0; c < nc; ++c)
         if (c != skip) nuc.push_back(make_double4(ce[c].q, ce[c].r[0], ce[c].r[1], ce[c].r[2]));
+    st.nnuc = (int)nuc.size();
+    st.no2 = (size_t)N * (N + 1) / 2;
+    st.o_pairs = 0;
+    st.o_refs = st.o_pairs + align256(sizeof(int2) * pairs.size());
+    st.o_pp = st.o_refs + align256(sizeof(PairRef) * refs.size());
+    st.o_ab = st.o_pp + align256(sizeof(PrimPair) * pp.size());
+    st.o_xyz = st.o_ab + align256(sizeof(double2) * ab.size());
+    st.o_off = st.o_xyz + align256(sizeof(double) * xyz.size());
+    st.o_nuc = st.o_off + align256(sizeof(int) * off.size());
+    st.o_out = st.o_nuc + align256(sizeof(double4) * (nuc.size() + 1));
+    std::vector<char> stage(st.o_out);
+    memcpy(stage.data() + st.o_pairs, pairs.data(), sizeof(int2) * pairs.size());
+    memcpy(stage.data() + st.o_refs, refs.data(), sizeof(PairRef) * refs.size());
+    memcpy(stage.data() + st.o_pp, pp.data(), sizeof(PrimPair) * pp.size());
+    memcpy(stage.data() + st.o_ab, ab.data(), sizeof(double2) * ab.size());
+    memcpy(stage.data() + st.o_xyz, xyz.data(), sizeof(double) * xyz.size());
+    memcpy(stage.data() + st.o_off, off.data(), sizeof(int) * off.size());
+    if (!nuc.empty()) memcpy(stage.data() + st.o_nuc, nuc.data(), sizeof(double4) * nuc.size());
+    const size_t bytes = st.o_out + nout_per_no2 * st.no2 * sizeof(double);
+    if (hipMalloc(&st.d, bytes) != hipSuccess ||
+        hipMemcpy(st.d, stage.data(), st.o_out, hipMemcpyHostToDevice) != hipSuccess) {
+        snprintf(msg, sizeof msg, "%s: device allocation or upload failed", who);
+        return aux_fail(PATMOL_HIP_EDEVICE, msg);
+    }
+    return 0;
+}
 
-    // One device block: inputs, then S | T | H.
-    const size_t no2 = (size_t)N * (N + 1) / 2;
-    const size_t o_pairs = 0, o_refs = o_pairs + align256(sizeof(int2) * pairs.size());
-    const size_t o_pp = o_refs + align256(sizeof(PairRef) * refs.size());
-    const size_t o_ab = o_pp + align256(sizeof(PrimPair) * pp.size());
-    const size_t o_xyz = o_ab + align256(sizeof(double2) * ab.size());
-    const size_t o_off = o_xyz + align256(sizeof(double) * xyz.size());
-    const size_t o_nuc = o_off + align256(sizeof(int) * off.size());
-    const size_t o_out = o_nuc + align256(sizeof(double4) * (nuc.size() + 1));
-    const size_t bytes = o_out + 3 * no2 * sizeof(double);
-    std::vector<char> stage(o_out);
-    memcpy(stage.data() + o_pairs, pairs.data(), sizeof(int2) * pairs.size());
-    memcpy(stage.data() + o_refs, refs.data(), sizeof(PairRef) * refs.size());
-    memcpy(stage.data() + o_pp, pp.data(), sizeof(PrimPair) * pp.size());
-    memcpy(stage.data() + o_ab, ab.data(), sizeof(double2) * ab.size());
-    memcpy(stage.data() + o_xyz, xyz.data(), sizeof(double) * xyz.size());
-    memcpy(stage.data() + o_off, off.data(), sizeof(int) * off.size());
-    if (!nuc.empty()) memcpy(stage.data() + o_nuc, nuc.data(), sizeof(double4) * nuc.size());
+}  // namespace
 
-    char *d = nullptr;
-    bool ok = hipMalloc(&d, bytes) == hipSuccess;
-    ok = ok && hipMemcpy(d, stage.data(), o_out, hipMemcpyHostToDevice) == hipSuccess;
-    double *dS = ok ? (double *)(d + o_out) : nullptr;
+// Packed moment integrals of the current basis into host arrays (9 * no2).
+int moment_ints_host(double *M, const char *who)
+{
+    RysTables rys{};
+    PairStage st;
+    if (stage_pairs(st, 9, who)) return PATMOL_HIP_EDEVICE;
+    bool ok = true;
     for (int c = 0; ok && c < 6; ++c) {
-        const int n = cls_first[c + 1] - cls_first[c];
-        if (!n) continue;
-        OneArgs args;
-        args.pairs = (const int2 *)(d + o_pairs) + cls_first[c];
-        args.refs = (const PairRef *)(d + o_refs) + cls_first[c];
-        args.pp = (const PrimPair *)(d + o_pp);
-        args.ab = (const double2 *)(d + o_ab);
-        args.xyz = (const double *)(d + o_xyz);
-        args.off = (const int *)(d + o_off);
-        args.nuc = (const double4 *)(d + o_nuc);
-        args.npairs = n;
-        args.nnuc = (int)nuc.size();
-        args.S = dS;
-        args.T = dS + no2;
-        args.H = dS + 2 * no2;
-        args.rys = rys;
-        ok = launch_onee(cls_l[c][0], cls_l[c][1], args, 0) == 0 && hipGetLastError() == hipSuccess;
+        OneArgs a = st.args(c, rys);
+        if (!a.npairs) continue;
+        a.M = st.out();
+        ok = launch_moment(kClassL[c][0], kClassL[c][1], a, 0) == 0 && hipGetLastError() == hipSuccess;
+    }
+    ok = ok && hipMemcpy(M, st.out(), sizeof(double) * 9 * st.no2, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "%s: device failure", who);
+        return aux_fail(PATMOL_HIP_EDEVICE, msg);
+    }
+    return 0;
+}
+
+}  // namespace pm
+
+// oneints.h: void one_electron_ints(double *Smat, double *Tmat, double *Hmat)
+extern "C" void one_electron_ints(double *Smat, double *Tmat, double *Hmat)
+{
+    using namespace pm;
+    if (!Smat || !Tmat || !Hmat) {
+        aux_fail(PATMOL_HIP_EBADARG, "one_electron_ints: null output");
+        return;
+    }
+    RysTables rys = aux_tables();
+    if (!rys.cheb) return;
+    PairStage st;
+    if (stage_pairs(st, 3, "one_electron_ints")) return;
+    const size_t no2 = st.no2;
+    double *dS = st.out();
+    bool ok = true;
+    for (int c = 0; ok && c < 6; ++c) {
+        OneArgs a = st.args(c, rys);
+        if (!a.npairs) continue;
+        a.S = dS;
+        a.T = dS + no2;
+        a.H = dS + 2 * no2;
+        ok = launch_onee(kClassL[c][0], kClassL[c][1], a, 0) == 0 && hipGetLastError() == hipSuccess;
     }
     ok = ok && hipMemcpy(Smat, dS, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess &&
          hipMemcpy(Tmat, dS + no2, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess &&
          hipMemcpy(Hmat, dS + 2 * no2, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
-    if (d) (void)hipFree(d);
     if (!ok) aux_fail(PATMOL_HIP_EDEVICE, "one_electron_ints: device failure");
 }
+
+// Dipole (3 * no2) and quadrupole (6 * no2: xx, xy, xz, yy, yz, zz) integrals
+// of the current basis, packed lower triangles, component-major.
+extern "C" int patmol_hip_moment_ints(double *dip, double *quad)
+{
+    using namespace pm;
+    if (!dip || !quad) return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_moment_ints: null output");
+    const int N = number_of_orbitals();
+    if (N <= 0) return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_moment_ints: no basis");
+    const size_t no2 = (size_t)N * (N + 1) / 2;
+    std::vector<double> M(9 * no2);
+    if (moment_ints_host(M.data(), "patmol_hip_moment_ints")) return PATMOL_HIP_EDEVICE;
+    memcpy(dip, M.data(), sizeof(double) * 3 * no2);
+    memcpy(quad, M.data() + 3 * no2, sizeof(double) * 6 * no2);
+    return 0;
+}
+
+// momints.h: void momints() -- MOMINTS.DAT in the working directory, the
+// reference's records and order (momints.c:113-180): for ish, jsh <= ish, the
+// components ir = off_I + a, jr = off_J + b with jr <= ir, each as
+// int ir, int jr, double dip[3], double quad[6].
+extern "C" int patmol_hip_write_momints(const char *path)
+{
+    using namespace pm;
+    const Shell *sh = shells();
+    const int ns = number_of_shells(), N = number_of_orbitals();
+    if (!path || !sh || ns <= 0 || N <= 0) return aux_fail(PATMOL_HIP_EBADARG, "momints: no basis or path");
+    const size_t no2 = (size_t)N * (N + 1) / 2;
+    std::vector<double> M(9 * no2);
+    if (moment_ints_host(M.data(), "momints")) return PATMOL_HIP_EDEVICE;
+    FILE *f = fopen(path, "wb");
+    if (!f) return aux_fail(PATMOL_HIP_EBADARG, "momints: cannot open the output file");
+    std::vector<char> rec;
+    rec.reserve(no2 * 80);
+    for (int is = 0; is < ns; ++is)
+        for (int js = 0; js <= is; ++js)
+            for (int a = 0; a < sh[is].nst; ++a) {
+                const int ir = sh[is].off + a;
+                for (int b = 0; b < sh[js].nst; ++b) {
+                    const int jr = sh[js].off + b;
+                    if (jr > ir) break;
+                    const size_t ij = (size_t)ir * (ir + 1) / 2 + jr;
+                    const size_t at = rec.size();
+                    rec.resize(at + 2 * sizeof(int) + 9 * sizeof(double));
+                    memcpy(&rec[at], &ir, sizeof(int));
+                    memcpy(&rec[at + sizeof(int)], &jr, sizeof(int));
+                    for (int m = 0; m < 9; ++m)
+                        memcpy(&rec[at + 2 * sizeof(int) + m * sizeof(double)], &M[m * no2 + ij], sizeof(double));
+                }
+            }
+    const bool ok = fwrite(rec.data(), 1, rec.size(), f) == rec.size();
+    if (fclose(f) != 0 || !ok) return aux_fail(PATMOL_HIP_EDEVICE, "momints: write failed");
+    return 0;
+}
+
+extern "C" void momints(void) { (void)patmol_hip_write_momints("MOMINTS.DAT"); }

@@ -57,6 +57,15 @@ typedef struct {
 int patmol_rhf_scf(double *P_out, double *evals_out, PatmolScfResult *res);
 int patmol_uhf_scf(double *PA_out, double *PB_out, PatmolScfResult *res);
 
+/* multipole analysis (scfaux.c:140-340) from the device moment integrals:
+ * electronic, nuclear and total (nuclear - electronic) moments, 9 values
+ * each (x, y, z, xx, xy, xz, yy, yz, zz) */
+int patmol_rhf_moments(const double *P, double *elec, double *nuc, double *total);
+int patmol_uhf_moments(const double *PA, const double *PB, double *elec, double *nuc, double *total);
+/* finite-field polarisability (rhfscf.c:176-215) through the device SCF:
+ * alfa[5] = xx, yy, zz, mean, and the reference's alfa2 */
+int patmol_rhf_finite_field(double efx, double *alfa, PatmolScfResult *gs);
+
 #ifdef __cplusplus
 }
 #endif

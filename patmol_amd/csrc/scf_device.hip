@@ -461,6 +461,7 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         if (first) {
             first = false;
             iteration = 0;
+            res->energy0 = enrg + enuc;
         } else {
             done = ret;
         }
@@ -584,6 +585,7 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         if (first) {
             first = false;
             iteration = 0;
+            res->energy0 = enrg + enuc;
         } else {
             done = ret;
         }

@@ -85,3 +85,70 @@ def test_host_one_electron_closed_forms():
 def test_device_one_electron_closed_forms():
     pm.load_system(_sys())
     _check(*pm.one_electron_ints())
+
+
+# ---------------------------------------------------------------------------
+# multipole integrals (momints.c:14-92): closed forms for the same primitives
+#   <s_a|x_k|s_b>      = P_k <s_a|s_b>
+#   <s_a|x_k^2|s_b>    = (P_k^2 + 1/(2p)) <s_a|s_b>
+#   <s_a|x_k x_l|s_b>  = P_k P_l <s_a|s_b>          (k != l)
+#   <p_z,a|z|s_b>      = N_pz N_b (pi/p)^(3/2) K ((P_z - A_z) P_z + 1/(2p))
+# ---------------------------------------------------------------------------
+def _moment_closed_forms():
+    p, mu = AL + BE, AL * BE / (AL + BE)
+    R2 = float(((A - B) ** 2).sum())
+    K = math.exp(-mu * R2)
+    P = (AL * A + BE * B) / p
+    Na, Nb = (2 * AL / math.pi) ** 0.75, (2 * BE / math.pi) ** 0.75
+    S = Na * Nb * (math.pi / p) ** 1.5 * K
+    dip = [P[k] * S for k in range(3)]
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    quad = [(P[k] * P[l] + (0.5 / p if k == l else 0.0)) * S for k, l in pairs]
+    Npz = 2 * math.sqrt(AL) * Na
+    pz_z_s = Npz * Nb * (math.pi / p) ** 1.5 * K * ((P[2] - A[2]) * P[2] + 0.5 / p)
+    return dip, quad, pz_z_s
+
+
+def _check_moments(dip, quad):
+    d, q, pzs = _moment_closed_forms()
+    ij = _idx(1, 0)
+    for k in range(3):
+        assert abs(dip[k][ij] - d[k]) < 1e-14
+    for k in range(6):
+        assert abs(quad[k][ij] - q[k]) < 1e-14
+    assert abs(dip[2][_idx(4, 1)] - pzs) < 1e-14
+    # s_A at the origin: <x> = 0, <x^2> = 1/(4 al)
+    assert abs(dip[0][_idx(0, 0)]) < 1e-15
+    assert abs(quad[0][_idx(0, 0)] - 0.25 / AL) < 1e-14
+
+
+def test_oracle_moment_closed_forms():
+    _check_moments(*oracle.Basis(_sys()).moment_ints())
+
+
+@pytest.mark.gpu
+def test_device_moment_closed_forms():
+    pm.load_system(_sys())
+    _check_moments(*pm.moment_ints())
+
+
+def test_oracle_dipole_is_energy_derivative():
+    """Hellmann-Feynman: with H(F) = H - F d_z (rhf_getDipoleHam,
+    rhfscf.c:474-489) the converged energy has dE/dF = -(electronic z moment
+    of rhf_analyzeMoments); and the reference's finite-field polarisability
+    -2 (E - E_first)/F^2 agrees with the central second difference to O(F)."""
+    s = systems.water()
+    b = oracle.Basis(s)
+    ints = b.two_electron_ints()
+    _, _, H = b.one_electron_ints()
+    dip, quad = b.moment_ints()
+    gs, P, _, _ = oracle.rhf_scf_ex(b, ints, s.nelec, eps=1e-12)
+    elec, nuc, tot = oracle.rhf_moments(P, dip, quad, s.charges, s.coords)
+    F = 2e-3
+    ep, _, _, _ = oracle.rhf_scf_ex(b, ints, s.nelec, H=H - F * dip[2], guess=P, eps=1e-12)
+    em, _, _, _ = oracle.rhf_scf_ex(b, ints, s.nelec, H=H + F * dip[2], guess=P, eps=1e-12)
+    dEdF = (ep.energy - em.energy) / (2 * F)
+    assert abs(dEdF + elec[2]) < 1e-5 * max(1.0, abs(elec[2]))
+    a_cd = -(ep.energy + em.energy - 2 * gs.energy) / F ** 2
+    alfa, a0, a2 = oracle.rhf_finite_field(b, ints, s.nelec, F, eps=1e-12)
+    assert a_cd > 0 and abs(alfa[2] - a_cd) < 2e-2 * a_cd

@@ -217,3 +217,17 @@ def test_diag_fock_vs_reference():
         assert np.abs(P - Pr).max() < 1e-11 * np.abs(Pr).max()
         sg = np.sign(np.sum(Cm * Cr, axis=0))
         assert np.abs(Cm - Cr * sg).max() < 1e-10 * np.abs(Cr).max()
+
+
+def test_moment_ints_vs_reference(sf6, tmp_path):
+    """or_moment_ints against the MOMINTS.DAT the reference's momints()
+    writes for SF6 (s/p/d, contracted): record order and values."""
+    s, b = sf6
+    rec = ref.momints(str(tmp_path))
+    n = b.norb
+    assert len(rec) == n * (n + 1) // 2
+    dip, quad = b.moment_ints()
+    ij = rec["ir"] * (rec["ir"] + 1) // 2 + rec["jr"]
+    assert np.all(rec["jr"] <= rec["ir"]) and len(np.unique(ij)) == len(ij)
+    assert np.abs(rec["dip"] - dip[:, ij].T).max() < 1e-11 * np.abs(dip).max()
+    assert np.abs(rec["quad"] - quad[:, ij].T).max() < 1e-11 * np.abs(quad).max()

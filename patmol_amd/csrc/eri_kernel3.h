@@ -370,17 +370,30 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                     kbc[Bl * NC + C_] += v * DIL[A_ * SL + D_];
                     kbd[Bl * ND + D_] += v * DIK[A_ * SK + C_];
                 };
-                for (int ib = 0; ib < nbp; ++ib) {
-                    const PrimPair bp = (ib == 0) ? bp0 : a.pp[bref.pp0 + min(ib, nbp_lane - 1)];
+                auto quartet = [&](const PrimPair &bp, const PrimPair &kp, R sc) {
+                    if constexpr (jk3_rootwise<LA, LB, LC, LD>())
+                        primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD>(), A0, A0 + NAH,
+                                                   B0, B0 + NBH>(bp, kp, bg, kg, rt, sc, digest);
+                    else
+                        primitive_quartet<R, LA, LB, LC, LD, true, A0, A0 + NAH, B0, B0 + NBH>(bp, kp, bg, kg, rt, sc,
+                                                                                           digest);
+                };
+                if (nbp == 1 && staged) {
+                    // uncontracted bra pairs and ket primitive pairs in LDS
+                    // (the common case): no per-iteration selects of the bra
+                    // pair, and LDS reads instead of generic-address loads
+                    const PrimPair *__restrict__ kst = sPP + sPPoff[t];
                     for (int ik = 0; ik < nkp; ++ik) {
-                        const PrimPair kp = kpp[min(ik, kref.npp - 1)];
-                        const R sc = (ib < nbp_lane && ik < kref.npp) ? scale : R(0);
-                        if constexpr (jk3_rootwise<LA, LB, LC, LD>())
-                            primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD>(), A0,
-                                                       A0 + NAH, B0, B0 + NBH>(bp, kp, bg, kg, rt, sc, digest);
-                        else
-                            primitive_quartet<R, LA, LB, LC, LD, true, A0, A0 + NAH, B0, B0 + NBH>(bp, kp, bg, kg, rt,
-                                                                                               sc, digest);
+                        const PrimPair kp = kst[min(ik, kref.npp - 1)];
+                        quartet(bp0, kp, ik < kref.npp ? scale : R(0));
+                    }
+                } else {
+                    for (int ib = 0; ib < nbp; ++ib) {
+                        const PrimPair bp = (ib == 0) ? bp0 : a.pp[bref.pp0 + min(ib, nbp_lane - 1)];
+                        for (int ik = 0; ik < nkp; ++ik) {
+                            const PrimPair kp = kpp[min(ik, kref.npp - 1)];
+                            quartet(bp, kp, (ib < nbp_lane && ik < kref.npp) ? scale : R(0));
+                        }
                     }
                 }
             };

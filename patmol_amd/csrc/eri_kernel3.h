@@ -62,6 +62,11 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 76: return 16 + 8 + 1 * 2 + 1;   // (dd|pp)
     case 78: return 32 + 8 + 1 * 2 + 1;   // (dd|ds)
     case 79: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dp)
+#ifdef PM_SHAPE_DDDD
+    case 80: return PM_SHAPE_DDDD;             // (dd|dd)
+#else
+    case 80: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dd)
+#endif
     default: return 1 * 2 + 1;
     }
 }
@@ -320,10 +325,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             const int lq = lexist ? lq0 : 0;
             const int t = kq * TL + lq;
             const int L = fL + lq;
-            bool valid = bvalid && lexist && sKok[t] != 0;
-            if (samePair) valid = valid && (I > K || (I == K && J >= L));
+            // validity from unconditional loads (t is always a staged slot)
+            // and bitwise logic, so that it compiles to selects, not branches
+            const int kok = sKok[t];
             const double qket = sKq[t];
-            valid = valid && (qbra * qket >= a.tau);
+            bool valid = bvalid & lexist & (kok != 0) & (qbra * qket >= a.tau);
+            if (samePair) valid = valid & ((I > K) | ((I == K) & (J >= L)));
             if (a.start_shell > 0) valid = valid && (max(origIJ, max(a.shell_orig[K], a.shell_orig[L])) >= a.start_shell);
             if (!__any(valid)) continue;  // uniform
             double wdeg = 1.0;
@@ -335,7 +342,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             // the stand-in partner of an invalid (K, L) is K itself (equal ket
             // tiles, so fK == fL) or the first shell of the L tile
             PairGeom<R> kg;
-            const int lqs = (sKok[t] != 0) ? lq : (sameKet ? kq : 0);
+            const int lqs = (kok != 0) ? lq : (sameKet ? kq : 0);
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
                 kg.A[x] = R(sXK[kq][x]);

@@ -696,7 +696,12 @@ inline void launch_jk_t(const JKArgs &args, int prec, int nk, hipStream_t s)
     dim3 grid(args.nitems), block(256);
     // (dd|dd): the ket-pair-per-lane kernel with run-time component loops;
     // every other class runs the LDS-staged kernel
-    if constexpr (LA == 2 && LB == 2 && LC == 2 && LD == 2) {
+#ifdef PM_DDDD_K3
+    constexpr bool dddd_v1 = false;
+#else
+    constexpr bool dddd_v1 = LA == 2 && LB == 2 && LC == 2 && LD == 2;
+#endif
+    if constexpr (dddd_v1) {
         if (prec == 0)
             hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, double, 1>), grid, block, 0, s, args);
         else

@@ -21,11 +21,24 @@ namespace pm {
 
 // Layout of rys_table.inc (gen_rys_tables.py): per root count n, TMAX(n) =
 // 36 + 8(n-1) unit intervals, each RYS_NC Chebyshev coefficients x 2n values
-// (n nodes then n weights); above TMAX(n) the Hermite large-T limit.
+// (n nodes then n weights); above TMAX(n) the Hermite large-T limit.  The
+// device tables keep that order but pad every interval by two doubles
+// (rys_stride): 24n + 2 doubles is 8(6n) + 4 dwords, so the intervals that
+// the lanes of a wave select start in different LDS bank groups instead of
+// all on the same bank (24n doubles is a multiple of 32 dwords for even n).
 constexpr int RYS_NC = 12;
 constexpr int RYS_NMAX = 7;
 constexpr int rys_tmax(int n) { return 36 + 8 * (n - 1); }
+constexpr int rys_stride(int n) { return RYS_NC * 2 * n + 2; }
 constexpr int rys_off(int n)
+{
+    int o = 0;
+    for (int m = 1; m < n; ++m) o += rys_tmax(m) * rys_stride(m);
+    return o;
+}
+constexpr int rys_table_doubles() { return rys_off(RYS_NMAX + 1); }
+// offsets of the unpadded layout of rys_table.inc
+constexpr int rys_src_off(int n)
 {
     int o = 0;
     for (int m = 1; m < n; ++m) o += rys_tmax(m) * RYS_NC * 2 * m;
@@ -150,7 +163,7 @@ __device__ __forceinline__ void rys_eval(double T, double (&x)[N], double (&w)[N
     if (T < (double)rys_tmax(N)) {
         int k = (int)T;
         const double u = 2.0 * (T - k) - 1.0;
-        const double *__restrict__ c = tb.cheb + rys_off(N) + k * (RYS_NC * 2 * N);
+        const double *__restrict__ c = tb.cheb + rys_off(N) + k * rys_stride(N);
         double y[2 * N];
 #pragma unroll
         for (int v = 0; v < 2 * N; ++v) y[v] = c[(RYS_NC - 1) * 2 * N + v];
@@ -183,7 +196,7 @@ __device__ __forceinline__ void rys_eval(float T, float (&x)[N], float (&w)[N], 
     if (T < (float)rys_tmax(N)) {
         int k = (int)T;
         const float u = 2.0f * (T - (float)k) - 1.0f;
-        const float *__restrict__ c = tb.cheb32 + rys_off(N) + k * (RYS_NC * 2 * N);
+        const float *__restrict__ c = tb.cheb32 + rys_off(N) + k * rys_stride(N);
         float y[2 * N];
 #pragma unroll
         for (int v = 0; v < 2 * N; ++v) y[v] = c[(NC32 - 1) * 2 * N + v];
@@ -215,7 +228,7 @@ __device__ __forceinline__ void rys_eval_one(double T, int r, double &x, double 
     if (T < (double)rys_tmax(N)) {
         int k = (int)T;
         const double u = 2.0 * (T - k) - 1.0;
-        const double *__restrict__ c = tb.cheb + rys_off(N) + k * (RYS_NC * 2 * N) + r;
+        const double *__restrict__ c = tb.cheb + rys_off(N) + k * rys_stride(N) + r;
         double yx = c[(RYS_NC - 1) * 2 * N], yw = c[(RYS_NC - 1) * 2 * N + N];
 #pragma unroll
         for (int j = RYS_NC - 2; j >= 0; --j) {
@@ -238,7 +251,7 @@ __device__ __forceinline__ void rys_eval_one(float T, int r, float &x, float &w,
     if (T < (float)rys_tmax(N)) {
         int k = (int)T;
         const float u = 2.0f * (T - (float)k) - 1.0f;
-        const float *__restrict__ c = tb.cheb32 + rys_off(N) + k * (RYS_NC * 2 * N) + r;
+        const float *__restrict__ c = tb.cheb32 + rys_off(N) + k * rys_stride(N) + r;
         float yx = c[(NC32 - 1) * 2 * N], yw = c[(NC32 - 1) * 2 * N + N];
 #pragma unroll
         for (int j = NC32 - 2; j >= 0; --j) {

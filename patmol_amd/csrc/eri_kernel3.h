@@ -166,7 +166,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     // the light classes (their lookups are otherwise global-memory latency
     // on the critical path of every step)
     constexpr bool RYS_LDS = std::is_same<R, double>::value && S::NR <= 2;
-    constexpr int RYS_LDS_N = RYS_LDS ? rys_off(S::NR) + rys_tmax(S::NR) * RYS_NC * 2 * S::NR : 2;
+    constexpr int RYS_LDS_N = RYS_LDS ? rys_off(S::NR) + rys_tmax(S::NR) * rys_stride(S::NR) : 2;
     __shared__ __attribute__((aligned(16))) double sRys[RYS_LDS_N];
     // block offsets inside sD / sO
     constexpr int oIJ = 0, oKL = oIJ + FI * SJ, oIK = oKL + FK * SL, oIL = oIK + FI * SK, oJK = oIL + FI * SL,

@@ -99,8 +99,9 @@ static int ensure_tables()
     // form is as accurate as Clenshaw's recurrence on these fits, 1.3e-14
     // relative at worst, with half the operations).  The FP32 table holds
     // the first 9 Chebyshev terms, converted the same way.
-    std::vector<double> mono(RYS_TABLE_SIZE);
-    std::vector<float> f(RYS_TABLE_SIZE, 0.0f);
+    const size_t ntab = (size_t)pm::rys_table_doubles();
+    std::vector<double> mono(ntab, 0.0);
+    std::vector<float> f(ntab, 0.0f);
     {
         constexpr int NC = kTableNC, NC32 = 9;
         long double tp[NC][NC] = {};  // tp[j][i]: coefficient of u^i in T_j
@@ -112,7 +113,8 @@ static int ensure_tables()
         for (int n = 1; n <= RYS_NMAX; ++n) {
             const int nser = 2 * n;
             for (int k = 0; k < pm::rys_tmax(n); ++k) {
-                const int base = pm::rys_off(n) + k * NC * nser;
+                const int base = pm::rys_src_off(n) + k * NC * nser;     // rys_table.inc
+                const int dst = pm::rys_off(n) + k * pm::rys_stride(n);  // padded device layout
                 for (int v = 0; v < nser; ++v) {
                     long double a[NC] = {}, a32[NC] = {};
                     for (int j = 0; j < NC; ++j)
@@ -121,23 +123,24 @@ static int ensure_tables()
                             if (j < NC32) a32[i] += (long double)rys_cheb[base + j * nser + v] * tp[j][i];
                         }
                     for (int i = 0; i < NC; ++i) {
-                        mono[base + i * nser + v] = (double)a[i];
-                        f[base + i * nser + v] = (float)a32[i];
+                        mono[dst + i * nser + v] = (double)a[i];
+                        f[dst + i * nser + v] = (float)a32[i];
                     }
                 }
             }
         }
     }
-    HIPCHK(hipMalloc(&c, sizeof(double) * RYS_TABLE_SIZE));
-    HIPCHK(hipMalloc(&c32, sizeof(float) * RYS_TABLE_SIZE));
+    HIPCHK(hipMalloc(&c, sizeof(double) * ntab));
+    HIPCHK(hipMalloc(&c32, sizeof(float) * ntab));
     HIPCHK(hipMalloc(&hr, sizeof(rys_herm_r)));
     HIPCHK(hipMalloc(&hh, sizeof(rys_herm_h)));
-    HIPCHK(hipMemcpy(c, mono.data(), sizeof(double) * RYS_TABLE_SIZE, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c32, f.data(), sizeof(float) * RYS_TABLE_SIZE, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c, mono.data(), sizeof(double) * ntab, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c32, f.data(), sizeof(float) * ntab, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(hr, rys_herm_r, sizeof(rys_herm_r), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(hh, rys_herm_h, sizeof(rys_herm_h), hipMemcpyHostToDevice));
     g_rys = {c, c32, hr, hh};
     static_assert(kTableNC == pm::RYS_NC, "table layout");
+    static_assert(pm::rys_src_off(pm::RYS_NMAX + 1) == RYS_TABLE_SIZE, "table layout");
     return 0;
 }
 

@@ -157,13 +157,15 @@ __device__ __forceinline__ double rsqrt_pos(double x)
     return fma(y * r, fma(0.375, r, 0.5), y);
 }
 
-template <int N>
+// With SECT the table pointer addresses the N-root section itself (a copy of
+// just that section, e.g. in LDS) instead of the start of all sections.
+template <int N, bool SECT = false>
 __device__ __forceinline__ void rys_eval(double T, double (&x)[N], double (&w)[N], const RysTables &tb)
 {
     if (T < (double)rys_tmax(N)) {
         int k = (int)T;
         const double u = 2.0 * (T - k) - 1.0;
-        const double *__restrict__ c = tb.cheb + rys_off(N) + k * rys_stride(N);
+        const double *__restrict__ c = tb.cheb + (SECT ? 0 : rys_off(N)) + k * rys_stride(N);
         double y[2 * N];
 #pragma unroll
         for (int v = 0; v < 2 * N; ++v) y[v] = c[(RYS_NC - 1) * 2 * N + v];
@@ -187,7 +189,7 @@ __device__ __forceinline__ void rys_eval(double T, double (&x)[N], double (&w)[N
     }
 }
 
-template <int N>
+template <int N, bool SECT = false>
 __device__ __forceinline__ void rys_eval(float T, float (&x)[N], float (&w)[N], const RysTables &tb)
 {
     // FP32 evaluation: the first NC32 Chebyshev terms of the same fits
@@ -222,13 +224,13 @@ __device__ __forceinline__ void rys_eval(float T, float (&x)[N], float (&w)[N], 
 
 // One root r (run-time index) of the N-point rule: the same fits, two
 // Clenshaw series instead of 2N.
-template <int N>
+template <int N, bool SECT = false>
 __device__ __forceinline__ void rys_eval_one(double T, int r, double &x, double &w, const RysTables &tb)
 {
     if (T < (double)rys_tmax(N)) {
         int k = (int)T;
         const double u = 2.0 * (T - k) - 1.0;
-        const double *__restrict__ c = tb.cheb + rys_off(N) + k * rys_stride(N) + r;
+        const double *__restrict__ c = tb.cheb + (SECT ? 0 : rys_off(N)) + k * rys_stride(N) + r;
         double yx = c[(RYS_NC - 1) * 2 * N], yw = c[(RYS_NC - 1) * 2 * N + N];
 #pragma unroll
         for (int j = RYS_NC - 2; j >= 0; --j) {
@@ -244,7 +246,7 @@ __device__ __forceinline__ void rys_eval_one(double T, int r, double &x, double 
     }
 }
 
-template <int N>
+template <int N, bool SECT = false>
 __device__ __forceinline__ void rys_eval_one(float T, int r, float &x, float &w, const RysTables &tb)
 {
     constexpr int NC32 = 9;
@@ -330,7 +332,7 @@ struct PairGeom {
 // Returns false when the primitive quartet falls under the prefactor screen
 // (its integrals are then passed as exact zeros).
 template <typename R, int LA, int LB, int LC, int LD, bool SCREEN = true, int CA0 = 0, int CA1 = ncart(LA),
-          int CB0 = 0, int CB1 = ncart(LB), typename F>
+          int CB0 = 0, int CB1 = ncart(LB), bool SECT = false, typename F>
 __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const PrimPair &kp, const PairGeom<R> &bra,
                                                   const PairGeom<R> &ket, const RysTables &tab, R scale,
                                                   F &&f)
@@ -349,7 +351,7 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
     const R rp = R(p), rq = R(q), ipq = R(ipq_d), irp = R(bp.ip), irq = R(kp.ip);
     const R T = (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]) * rp * rq * ipq;
     R x[NR], w[NR];
-    rys_eval<NR>(T, x, w, tab);
+    rys_eval<NR, SECT>(T, x, w, tab);
     const R PA[3] = {R(bp.P[0]) - bra.A[0], R(bp.P[1]) - bra.A[1], R(bp.P[2]) - bra.A[2]};
     const R QC[3] = {R(kp.P[0]) - ket.A[0], R(kp.P[1]) - ket.A[1], R(kp.P[2]) - ket.A[2]};
     R Gx[NR][L12 + 1][L34 + 1], Gy[NR][L12 + 1][L34 + 1], Gz[NR][L12 + 1][L34 + 1];
@@ -408,7 +410,7 @@ __device__ __forceinline__ bool primitive_quartet(const PrimPair &bp, const Prim
 // roots, and every use of it here is linear), so only one root's 2-D tables
 // are live instead of all of them.
 template <typename R, int LA, int LB, int LC, int LD, bool BLOCK_FENCE, int CA0 = 0, int CA1 = ncart(LA),
-          int CB0 = 0, int CB1 = ncart(LB), typename F>
+          int CB0 = 0, int CB1 = ncart(LB), bool SECT = false, typename F>
 __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, const PrimPair &kp,
                                                            const PairGeom<R> &bra, const PairGeom<R> &ket,
                                                            const RysTables &tab, R scale, F &&f)
@@ -440,7 +442,7 @@ __device__ __forceinline__ void primitive_quartet_rootwise(const PrimPair &bp, c
     for (int r = 0; r < NR; ++r) {
         __asm__ volatile("" ::: "memory");
         R xr, wr;
-        rys_eval_one<NR>(T, r, xr, wr, tab);
+        rys_eval_one<NR, SECT>(T, r, xr, wr, tab);
         R Gx[L12 + 1][L34 + 1], Gy[L12 + 1][L34 + 1], Gz[L12 + 1][L34 + 1];
         const R fff = xr * ipq;
         const R B00 = R(0.5) * fff;

@@ -162,11 +162,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     __shared__ int sPPoff[KLP];
     __shared__ PrimPair sPP[S::PPCAP];
     __shared__ int sStaged, sKmax;
-    // Rys coefficient tables of the one- and two-root rules kept in LDS for
-    // the light classes (their lookups are otherwise global-memory latency
-    // on the critical path of every step)
-    constexpr bool RYS_LDS = std::is_same<R, double>::value && S::NR <= 2;
-    constexpr int RYS_LDS_N = RYS_LDS ? rys_off(S::NR) + rys_tmax(S::NR) * rys_stride(S::NR) : 2;
+    // The Rys coefficient section of this class's root count kept in LDS
+    // for the classes with up to three roots (their lookups are otherwise
+    // global-memory latency on the critical path of every step)
+#ifndef PM_RYS_LDS_MAXN
+#define PM_RYS_LDS_MAXN 3
+#endif
+    constexpr bool RYS_LDS = std::is_same<R, double>::value && S::NR <= PM_RYS_LDS_MAXN;
+    // only the NR-root section of the table (rys_eval<NR, true>)
+    constexpr int RYS_LDS_N = RYS_LDS ? rys_tmax(S::NR) * rys_stride(S::NR) : 2;
     __shared__ __attribute__((aligned(16))) double sRys[RYS_LDS_N];
     // block offsets inside sD / sO
     constexpr int oIJ = 0, oKL = oIJ + FI * SJ, oIK = oKL + FK * SL, oIL = oIK + FI * SK, oJK = oIL + FI * SL,
@@ -238,7 +242,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         }
     }
     if constexpr (RYS_LDS) {
-        const double2 *__restrict__ src = reinterpret_cast<const double2 *>(a.rys.cheb);
+        const double2 *__restrict__ src = reinterpret_cast<const double2 *>(a.rys.cheb + rys_off(S::NR));
         double2 *dst = reinterpret_cast<double2 *>(sRys);
         for (int e = tid; e < RYS_LDS_N / 2; e += 256) dst[e] = src[e];
     }
@@ -380,10 +384,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 auto quartet = [&](const PrimPair &bp, const PrimPair &kp, R sc) {
                     if constexpr (jk3_rootwise<LA, LB, LC, LD>())
                         primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD>(), A0, A0 + NAH,
-                                                   B0, B0 + NBH>(bp, kp, bg, kg, rt, sc, digest);
+                                                   B0, B0 + NBH, RYS_LDS>(bp, kp, bg, kg, rt, sc, digest);
                     else
-                        primitive_quartet<R, LA, LB, LC, LD, true, A0, A0 + NAH, B0, B0 + NBH>(bp, kp, bg, kg, rt, sc,
-                                                                                           digest);
+                        primitive_quartet<R, LA, LB, LC, LD, true, A0, A0 + NAH, B0, B0 + NBH, RYS_LDS>(bp, kp, bg, kg,
+                                                                                                    rt, sc, digest);
                 };
                 if (nbp == 1 && staged) {
                     // uncontracted bra pairs and ket primitive pairs in LDS

@@ -123,6 +123,7 @@ typedef struct {
     double other_ms;    /* the rest of the loop (eigensolver, density, sums, host logic) */
     double total_ms;
     double energy0;     /* energy after the first update (rhfscf.c:86, 191) */
+    int purify_fallbacks; /* iterations whose purification did not converge */
 } PatmolHipScfResult;
 int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Smat, const double *Hmat,
                        int nocc, int maxits, double scf_eps, int scf_accel, double enuc,
@@ -132,6 +133,13 @@ int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Sma
                        int nocca, int noccb, int maxits, double scf_eps, int scf_accel, double enuc,
                        const double *PAguess, const double *PBguess, double *PAout, double *PBout,
                        double *evalsA, double *evalsB, PatmolHipScfResult *res);
+/* Density inside the device SCF loop: mode 1 (default) by trace-correcting
+ * purification of X^T F X for bases of 128 functions or more (GEMMs only;
+ * an iteration whose purification does not converge, e.g. for a degenerate
+ * Fermi level, is diagonalised instead), mode 0 by the eigensolver every
+ * iteration as the reference does.  The guess and the returned orbitals
+ * always come from the eigensolver. */
+void patmol_hip_set_scf_density(int mode);
 /* One diagonalisation (spack_trans + spack_diag + form_cmatrix +
  * form_pmatrix) of host packed F in the basis orthogonalised by S: ascending
  * eigenvalues, C (row-major, phase-fixed) and P for nocc occupied orbitals. */

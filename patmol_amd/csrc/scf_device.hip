@@ -43,6 +43,8 @@ int aux_fail(int code, const char *msg);
 
 namespace {
 
+int g_scf_density = 1;   // 1: purification inside the loop, 0: eigensolver
+
 constexpr int RED_THREADS = 1024;
 
 // full symmetric F = a + b from packed lower triangles (b may be null)
@@ -194,6 +196,107 @@ __global__ void average_kernel(double *__restrict__ p, const double *__restrict_
     if (i < no2) p[i] = 0.5 * (p1[i] + p2[i]);
 }
 
+// ---- density by trace-correcting purification (Niklasson, Phys. Rev. B 66,
+// 155115 (2002)) ----
+// Gershgorin interval of symmetric F (column-major, n x n): per row the
+// centre F_ii and radius sum_{j != i} |F_ij|; one block per row.
+__global__ __launch_bounds__(256) void gersh_row_kernel(const double *__restrict__ F, int n, double *__restrict__ lo,
+                                                        double *__restrict__ hi)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x;
+    double r = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256)
+        if (j != i) r += fabs(F[(size_t)j * n + i]);
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double c = F[(size_t)i * n + i];
+        lo[i] = c - red[0];
+        hi[i] = c + red[0];
+    }
+}
+
+// bounds[0] = min lo, bounds[1] = max hi (one block)
+__global__ __launch_bounds__(256) void gersh_reduce_kernel(const double *__restrict__ lo, const double *__restrict__ hi,
+                                                           int n, double *__restrict__ bounds)
+{
+    __shared__ double a[256], b[256];
+    double mn = 1e300, mx = -1e300;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        mn = fmin(mn, lo[i]);
+        mx = fmax(mx, hi[i]);
+    }
+    a[threadIdx.x] = mn;
+    b[threadIdx.x] = mx;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            a[threadIdx.x] = fmin(a[threadIdx.x], a[threadIdx.x + st]);
+            b[threadIdx.x] = fmax(b[threadIdx.x], b[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bounds[0] = a[0];
+        bounds[1] = b[0];
+    }
+}
+
+// X0 = (emax I - F) / (emax - emin): spectrum mapped into [0, 1], occupied
+// levels towards 1
+__global__ void tc2_init_kernel(const double *__restrict__ F, double *__restrict__ X, int n,
+                                const double *__restrict__ bounds)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)n * n) return;
+    const double emin = bounds[0], emax = bounds[1];
+    const double sc = 1.0 / (emax - emin);
+    const int i = (int)(k % n), j = (int)(k / n);
+    X[k] = ((i == j ? emax : 0.0) - F[k]) * sc;
+}
+
+// traces of X and X^2 (one block); st[0] = tr X, st[1] = tr X^2, and the
+// choice of the next step: st[2] = 1 for X^2, 0 for 2X - X^2, whichever
+// brings the trace closer to the occupation
+__global__ __launch_bounds__(256) void tc2_trace_kernel(const double *__restrict__ X, const double *__restrict__ X2,
+                                                        int n, double nocc, double *__restrict__ st)
+{
+    __shared__ double a[256], b[256];
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        t1 += X[(size_t)i * n + i];
+        t2 += X2[(size_t)i * n + i];
+    }
+    a[threadIdx.x] = t1;
+    b[threadIdx.x] = t2;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) {
+            a[threadIdx.x] += a[threadIdx.x + s2];
+            b[threadIdx.x] += b[threadIdx.x + s2];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        st[0] = a[0];
+        st[1] = b[0];
+        st[2] = fabs(b[0] - nocc) < fabs(2.0 * a[0] - b[0] - nocc) ? 1.0 : 0.0;
+    }
+}
+
+__global__ void tc2_update_kernel(double *__restrict__ X, const double *__restrict__ X2, int64_t nn,
+                                  const double *__restrict__ st)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    X[k] = st[2] != 0.0 ? X2[k] : 2.0 * X[k] - X2[k];
+}
+
 struct DevMem {
     std::vector<void *> ptrs;
     ~DevMem()
@@ -219,6 +322,9 @@ struct Scf {
     rocblas_handle h = nullptr;
     DevMem mem;
     double *X = nullptr, *F = nullptr, *T = nullptr, *C = nullptr, *ev = nullptr, *off = nullptr, *sums = nullptr;
+    double *X2 = nullptr, *glo = nullptr, *ghi = nullptr, *tc = nullptr;
+    bool purify = false;   // density by purification inside the iteration loop
+    int purify_fallbacks = 0;
     rocblas_int *info = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const char *err = nullptr;
@@ -248,7 +354,12 @@ struct Scf {
         off = mem.get<double>(n);
         sums = mem.get<double>(8);
         info = mem.get<rocblas_int>(1);
-        if (!X || !F || !T || !C || !ev || !off || !sums || !info) return fail("device allocation failed");
+        X2 = mem.get<double>(nn);
+        glo = mem.get<double>(n);
+        ghi = mem.get<double>(n);
+        tc = mem.get<double>(8);
+        if (!X || !F || !T || !C || !ev || !off || !sums || !info || !X2 || !glo || !ghi || !tc)
+            return fail("device allocation failed");
         return true;
     }
 
@@ -292,17 +403,102 @@ struct Scf {
 
     // C = X V with V the eigenvectors of X^T (a + b) X; P = C_occ C_occ^T
     // packed (rhfscf.c:37-46).  Leaves the eigenvalues in ev.
-    bool diag_density(const double *a, const double *b, int nocc, double *P)
+    // F' = X^T (a + b) X into F
+    bool fprime(const double *a, const double *b)
     {
         const int64_t nn = (int64_t)n * n;
         const double one = 1.0, zero = 0.0;
         hipLaunchKernelGGL(sym_from_packed_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, a, b, F, n);
-        // T = F X, F' = X^T T
         if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, F, n, X, n, &zero, T, n) !=
                 rocblas_status_success ||
             rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, X, n, T, n, &zero, F,
                           n) != rocblas_status_success)
             return fail("rocblas_dgemm failed");
+        return true;
+    }
+
+    // P = X P' X^T packed, with P' the projector onto the nocc lowest
+    // eigenvectors of F' by trace-correcting purification: from X0 =
+    // (emax - F')/(emax - emin) (Gershgorin bounds), X <- X^2 or 2X - X^2,
+    // whichever brings tr X^2 / tr(2X - X^2) nearer nocc, until
+    // tr X - tr X^2 (the idempotency error, sum of l(1-l) over the
+    // eigenvalues) is below 1e-12, then two more steps.  Returns false with
+    // `converged` unset when 120 steps do not get there (a degenerate or
+    // vanishing gap at the Fermi level); the caller then diagonalises.
+    bool purify_density(int nocc, double *P, bool &converged)
+    {
+        const int64_t nn = (int64_t)n * n;
+        const double one = 1.0, zero = 0.0;
+        converged = false;
+        if (nocc <= 0 || nocc >= n) return true;   // the caller diagonalises these
+        hipLaunchKernelGGL(gersh_row_kernel, dim3(n), dim3(256), 0, s, F, n, glo, ghi);
+        hipLaunchKernelGGL(gersh_reduce_kernel, dim3(1), dim3(256), 0, s, glo, ghi, n, tc + 4);
+        hipLaunchKernelGGL(tc2_init_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, F, X2, n, tc + 4);
+        // X2 holds X0; iterate in (T = X, X2 = X^2)
+        if (hipMemcpyAsync(T, X2, sizeof(double) * nn, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return fail("copy failed");
+        int extra = -1;
+        for (int it = 0; it < 120; ++it) {
+            if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, T, n, T, n, &zero, X2,
+                              n) != rocblas_status_success)
+                return fail("rocblas_dgemm failed");
+            hipLaunchKernelGGL(tc2_trace_kernel, dim3(1), dim3(256), 0, s, T, X2, n, (double)nocc, tc);
+            hipLaunchKernelGGL(tc2_update_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, T, X2, nn, tc);
+            if (extra > 0) {
+                if (--extra == 0) {
+                    converged = true;
+                    break;
+                }
+                continue;
+            }
+            if (it % 4 == 3) {
+                double t[2];
+                if (hipMemcpyAsync(t, tc, sizeof(t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess)
+                    return fail("device failure in the purification");
+                if (t[0] - t[1] < 1e-12 && fabs(t[0] - nocc) < 1e-8) extra = 2;
+            }
+        }
+        if (!converged) return true;
+        // P = X P' X^T (lower triangle into C as scratch), then pack
+        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, T, n, &zero, X2, n) !=
+                rocblas_status_success ||
+            rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, X2, n, X, n, &zero, T,
+                          n) != rocblas_status_success)
+            return fail("rocblas_dgemm failed");
+        hipLaunchKernelGGL(pack_lower_kernel, dim3(n), dim3(256), 0, s, T, P, n);
+        return hipGetLastError() == hipSuccess || fail("launch failure");
+    }
+
+    // the density of (a + b): purification inside the loop when enabled,
+    // otherwise (and as its fallback) the eigensolver path
+    bool density(const double *a, const double *b, int nocc, double *P)
+    {
+        if (!purify || nocc <= 0 || nocc >= n) return diag_density(a, b, nocc, P);
+        if (!fprime(a, b)) return false;
+        bool conv = false;
+        if (!purify_density(nocc, P, conv)) return false;
+        if (conv) return true;
+        ++purify_fallbacks;
+        return diag_density(a, b, nocc, P);
+    }
+
+    // orbitals of (a + b): eigenvalues into ev, C = X V
+    bool orbitals(const double *a, const double *b)
+    {
+        const double one = 1.0, zero = 0.0;
+        if (!fprime(a, b) || !eigen()) return false;
+        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, F, n, &zero, C, n) !=
+            rocblas_status_success)
+            return fail("rocblas_dgemm failed");
+        return true;
+    }
+
+    bool diag_density(const double *a, const double *b, int nocc, double *P)
+    {
+        const int64_t nn = (int64_t)n * n;
+        const double one = 1.0, zero = 0.0;
+        if (!fprime(a, b)) return false;
         if (!eigen()) return false;
         // C = X V, D = C_occ C_occ^T (lower), P = pack(D)
         if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, F, n, &zero, C, n) !=
@@ -422,6 +618,9 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     }
     double eps = scf_eps;
     if (eps < EPS10 * n * n) eps = EPS10 * n * n;
+    // below ~128 functions the eigensolver is faster than the ~40 GEMMs of
+    // the purification (their launches and trace checks dominate)
+    c.purify = g_scf_density == 1 && n >= 128;
     const bool log = scf_log();
     double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0, dms = 0.0;
     int iteration = 0, done = 0;
@@ -440,7 +639,7 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         P2 = P1;
         P1 = P;
         P = fresh;
-        if (!c.diag_density(dH, dG, nocc, P)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        if (!c.density(dH, dG, nocc, P)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
         double sm[3];
         if (!c.sums3(P, P1, dH, dG, sm)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
         const float g_ms = c.elapsed_ms();
@@ -482,6 +681,10 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     res->gbuild_ms = gms;
     res->other_ms = dms;
     res->total_ms = tot;
+    res->purify_fallbacks = c.purify_fallbacks;
+    // orbitals of the last Fock matrix (the loop's densities came from
+    // purification)
+    if (c.purify && (evals || Cmat) && !c.orbitals(dH, dG)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
     bool ok = true;
     if (Pout) ok = ok && hipMemcpy(Pout, P, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
     if (evals) ok = ok && hipMemcpy(evals, c.ev, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess;
@@ -539,6 +742,9 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     }
     double eps = scf_eps;
     if (eps < EPS10 * n * n) eps = EPS10 * n * n;
+    // below ~128 functions the eigensolver is faster than the ~40 GEMMs of
+    // the purification (their launches and trace checks dominate)
+    c.purify = g_scf_density == 1 && n >= 128;
     const bool log = scf_log();
     double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0;
     int iteration = 0, done = 0;
@@ -561,9 +767,9 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         PB1 = PB;
         PB = fb;
         double sa[3], sb[3];
-        if (!c.diag_density(dH, GA, nocca, PA) ||
+        if (!c.density(dH, GA, nocca, PA) ||
             hipMemcpyAsync(evA, c.ev, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s) != hipSuccess ||
-            !c.sums3(PA, PA1, dH, GA, sa) || !c.diag_density(dH, GB, noccb, PB) || !c.sums3(PB, PB1, dH, GB, sb))
+            !c.sums3(PA, PA1, dH, GA, sa) || !c.density(dH, GB, noccb, PB) || !c.sums3(PB, PB1, dH, GB, sb))
             return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
         gms += c.elapsed_ms();
         pdiff = sqrt(2.0 * sa[2]) / n + sqrt(2.0 * sb[2]) / n;
@@ -605,6 +811,13 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     res->gbuild_ms = gms;
     res->other_ms = tot - gms;
     res->total_ms = tot;
+    res->purify_fallbacks = c.purify_fallbacks;
+    if (c.purify && (evalsA || evalsB)) {
+        if (!c.orbitals(dH, GA) ||
+            hipMemcpyAsync(evA, c.ev, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s) != hipSuccess ||
+            !c.orbitals(dH, GB))
+            return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
+    }
     bool ok = true;
     if (PAout) ok = ok && hipMemcpy(PAout, PA, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
     if (PBout) ok = ok && hipMemcpy(PBout, PB, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
@@ -638,3 +851,5 @@ extern "C" int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int 
     if (!ok) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock: download failed");
     return 0;
 }
+
+extern "C" void patmol_hip_set_scf_density(int mode) { pm::g_scf_density = mode == 0 ? 0 : 1; }

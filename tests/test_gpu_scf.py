@@ -3,6 +3,9 @@ patmol_hip_diag_fock) against the oracle's restatement of the reference's
 iteration (rhfscf.c:21-95, uhfscf.c:22-100, spack_math.c:274-319,
 scfaux.c:5-136) and against the SF6 golden run.
 
+The SCF loop's densities come from trace-correcting purification by
+default; one test also runs the eigensolver path.
+
 Tolerances: eigenvalues, densities and MO coefficients relative to their
 largest element for one diagonalisation, scaled with the overlap condition
 number (below); MO coefficients up to the sign of each
@@ -51,15 +54,21 @@ def test_diag_fock_vs_oracle(name, factory, tol):
     assert de < tol[0] and dp < tol[1] and dc < tol[2]
 
 
-def test_rhf_scf_device_water_vs_oracle():
+@pytest.mark.parametrize("density_mode", [1, 0])
+def test_rhf_scf_device_water_vs_oracle(density_mode):
+    """Both settings of the density path (water has 24 functions, so both
+    use the eigensolver: purification starts at 128 functions; it is
+    exercised by the SF6 (214 functions) and benzene-ring tests)."""
     s = systems.water()
     pm.load_system(s)
-    pm.set_options(tau=0.0, precision=0)
+    pm.set_options(tau=0.0, precision=0, scf_density=density_mode)
     res, P, ev = pm.rhf_scf_device()
+    pm.set_options(scf_density=1)
+    assert res["purify_fallbacks"] == 0
     b = oracle.Basis(s)
     ints = b.two_electron_ints()
     ref, Pr, evr = oracle.rhf_scf(b, ints, s.nelec, maxits=s.maxits, eps=s.scf_eps, scf_a=s.scf_a)
-    note(f"device RHF water: E={res['energy']:.12f} it={res['iterations']} "
+    note(f"device RHF water (density mode {density_mode}): E={res['energy']:.12f} it={res['iterations']} "
          f"(oracle {ref.energy:.12f}, {ref.iterations}); G {res['gbuild_ms']:.1f} ms, rest {res['other_ms']:.1f} ms")
     assert res["converged"] == 1
     assert abs(res["energy"] - ref.energy) < 1e-8
@@ -124,7 +133,8 @@ def test_sf6_golden_energy_device(golden_dir):
     pm.set_options(tau=1e-14, precision=0)
     res, P, ev = pm.rhf_scf_device()
     note(f"SF6 device SCF: E={res['energy']:.12f} it={res['iterations']} (golden {gold['energy']:.12f}, "
-         f"{gold['iterations']}); {res['total_ms']:.0f} ms")
+         f"{gold['iterations']}); {res['total_ms']:.0f} ms, purification fallbacks {res['purify_fallbacks']}")
+    assert res["purify_fallbacks"] == 0
     assert abs(res["enuc"] - gold["nuclear_repulsion"]) < 1e-9
     # the reference's own Rys fits carry ~5e-9 relative error: with exact
     # roots the energy sits 5.1e-6 Eh below the golden
@@ -132,3 +142,22 @@ def test_sf6_golden_energy_device(golden_dir):
     assert abs(res["iterations"] - gold["iterations"]) <= 3
     occ = np.array(gold["orbital_energies"][: s.nelec // 2])
     assert np.abs(ev[: s.nelec // 2] - occ).max() < 2e-5
+
+
+def test_purification_matches_eigensolver_benzene():
+    """The 300-function benzene ring: the SCF with purified densities and
+    the SCF with the eigensolver in every iteration agree."""
+    s = systems.benzene_ring()
+    pm.load_system(s)
+    pm.set_options(tau=1e-12, precision=0, scf_density=1)
+    r1, P1, e1 = pm.rhf_scf_device()
+    pm.set_options(scf_density=0)
+    r0, P0, e0 = pm.rhf_scf_device()
+    pm.set_options(scf_density=1)
+    note(f"benzene ring: purified E={r1['energy']:.12f} ({r1['iterations']} it, "
+         f"{r1['total_ms'] / (r1['iterations'] + 1):.2f} ms/it), eigensolver E={r0['energy']:.12f} "
+         f"({r0['iterations']} it, {r0['total_ms'] / (r0['iterations'] + 1):.2f} ms/it)")
+    assert r1["converged"] == 1 and r0["converged"] == 1 and r1["purify_fallbacks"] == 0
+    assert abs(r1["energy"] - r0["energy"]) < 1e-8
+    assert np.abs(P1 - P0).max() < 1e-6
+    assert np.abs(e1 - e0).max() < 1e-6

@@ -41,6 +41,14 @@ struct K3 {
 constexpr int jk3_shape(int la, int lb, int lc, int ld)
 {
     const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
+#if defined(PM_SHAPE_CLS) && defined(PM_SHAPE_VAL)
+    if (c == PM_SHAPE_CLS) return PM_SHAPE_VAL;   // timing experiments
+#endif
+#ifdef PM_SHAPE_TABLE
+    constexpr int ovr[][2] = {PM_SHAPE_TABLE};   // timing experiments: {class, code}, ...
+    for (const auto &o : ovr)
+        if (o[0] == c) return o[1];
+#endif
     switch (c) {
     case 0: return 1 * 2 + 1;    // (ss|ss)
     case 27: return 2 * 2 + 1;   // (ps|ss)
@@ -315,7 +323,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     for (int i = 0; i < NAH * NBH; ++i) jab[i] = 0.0;
     const double *__restrict__ DIJ = sD + oIJ + (ii * NA) * SJ + jj * NB;
 
+#ifdef PM_K3_STAGEONLY
+    for (int kq = TK; kq < TK; kq += KSTRIDE) {   // timing experiment: staging and flush only
+#else
     for (int kq = wave / SPLIT; kq < TK; kq += KSTRIDE) {
+#endif
         if (kq >= nK) break;  // uniform
         const int K = fK + kq;
         double kac[NAH * NC], kbc[NBH * NC];

@@ -323,11 +323,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     for (int i = 0; i < NAH * NBH; ++i) jab[i] = 0.0;
     const double *__restrict__ DIJ = sD + oIJ + (ii * NA) * SJ + jj * NB;
 
-#ifdef PM_K3_STAGEONLY
-    for (int kq = TK; kq < TK; kq += KSTRIDE) {   // timing experiment: staging and flush only
-#else
     for (int kq = wave / SPLIT; kq < TK; kq += KSTRIDE) {
-#endif
         if (kq >= nK) break;  // uniform
         const int K = fK + kq;
         double kac[NAH * NC], kbc[NBH * NC];

@@ -54,11 +54,11 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 27: return 2 * 2 + 1;   // (ps|ss)
     case 30: return 2 * 2 + 0;   // (ps|ps)
     case 36: return 2 * 2 + 0;   // (pp|ss)
-    case 39: return 2 * 2 + 0;   // (pp|ps)
+    case 39: return 2 * 2 + 1;   // (pp|ps)
     case 40: return 1 * 2 + 1;   // (pp|pp)
     case 54: return 2 * 2 + 0;   // (ds|ss)
     case 57: return 2 * 2 + 1;   // (ds|ps)
-    case 58: return 1 * 2 + 0;   // (ds|pp)
+    case 58: return 16 + 2 * 2 + 1;   // (ds|pp)
     case 60: return 1 * 2 + 0;   // (ds|ds)
     case 63: return 2 * 2 + 1;   // (dp|ss)
     case 66: return 1 * 2 + 0;   // (dp|ps)
@@ -67,8 +67,8 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 70: return 16 + 8 + 1 * 2 + 1;   // (dp|dp)
     case 72: return 1 * 2 + 0;   // (dd|ss)
     case 75: return 1 * 2 + 1;   // (dd|ps)
-    case 76: return 16 + 8 + 1 * 2 + 1;   // (dd|pp)
-    case 78: return 32 + 8 + 1 * 2 + 1;   // (dd|ds)
+    case 76: return 32 + 16 + 1 * 2 + 1;   // (dd|pp)
+    case 78: return 32 + 1 * 2 + 1;   // (dd|ds)
     case 79: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dp)
 #ifdef PM_SHAPE_DDDD
     case 80: return PM_SHAPE_DDDD;             // (dd|dd)

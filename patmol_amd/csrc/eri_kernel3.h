@@ -36,8 +36,9 @@ struct K3 {
 // Per-class launch shape, from timing every class on (H2O)8 with both
 // digestion orders, with and without the density-read fence, at 1 and 2
 // waves per SIMD, with and without splitting the bra components over the
-// waves: code = (split_b - 1) * 32 + (split_a - 1) * 16 + fence * 8 +
-// waves * 2 + rootwise.
+// waves: code = rys_lds * 128 + (split_b - 1) * 32 + (split_a - 1) * 16 +
+// fence * 8 + waves * 2 + rootwise (rys_lds: the Rys coefficients of the
+// class's root count are staged in LDS also above three roots).
 constexpr int jk3_shape(int la, int lb, int lc, int ld)
 {
     const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
@@ -67,8 +68,8 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 70: return 16 + 8 + 1 * 2 + 1;   // (dp|dp)
     case 72: return 1 * 2 + 0;   // (dd|ss)
     case 75: return 1 * 2 + 1;   // (dd|ps)
-    case 76: return 32 + 16 + 1 * 2 + 1;   // (dd|pp)
-    case 78: return 32 + 1 * 2 + 1;   // (dd|ds)
+    case 76: return 128 + 32 + 16 + 1 * 2 + 1;   // (dd|pp)
+    case 78: return 128 + 32 + 1 * 2 + 1;   // (dd|ds)
     case 79: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dp)
 #ifdef PM_SHAPE_DDDD
     case 80: return PM_SHAPE_DDDD;             // (dd|dd)
@@ -87,6 +88,18 @@ constexpr bool jk3_rootwise()
 #else
     return jk3_shape(LA, LB, LC, LD) & 1;
 #endif
+}
+
+// The Rys coefficient section of the class's root count is kept in LDS for
+// the classes with up to three roots (7 / 17 / 30 KB) and for the classes
+// whose shape asks for it (4 roots: 47 KB)
+#ifndef PM_RYS_LDS_MAXN
+#define PM_RYS_LDS_MAXN 3
+#endif
+template <int LA, int LB, int LC, int LD>
+constexpr bool jk3_rys_lds()
+{
+    return (LA + LB + LC + LD) / 2 + 1 <= PM_RYS_LDS_MAXN || ((jk3_shape(LA, LB, LC, LD) >> 7) & 1);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -171,12 +184,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     __shared__ PrimPair sPP[S::PPCAP];
     __shared__ int sStaged, sKmax;
     // The Rys coefficient section of this class's root count kept in LDS
-    // for the classes with up to three roots (their lookups are otherwise
-    // global-memory latency on the critical path of every step)
-#ifndef PM_RYS_LDS_MAXN
-#define PM_RYS_LDS_MAXN 3
-#endif
-    constexpr bool RYS_LDS = std::is_same<R, double>::value && S::NR <= PM_RYS_LDS_MAXN;
+    // (the lookups are otherwise global-memory latency on the critical path
+    // of every step)
+    constexpr bool RYS_LDS = std::is_same<R, double>::value && jk3_rys_lds<LA, LB, LC, LD>();
     // only the NR-root section of the table (rys_eval<NR, true>)
     constexpr int RYS_LDS_N = RYS_LDS ? rys_tmax(S::NR) * rys_stride(S::NR) : 2;
     __shared__ __attribute__((aligned(16))) double sRys[RYS_LDS_N];

@@ -20,7 +20,7 @@ struct JKArgs {
     const int *tile_first;     // first sorted shell of each tile
     const int *tile_size;
     const double *shell_xyz;   // sorted shells, 3 per shell (bohr)
-    const int *shell_off;      // first function of each sorted shell
+    const int *shell_off;      // first function of each sorted shell, in sorted function order
     const int *shell_orig;     // original shell index of each sorted shell
     const PairRef *pairs;      // sorted pairs I >= J at I(I+1)/2 + J
     const double *pair_q;      // Schwarz bound of each pair

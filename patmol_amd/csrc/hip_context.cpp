@@ -187,6 +187,9 @@ struct Ctx {
     double tau = 1e-12;
     // host tables
     std::vector<int> sorted_orig, orig_to_sorted, shell_l, shell_nst, shell_off;
+    // functions renumbered in sorted shell order (the J/K path's layout of D,
+    // J' and K': the shells of a tile own one contiguous range of functions)
+    std::vector<int> shell_soff, fn_sorted_to_orig, fn_orig_to_sorted;
     std::vector<double> shell_xyz;
     std::vector<int> tile_first, tile_size, tile_l;
     std::vector<PairRef> pairs;
@@ -197,7 +200,7 @@ struct Ctx {
     std::vector<ClassRange> ranges;
     // device tables
     DevBuf<double> d_xyz, d_q, d_bound;
-    DevBuf<int> d_off, d_orig, d_tile_first, d_tile_size;
+    DevBuf<int> d_off, d_soff, d_fperm, d_finv, d_orig, d_tile_first, d_tile_size;
     DevBuf<PairRef> d_pairs;
     DevBuf<PrimPair> d_pp;
     DevBuf<int4> d_items;
@@ -268,13 +271,31 @@ static int build_tables(Ctx &c, int start)
     c.shell_nst.resize(ns);
     c.shell_off.resize(ns);
     c.shell_xyz.resize(3 * ns);
-    for (int i = 0; i < ns; ++i) {
+    c.shell_soff.resize(ns);
+    c.fn_sorted_to_orig.assign(N, 0);
+    c.fn_orig_to_sorted.assign(N, 0);
+    for (int i = 0, f = 0; i < ns; ++i) {
         const Shell &s = sh[c.sorted_orig[i]];
         c.orig_to_sorted[c.sorted_orig[i]] = i;
         c.shell_l[i] = s.lsh;
         c.shell_nst[i] = (s.lsh + 1) * (s.lsh + 2) / 2;
         c.shell_off[i] = s.off;
+        c.shell_soff[i] = f;
+        if (s.off < 0 || s.off + c.shell_nst[i] > N || f + c.shell_nst[i] > N)
+            return fail(PATMOL_HIP_ESTATE, "shell %d: functions [%d, %d) outside the basis of %d", c.sorted_orig[i], s.off,
+                        s.off + c.shell_nst[i], N);
+        for (int k = 0; k < c.shell_nst[i]; ++k, ++f) {
+            c.fn_sorted_to_orig[f] = s.off + k;
+            c.fn_orig_to_sorted[s.off + k] = f;
+        }
         for (int x = 0; x < 3; ++x) c.shell_xyz[3 * i + x] = ce[s.cen].r[x];
+    }
+    {
+        // the renumbering must be a permutation of the basis functions
+        std::vector<char> seen(N, 0);
+        int nseen = 0;
+        for (int f = 0; f < N; ++f) nseen += seen[c.fn_sorted_to_orig[f]]++ == 0;
+        if (nseen != N) return fail(PATMOL_HIP_ESTATE, "shell function ranges do not cover the %d basis functions", N);
     }
     // pairs I >= J (sorted) and their primitive pairs
     const int64_t np = (int64_t)ns * (ns + 1) / 2;
@@ -325,7 +346,8 @@ static int build_tables(Ctx &c, int start)
     }
     std::vector<int> orig_sorted_arr(ns);
     for (int i = 0; i < ns; ++i) orig_sorted_arr[i] = c.sorted_orig[i];
-    if (c.d_xyz.upload(c.shell_xyz) || c.d_off.upload(c.shell_off) || c.d_orig.upload(orig_sorted_arr) ||
+    if (c.d_xyz.upload(c.shell_xyz) || c.d_off.upload(c.shell_off) || c.d_soff.upload(c.shell_soff) ||
+        c.d_fperm.upload(c.fn_sorted_to_orig) || c.d_finv.upload(c.fn_orig_to_sorted) || c.d_orig.upload(orig_sorted_arr) ||
         c.d_pairs.upload(c.pairs) || c.d_pp.upload(c.pp) || c.d_tile_first.upload(c.tile_first) ||
         c.d_tile_size.upload(c.tile_size))
         return g_err;
@@ -416,12 +438,13 @@ static int build_tables(Ctx &c, int start)
     return 0;
 }
 
-// unpack packed lower triangle (no2) to full symmetric N x N
-__global__ void unpack_kernel(const double *__restrict__ p, double *__restrict__ f, int N)
+// unpack packed lower triangle (no2) to full symmetric N x N with rows and
+// columns in sorted function order: f[si][sj] = P(perm[si], perm[sj])
+__global__ void unpack_kernel(const double *__restrict__ p, double *__restrict__ f, const int *__restrict__ perm, int N)
 {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)N * N) return;
-    int i = (int)(idx / N), j = (int)(idx % N);
+    int i = perm[idx / N], j = perm[idx % N];
     int a = i >= j ? i : j, b = i >= j ? j : i;
     f[idx] = p[(int64_t)a * (a + 1) / 2 + b];
 }
@@ -433,9 +456,10 @@ __global__ void add_kernel(const double *__restrict__ a, const double *__restric
 }
 
 // G[ij] += cj (J'ij + J'ji) - (K'ij + K'ji), packed i >= j, with J' and K'
-// the sums of the per-XCD copies
+// the sums of the per-XCD copies, held in sorted function order (inv maps a
+// function to its sorted index)
 __global__ void finalize_kernel(const double *__restrict__ J, const double *__restrict__ K, double *__restrict__ G,
-                                int N, double cj)
+                                const int *__restrict__ inv, int N, double cj)
 {
     int64_t ij = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t no2 = (int64_t)N * (N + 1) / 2;
@@ -444,7 +468,8 @@ __global__ void finalize_kernel(const double *__restrict__ J, const double *__re
     while ((int64_t)(i + 1) * (i + 2) / 2 <= ij) ++i;
     while ((int64_t)i * (i + 1) / 2 > ij) --i;
     int j = (int)(ij - (int64_t)i * (i + 1) / 2);
-    const int64_t NN = (int64_t)N * N, a = (int64_t)i * N + j, b = (int64_t)j * N + i;
+    const int si = inv[i], sj = inv[j];
+    const int64_t NN = (int64_t)N * N, a = (int64_t)si * N + sj, b = (int64_t)sj * N + si;
     double jj = 0.0, kk = 0.0;
     for (int x = 0; x < PM_XCD_SLOTS; ++x) {
         jj += J[x * NN + a] + J[x * NN + b];
@@ -518,7 +543,7 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1
         a.tile_first = c.d_tile_first.p;
         a.tile_size = c.d_tile_size.p;
         a.shell_xyz = c.d_xyz.p;
-        a.shell_off = c.d_off.p;
+        a.shell_off = c.d_soff.p;   // D, J', K' in sorted function order
         a.shell_orig = c.d_orig.p;
         a.pairs = c.d_pairs.p;
         a.pair_q = c.d_q.p;
@@ -601,11 +626,12 @@ static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
 {
     if (alloc_work(c, 1)) return g_err;
     const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
-    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dP, c.d_D.p, c.N);
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dP, c.d_D.p, c.d_fperm.p, c.N);
     double dmax = 0.0;
     if (g_prec_mode == 1) dmax = device_absmax(c, dP, no2, s);
     if (run_jk(c, c.d_D.p, c.d_D.p, nullptr, 1, dmax, s)) return g_err;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, c.d_J.p, c.d_K0.p, dG, c.N, 2.0);
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, c.d_J.p, c.d_K0.p, dG, c.d_finv.p,
+                       c.N, 2.0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -617,15 +643,17 @@ static int uhf_device(Ctx &x, const double *dPA, const double *dPB, double *dGA,
 {
     if (alloc_work(x, 2)) return g_err;
     const int64_t NN = (int64_t)x.N * x.N, no2 = (int64_t)x.N * (x.N + 1) / 2;
-    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPA, x.d_D.p, x.N);
-    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPB, x.d_D2.p, x.N);
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPA, x.d_D.p, x.d_fperm.p, x.N);
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPB, x.d_D2.p, x.d_fperm.p, x.N);
     hipLaunchKernelGGL(add_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, x.d_D.p, x.d_D2.p, x.d_DJ.p, NN);
     double dmax = 0.0;
     if (g_prec_mode == 1) dmax = std::max(device_absmax(x, dPA, no2, s), device_absmax(x, dPB, no2, s));
     if (run_jk(x, x.d_DJ.p, x.d_D.p, nullptr, 1, dmax, s)) return g_err;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGA, x.N, 1.0);
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGA, x.d_finv.p,
+                       x.N, 1.0);
     if (run_jk(x, x.d_DJ.p, x.d_D2.p, nullptr, 1, dmax, s)) return g_err;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGB, x.N, 1.0);
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGB, x.d_finv.p,
+                       x.N, 1.0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -700,7 +728,8 @@ void clean_ERIFS(ERIFS *e)
         for (auto *b : {&c->d_xyz, &c->d_q, &c->d_bound, &c->d_P, &c->d_PB, &c->d_D, &c->d_D2, &c->d_DJ, &c->d_J,
                         &c->d_K0, &c->d_K1, &c->d_G, &c->d_G2, &c->d_dmax})
             b->release();
-        for (auto *b : {&c->d_off, &c->d_orig, &c->d_tile_first, &c->d_tile_size}) b->release();
+        for (auto *b : {&c->d_off, &c->d_soff, &c->d_fperm, &c->d_finv, &c->d_orig, &c->d_tile_first, &c->d_tile_size})
+            b->release();
         c->d_pairs.release();
         c->d_pp.release();
         c->d_items.release();

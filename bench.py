@@ -283,10 +283,10 @@ def main():
     nupd = scf["iterations"] + 1
     scf_dev = {"energy": scf["energy"], "iterations": scf["iterations"], "converged": scf["converged"],
                "ms_per_iteration": scf["total_ms"] / nupd, "gbuild_ms_per_iteration": scf["gbuild_ms"] / nupd,
-               "other_ms_per_iteration": scf["other_ms"] / nupd}
+               "other_ms_per_iteration": scf["other_ms"] / nupd, "purify_fallbacks": scf["purify_fallbacks"]}
     log(f"device SCF: E {scf['energy']:.10f} in {scf['iterations']} iterations, "
         f"{scf_dev['ms_per_iteration']:.2f} ms/iteration (G {scf_dev['gbuild_ms_per_iteration']:.2f}, "
-        f"rest {scf_dev['other_ms_per_iteration']:.2f})")
+        f"rest {scf_dev['other_ms_per_iteration']:.2f}), purification fallbacks {scf['purify_fallbacks']}")
 
     use_mixed = dE <= 1e-6 and stm["work_items_fp32"] > 0 and msmx < ms64
     ms = msmx if use_mixed else ms64

@@ -13,8 +13,8 @@
 //   C = X V dgemm (form_cmatrix, scfaux.c:5-21);
 //   P = C_occ C_occ^T     dsyrk + pack (form_pmatrix, scfaux.c:23-41);
 //   sums    calc_energy and pdiffNorm (scfaux.c:97-136), and the three dot
-//           products of scfAccelerator (scfaux.c:64-95), each one
-//           fixed-order block reduction;
+//           products of scfAccelerator (scfaux.c:64-95), each in a fixed
+//           summation order;
 //   damping the averaging / two-point Anderson updates (rhfscf.c:60-69).
 // The host keeps only the iteration logic: per iteration it reads back the
 // energy and density change (two doubles) to take the reference's
@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "patmol_hip.h"
@@ -116,37 +117,48 @@ __global__ void pack_lower_kernel(const double *__restrict__ D, double *__restri
 }
 
 // Weighted sums over packed triangles, off-diagonal weight 1 and diagonal
-// 1/2 (calc_energy and pdiffNorm, scfaux.c:97-136), one block, fixed order:
-//   out[0] = sum w H P1, out[1] = sum w G P1, out[2] = sum w (P - P1)^2.
-__global__ __launch_bounds__(RED_THREADS) void scf_sums_kernel(const double *__restrict__ P,
-                                                              const double *__restrict__ P1,
-                                                              const double *__restrict__ H,
-                                                              const double *__restrict__ G, int n,
-                                                              double *__restrict__ out)
+// 1/2 (calc_energy and pdiffNorm, scfaux.c:97-136):
+//   out[0] = sum w H P1, out[1] = sum w G P1, out[2] = sum w (P - P1)^2,
+// in a fixed order: SUMS_BLOCKS blocks each reduce one contiguous chunk of
+// the packed triangle (strided over the threads, then a fixed tree), and one
+// thread per sum adds the block partials in block order.
+constexpr int SUMS_BLOCKS = 64;
+__global__ __launch_bounds__(256) void scf_sums_part_kernel(const double *__restrict__ P, const double *__restrict__ P1,
+                                                           const double *__restrict__ H, const double *__restrict__ G,
+                                                           int64_t no2, double *__restrict__ part)
 {
-    __shared__ double red[3][RED_THREADS];
+    __shared__ double red[3][256];
+    const int64_t chunk = (no2 + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < no2 ? lo + chunk : no2;
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const size_t r0 = (size_t)i * (i + 1) / 2;
-        for (int j = threadIdx.x; j <= i; j += RED_THREADS) {
-            const size_t ij = r0 + j;
-            const double w = j == i ? 0.5 : 1.0, p1 = P1[ij], d = P[ij] - p1;
-            a += w * H[ij] * p1;
-            b += w * G[ij] * p1;
-            c += w * d * d;
-        }
+    for (int64_t ij = lo + threadIdx.x; ij < hi; ij += 256) {
+        // row i of the packed index: i(i+1)/2 <= ij < (i+1)(i+2)/2
+        int64_t i = (int64_t)((sqrt(8.0 * (double)ij + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= ij) ++i;
+        while (i * (i + 1) / 2 > ij) --i;
+        const double w = ij == i * (i + 3) / 2 ? 0.5 : 1.0, p1 = P1[ij], d = P[ij] - p1;
+        a += w * H[ij] * p1;
+        b += w * G[ij] * p1;
+        c += w * d * d;
     }
     red[0][threadIdx.x] = a;
     red[1][threadIdx.x] = b;
     red[2][threadIdx.x] = c;
     __syncthreads();
-    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 3; ++q) out[q] = red[q][0];
+    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void scf_sums_final_kernel(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    if (threadIdx.x >= 3) return;
+    double v = 0.0;
+    for (int b = 0; b < nb; ++b) v += part[3 * b + threadIdx.x];
+    out[threadIdx.x] = v;
 }
 
 // dot products of scfAccelerator (scfaux.c:70-77)
@@ -248,7 +260,8 @@ __global__ __launch_bounds__(256) void gersh_reduce_kernel(const double *__restr
 }
 
 // X0 = (emax I - F) / (emax - emin): spectrum mapped into [0, 1], occupied
-// levels towards 1
+// levels towards 1.  F is symmetrised on the way ((F + F^T) / 2), so that X0
+// and every later X are exactly symmetric (tc2_step_kernel relies on it).
 __global__ void tc2_init_kernel(const double *__restrict__ F, double *__restrict__ X, int n,
                                 const double *__restrict__ bounds)
 {
@@ -257,44 +270,160 @@ __global__ void tc2_init_kernel(const double *__restrict__ F, double *__restrict
     const double emin = bounds[0], emax = bounds[1];
     const double sc = 1.0 / (emax - emin);
     const int i = (int)(k % n), j = (int)(k / n);
-    X[k] = ((i == j ? emax : 0.0) - F[k]) * sc;
+    const double f = 0.5 * (F[k] + F[(int64_t)i * n + j]);
+    X[k] = ((i == j ? emax : 0.0) - f) * sc;
 }
 
-// traces of X and X^2 (one block); st[0] = tr X, st[1] = tr X^2, and the
-// choice of the next step: st[2] = 1 for X^2, 0 for 2X - X^2, whichever
-// brings the trace closer to the occupation
-__global__ __launch_bounds__(256) void tc2_trace_kernel(const double *__restrict__ X, const double *__restrict__ X2,
-                                                        int n, double nocc, double *__restrict__ st)
+// Sum of a and b over the 256 threads of a block in a fixed order (xor
+// butterflies inside each wave, then the four waves in order); the result is
+// valid in thread 0.
+__device__ __forceinline__ void block_sum2_256(double &a, double &b, double *sh)
 {
-    __shared__ double a[256], b[256];
-    double t1 = 0.0, t2 = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        t1 += X[(size_t)i * n + i];
-        t2 += X2[(size_t)i * n + i];
+    for (int d = 32; d > 0; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        b += __shfl_xor(b, d, 64);
     }
-    a[threadIdx.x] = t1;
-    b[threadIdx.x] = t2;
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[2 * wave] = a;
+        sh[2 * wave + 1] = b;
+    }
     __syncthreads();
-    for (int s2 = 128; s2 > 0; s2 >>= 1) {
-        if (threadIdx.x < s2) {
-            a[threadIdx.x] += a[threadIdx.x + s2];
-            b[threadIdx.x] += b[threadIdx.x + s2];
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        st[0] = a[0];
-        st[1] = b[0];
-        st[2] = fabs(b[0] - nocc) < fabs(2.0 * a[0] - b[0] - nocc) ? 1.0 : 0.0;
+        a = ((sh[0] + sh[2]) + sh[4]) + sh[6];
+        b = ((sh[1] + sh[3]) + sh[5]) + sh[7];
     }
 }
 
-__global__ void tc2_update_kernel(double *__restrict__ X, const double *__restrict__ X2, int64_t nn,
-                                  const double *__restrict__ st)
+// Purification steps in TC2_T x TC2_T tiles of X (column-major n x n), one
+// 256-thread block per tile: block b covers rows (b % nt) TC2_T.. and columns
+// (b / nt) TC2_T.., nt = ceil(n / TC2_T).  part[2b] = the tile's share of
+// tr X, part[2b + 1] of sum X_ij^2 (= tr X^2 for symmetric X).
+constexpr int TC2_T = 32;
+constexpr int TC2_KB = 14;   // k-steps of 4 whose operands a wave loads before their MFMAs
+typedef double tc2_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void tc2_partials_kernel(const double *__restrict__ X, int n,
+                                                          double *__restrict__ part)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nn) return;
-    X[k] = st[2] != 0.0 ? X2[k] : 2.0 * X[k] - X2[k];
+    __shared__ double sh[8];
+    const int nt = (n + TC2_T - 1) / TC2_T;
+    const int i0 = (blockIdx.x % nt) * TC2_T, j0 = (blockIdx.x / nt) * TC2_T;
+    double tr = 0.0, sq = 0.0;
+    for (int e = threadIdx.x; e < TC2_T * TC2_T; e += 256) {
+        const int i = i0 + e % TC2_T, j = j0 + e / TC2_T;
+        if (i < n && j < n) {
+            const double v = X[(size_t)j * n + i];
+            sq += v * v;
+            if (i == j) tr += v;
+        }
+    }
+    block_sum2_256(tr, sq, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = tr;
+        part[2 * blockIdx.x + 1] = sq;
+    }
+}
+
+// One step X <- X^2 or 2X - X^2 for an exactly symmetric X (trace-correcting
+// purification, Niklasson 2002).  The choice comes from tr X and tr X^2,
+// summed from the previous step's tile partials `pin` in the same fixed order
+// by every block (xor butterflies: every lane and block gets the same bits):
+// X^2 when |tr X^2 - nocc| < |2 tr X - tr X^2 - nocc|.  Block 0 stores tr X,
+// tr X^2 and the choice in st[0..2] for the host's convergence check.
+// Y = X X on the FP64 matrix cores: the block's 32 x 32 tile of Y is four
+// 16 x 16 sub-tiles; every wave computes all four over a quarter of k (wave w
+// takes k in [w kc, (w + 1) kc)) with v_mfma_f64_16x16x4f64 (lane l supplies
+// A(l % 16, k + l / 16) and B(k + l / 16, l % 16) and receives
+// Y(l / 16 + 4q, l % 16), q = 0..3), its operands loaded TC2_KB k-steps
+// ahead; as X is symmetric, B(k, j) = X(j, k) and both operands are read down
+// columns.  Wave w then adds sub-tile w over the four waves in wave order and
+// writes the new X to Xn (another buffer); tile partials go to pout.
+__global__ __launch_bounds__(256) void tc2_step_kernel(const double *__restrict__ X, double *__restrict__ Xn, int n,
+                                                      const double *__restrict__ pin, double *__restrict__ pout,
+                                                      int nblk, double nocc, double *__restrict__ st)
+{
+    __shared__ double sh[8];
+    __shared__ tc2_d4 sacc[4][4][64];   // [wave][sub-tile][lane]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nt = (n + TC2_T - 1) / TC2_T;
+    const int i0 = (blockIdx.x % nt) * TC2_T, j0 = (blockIdx.x / nt) * TC2_T;
+    const int r = lane & 15, kq = lane >> 4;
+    // sub-tile w of this wave's epilogue: rows i0 + 16 (w & 1), columns
+    // j0 + 16 (w >> 1); the old X at its four outputs per lane, loaded early
+    const int ei0 = i0 + 16 * (wave & 1), ej = j0 + 16 * (wave >> 1) + r;
+    double xold[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = ei0 + kq + 4 * q;
+        xold[q] = (i < n && ej < n) ? X[(size_t)i * n + ej] : 0.0;
+    }
+    const bool rin0 = i0 + r < n, rin1 = i0 + 16 + r < n, cin0 = j0 + r < n, cin1 = j0 + 16 + r < n;
+    const int kc = 4 * ((n + 15) / 16);   // k range of one wave, a multiple of 4
+    const int kbeg = wave * kc, kend = kbeg + kc;
+    tc2_d4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = tc2_d4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kbeg; k0 < kend; k0 += 4 * TC2_KB) {
+        double a0[TC2_KB], a1[TC2_KB], b0[TC2_KB], b1[TC2_KB];
+#pragma unroll
+        for (int t = 0; t < TC2_KB; ++t) {
+            const int kk = k0 + 4 * t + kq;
+            const bool kin = kk < n && kk < kend;
+            const double *col = X + (size_t)kk * n;
+            a0[t] = (rin0 && kin) ? col[i0 + r] : 0.0;
+            a1[t] = (rin1 && kin) ? col[i0 + 16 + r] : 0.0;
+            b0[t] = (cin0 && kin) ? col[j0 + r] : 0.0;
+            b1[t] = (cin1 && kin) ? col[j0 + 16 + r] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < TC2_KB; ++t) {
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0[t], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b0[t], acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b1[t], acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1[t], acc[3], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) sacc[wave][t][lane] = acc[t];
+    // traces of the old X (needed only from here on)
+    double tr = 0.0, sq = 0.0;
+    for (int b = lane; b < nblk; b += 64) {
+        tr += pin[2 * b];
+        sq += pin[2 * b + 1];
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        tr += __shfl_xor(tr, d, 64);
+        sq += __shfl_xor(sq, d, 64);
+    }
+    const bool square = fabs(sq - nocc) < fabs(2.0 * tr - sq - nocc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[0] = tr;
+        st[1] = sq;
+        st[2] = square ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // Y(i, j), i = ei0 + kq + 4q, j = ej, stored at (j, i), which holds the
+    // same value (Y and X symmetric), so the 16 lanes of a row group write
+    // one contiguous run
+    const tc2_d4 y = ((sacc[0][wave][lane] + sacc[1][wave][lane]) + sacc[2][wave][lane]) + sacc[3][wave][lane];
+    double ptr = 0.0, psq = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = ei0 + kq + 4 * q;
+        if (i < n && ej < n) {
+            const size_t e = (size_t)i * n + ej;
+            const double v = square ? y[q] : 2.0 * xold[q] - y[q];
+            Xn[e] = v;
+            psq += v * v;
+            if (i == ej) ptr += v;
+        }
+    }
+    block_sum2_256(ptr, psq, sh);
+    if (threadIdx.x == 0) {
+        pout[2 * blockIdx.x] = ptr;
+        pout[2 * blockIdx.x + 1] = psq;
+    }
 }
 
 struct DevMem {
@@ -323,6 +452,8 @@ struct Scf {
     DevMem mem;
     double *X = nullptr, *F = nullptr, *T = nullptr, *C = nullptr, *ev = nullptr, *off = nullptr, *sums = nullptr;
     double *X2 = nullptr, *glo = nullptr, *ghi = nullptr, *tc = nullptr;
+    double *part = nullptr, *spart = nullptr;   // tile partials of the purification, block partials of the sums
+    int nblk = 0;                               // purification tiles
     bool purify = false;   // density by purification inside the iteration loop
     int purify_fallbacks = 0;
     rocblas_int *info = nullptr;
@@ -358,7 +489,11 @@ struct Scf {
         glo = mem.get<double>(n);
         ghi = mem.get<double>(n);
         tc = mem.get<double>(8);
-        if (!X || !F || !T || !C || !ev || !off || !sums || !info || !X2 || !glo || !ghi || !tc)
+        const int nt = (n + TC2_T - 1) / TC2_T;
+        nblk = nt * nt;
+        part = mem.get<double>(4 * (size_t)nblk);
+        spart = mem.get<double>(3 * SUMS_BLOCKS);
+        if (!X || !F || !T || !C || !ev || !off || !sums || !info || !X2 || !glo || !ghi || !tc || !part || !spart)
             return fail("device allocation failed");
         return true;
     }
@@ -433,17 +568,17 @@ struct Scf {
         if (nocc <= 0 || nocc >= n) return true;   // the caller diagonalises these
         hipLaunchKernelGGL(gersh_row_kernel, dim3(n), dim3(256), 0, s, F, n, glo, ghi);
         hipLaunchKernelGGL(gersh_reduce_kernel, dim3(1), dim3(256), 0, s, glo, ghi, n, tc + 4);
-        hipLaunchKernelGGL(tc2_init_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, F, X2, n, tc + 4);
-        // X2 holds X0; iterate in (T = X, X2 = X^2)
-        if (hipMemcpyAsync(T, X2, sizeof(double) * nn, hipMemcpyDeviceToDevice, s) != hipSuccess)
-            return fail("copy failed");
+        // X0 into cur with its tile partials; each step writes the next X
+        // into the other buffer
+        double *cur = T, *nxt = X2, *pin = part, *pout = part + 2 * nblk;
+        hipLaunchKernelGGL(tc2_init_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, F, cur, n, tc + 4);
+        hipLaunchKernelGGL(tc2_partials_kernel, dim3(nblk), dim3(256), 0, s, cur, n, pin);
         int extra = -1;
         for (int it = 0; it < 120; ++it) {
-            if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, T, n, T, n, &zero, X2,
-                              n) != rocblas_status_success)
-                return fail("rocblas_dgemm failed");
-            hipLaunchKernelGGL(tc2_trace_kernel, dim3(1), dim3(256), 0, s, T, X2, n, (double)nocc, tc);
-            hipLaunchKernelGGL(tc2_update_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, T, X2, nn, tc);
+            hipLaunchKernelGGL(tc2_step_kernel, dim3(nblk), dim3(256), 0, s, cur, nxt, n, pin, pout, nblk,
+                               (double)nocc, tc);
+            std::swap(cur, nxt);
+            std::swap(pin, pout);
             if (extra > 0) {
                 if (--extra == 0) {
                     converged = true;
@@ -460,13 +595,13 @@ struct Scf {
             }
         }
         if (!converged) return true;
-        // P = X P' X^T (lower triangle into C as scratch), then pack
-        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, T, n, &zero, X2, n) !=
-                rocblas_status_success ||
-            rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, X2, n, X, n, &zero, T,
+        // P = X P' X^T (P' in cur, nxt as scratch), then pack
+        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, cur, n, &zero, nxt,
+                          n) != rocblas_status_success ||
+            rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, nxt, n, X, n, &zero, cur,
                           n) != rocblas_status_success)
             return fail("rocblas_dgemm failed");
-        hipLaunchKernelGGL(pack_lower_kernel, dim3(n), dim3(256), 0, s, T, P, n);
+        hipLaunchKernelGGL(pack_lower_kernel, dim3(n), dim3(256), 0, s, cur, P, n);
         return hipGetLastError() == hipSuccess || fail("launch failure");
     }
 
@@ -518,7 +653,8 @@ struct Scf {
     // sums over (P, P1, H, G) -> host
     bool sums3(const double *P, const double *P1, const double *H, const double *G, double *out)
     {
-        hipLaunchKernelGGL(scf_sums_kernel, dim3(1), dim3(RED_THREADS), 0, s, P, P1, H, G, n, sums);
+        hipLaunchKernelGGL(scf_sums_part_kernel, dim3(SUMS_BLOCKS), dim3(256), 0, s, P, P1, H, G, (int64_t)no2, spart);
+        hipLaunchKernelGGL(scf_sums_final_kernel, dim3(1), dim3(64), 0, s, spart, SUMS_BLOCKS, sums);
         if (hipMemcpyAsync(out, sums, sizeof(double) * 3, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             return fail("device failure in the energy sums");

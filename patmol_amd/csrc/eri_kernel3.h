@@ -36,9 +36,11 @@ struct K3 {
 // Per-class launch shape, from timing every class on (H2O)8 with both
 // digestion orders, with and without the density-read fence, at 1 and 2
 // waves per SIMD, with and without splitting the bra components over the
-// waves: code = rys_lds * 128 + (split_b - 1) * 32 + (split_a - 1) * 16 +
-// fence * 8 + waves * 2 + rootwise (rys_lds: the Rys coefficients of the
-// class's root count are staged in LDS also above three roots).
+// waves: code = rys_lds * 128 + ksplit * 64 + (split_b - 1) * 32 +
+// (split_a - 1) * 16 + fence * 8 + waves * 2 + rootwise (rys_lds: the Rys
+// coefficients of the class's root count are staged in LDS also above three
+// roots; ksplit: every item is two workgroups, each taking half of the ket
+// tile's K shells).
 constexpr int jk3_shape(int la, int lb, int lc, int ld)
 {
     const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
@@ -65,10 +67,10 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 66: return 1 * 2 + 0;   // (dp|ps)
     case 67: return 1 * 2 + 1;   // (dp|pp)
     case 69: return 1 * 2 + 1;   // (dp|ds)
-    case 70: return 16 + 8 + 1 * 2 + 1;   // (dp|dp)
+    case 70: return 64 + 16 + 8 + 1 * 2 + 1;   // (dp|dp)
     case 72: return 1 * 2 + 0;   // (dd|ss)
     case 75: return 1 * 2 + 1;   // (dd|ps)
-    case 76: return 128 + 32 + 16 + 1 * 2 + 1;   // (dd|pp)
+    case 76: return 128 + 64 + 32 + 16 + 1 * 2 + 1;   // (dd|pp)
     case 78: return 128 + 32 + 1 * 2 + 1;   // (dd|ds)
     case 79: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dp)
 #ifdef PM_SHAPE_DDDD
@@ -100,6 +102,15 @@ template <int LA, int LB, int LC, int LD>
 constexpr bool jk3_rys_lds()
 {
     return (LA + LB + LC + LD) / 2 + 1 <= PM_RYS_LDS_MAXN || ((jk3_shape(LA, LB, LC, LD) >> 7) & 1);
+}
+
+// Workgroups per work item (jk3_shape's ksplit): the items of the largest
+// classes take hundreds of microseconds each and there are only a few
+// hundred of them, so halving them evens out the end of the launch.
+template <int LA, int LB, int LC, int LD>
+constexpr int jk3_wg_per_item()
+{
+    return ((jk3_shape(LA, LB, LC, LD) >> 6) & 1) ? 2 : 1;
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -194,7 +205,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     constexpr int oIJ = 0, oKL = oIJ + FI * SJ, oIK = oKL + FK * SL, oIL = oIK + FI * SK, oJK = oIL + FI * SL,
                   oJL = oJK + FJ * SK, oEND = oJL + FJ * SL;
 
-    const int item = blockIdx.x;
+    // with KS workgroups per item, workgroups KS i .. KS i + KS - 1 share
+    // item i and each takes one range of the K shells of the ket tile
+    constexpr int KS = jk3_wg_per_item<LA, LB, LC, LD>();
+    const int item = (int)(blockIdx.x / (unsigned)KS), ksub = (int)(blockIdx.x % (unsigned)KS);
     if (item >= a.nitems) return;
     const int4 it = a.items[item];
     const int tid = threadIdx.x;
@@ -333,7 +347,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     for (int i = 0; i < NAH * NBH; ++i) jab[i] = 0.0;
     const double *__restrict__ DIJ = sD + oIJ + (ii * NA) * SJ + jj * NB;
 
-    for (int kq = wave / SPLIT; kq < TK; kq += KSTRIDE) {
+    constexpr int KPER = (TK + KS - 1) / KS;
+    const int kq_end = min(TK, (ksub + 1) * KPER);
+    for (int kq = ksub * KPER + wave / SPLIT; kq < kq_end; kq += KSTRIDE) {
         if (kq >= nK) break;  // uniform
         const int K = fK + kq;
         double kac[NAH * NC], kbc[NBH * NC];

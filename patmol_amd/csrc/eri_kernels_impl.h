@@ -713,10 +713,12 @@ inline void launch_jk_t(const JKArgs &args, int prec, int nk, hipStream_t s)
         else
             hipLaunchKernelGGL((jk_kernel2<LA, LB, LC, LD, float>), grid, block, 0, s, args);
 #else
+        // jk_kernel3 may take several workgroups per item (jk3_wg_per_item)
+        const dim3 grid3((unsigned)args.nitems * (unsigned)jk3_wg_per_item<LA, LB, LC, LD>());
         if (prec == 0)
-            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, double>), grid, block, 0, s, args);
+            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, double>), grid3, block, 0, s, args);
         else
-            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, float>), grid, block, 0, s, args);
+            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, float>), grid3, block, 0, s, args);
 #endif
     }
 }

@@ -137,8 +137,10 @@ int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Sma
  * purification of X^T F X for bases of 128 functions or more (GEMMs only;
  * an iteration whose purification does not converge, e.g. for a degenerate
  * Fermi level, is diagonalised instead), mode 0 by the eigensolver every
- * iteration as the reference does.  The guess and the returned orbitals
- * always come from the eigensolver. */
+ * iteration as the reference does.  Mode 2 is mode 1 with the traces of
+ * each purification step summed by a kernel of their own (the form mode 1
+ * takes by itself above 1024 tiles of 32 x 32, N > 1024).  The guess and the
+ * returned orbitals always come from the eigensolver. */
 void patmol_hip_set_scf_density(int mode);
 /* One diagonalisation (spack_trans + spack_diag + form_cmatrix +
  * form_pmatrix) of host packed F in the basis orthogonalised by S: ascending

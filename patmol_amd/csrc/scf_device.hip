@@ -44,7 +44,7 @@ int aux_fail(int code, const char *msg);
 
 namespace {
 
-int g_scf_density = 1;   // 1: purification inside the loop, 0: eigensolver
+int g_scf_density = 1;   // 1: purification inside the loop, 2: the same with separate trace totals, 0: eigensolver
 
 constexpr int RED_THREADS = 1024;
 
@@ -325,12 +325,48 @@ __global__ __launch_bounds__(256) void tc2_partials_kernel(const double *__restr
     }
 }
 
+// tr X and tr X^2 from the tile partials: lane-strided sums, then xor
+// butterflies (every lane, and every block that does this, gets the same
+// bits); the TC2 choice X^2 when |tr X^2 - nocc| < |2 tr X - tr X^2 - nocc|.
+__device__ __forceinline__ bool tc2_choice(const double *__restrict__ pin, int nblk, double nocc, double &tr,
+                                           double &sq)
+{
+    const int lane = threadIdx.x & 63;
+    tr = 0.0;
+    sq = 0.0;
+    for (int b = lane; b < nblk; b += 64) {
+        tr += pin[2 * b];
+        sq += pin[2 * b + 1];
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        tr += __shfl_xor(tr, d, 64);
+        sq += __shfl_xor(sq, d, 64);
+    }
+    return fabs(sq - nocc) < fabs(2.0 * tr - sq - nocc);
+}
+
+// Above TC2_FUSED_MAX tiles the totals are formed once per step by this
+// one-wave kernel instead of by every block of the step kernel (whose cost
+// would grow as the square of the tile count).
+constexpr int TC2_FUSED_MAX = 1024;
+__global__ __launch_bounds__(64) void tc2_total_kernel(const double *__restrict__ pin, int nblk, double nocc,
+                                                       double *__restrict__ st)
+{
+    double tr, sq;
+    const bool square = tc2_choice(pin, nblk, nocc, tr, sq);
+    if (threadIdx.x == 0) {
+        st[0] = tr;
+        st[1] = sq;
+        st[2] = square ? 1.0 : 0.0;
+    }
+}
+
 // One step X <- X^2 or 2X - X^2 for an exactly symmetric X (trace-correcting
-// purification, Niklasson 2002).  The choice comes from tr X and tr X^2,
-// summed from the previous step's tile partials `pin` in the same fixed order
-// by every block (xor butterflies: every lane and block gets the same bits):
-// X^2 when |tr X^2 - nocc| < |2 tr X - tr X^2 - nocc|.  Block 0 stores tr X,
-// tr X^2 and the choice in st[0..2] for the host's convergence check.
+// purification, Niklasson 2002).  The choice comes from tr X and tr X^2:
+// summed from the previous step's tile partials `pin` by every block
+// (tc2_choice; block 0 then stores tr X, tr X^2 and the choice in st[0..2]
+// for the host's convergence check), or, with `presummed`, read from st
+// where tc2_total_kernel put them.
 // Y = X X on the FP64 matrix cores: the block's 32 x 32 tile of Y is four
 // 16 x 16 sub-tiles; every wave computes all four over a quarter of k (wave w
 // takes k in [w kc, (w + 1) kc)) with v_mfma_f64_16x16x4f64 (lane l supplies
@@ -341,7 +377,7 @@ __global__ __launch_bounds__(256) void tc2_partials_kernel(const double *__restr
 // writes the new X to Xn (another buffer); tile partials go to pout.
 __global__ __launch_bounds__(256) void tc2_step_kernel(const double *__restrict__ X, double *__restrict__ Xn, int n,
                                                       const double *__restrict__ pin, double *__restrict__ pout,
-                                                      int nblk, double nocc, double *__restrict__ st)
+                                                      int nblk, double nocc, double *__restrict__ st, int presummed)
 {
     __shared__ double sh[8];
     __shared__ tc2_d4 sacc[4][4][64];   // [wave][sub-tile][lane]
@@ -386,21 +422,18 @@ __global__ __launch_bounds__(256) void tc2_step_kernel(const double *__restrict_
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) sacc[wave][t][lane] = acc[t];
-    // traces of the old X (needed only from here on)
-    double tr = 0.0, sq = 0.0;
-    for (int b = lane; b < nblk; b += 64) {
-        tr += pin[2 * b];
-        sq += pin[2 * b + 1];
-    }
-    for (int d = 32; d > 0; d >>= 1) {
-        tr += __shfl_xor(tr, d, 64);
-        sq += __shfl_xor(sq, d, 64);
-    }
-    const bool square = fabs(sq - nocc) < fabs(2.0 * tr - sq - nocc);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st[0] = tr;
-        st[1] = sq;
-        st[2] = square ? 1.0 : 0.0;
+    // the choice for the old X (needed only from here on)
+    bool square;
+    if (presummed) {
+        square = st[2] != 0.0;
+    } else {
+        double tr, sq;
+        square = tc2_choice(pin, nblk, nocc, tr, sq);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st[0] = tr;
+            st[1] = sq;
+            st[2] = square ? 1.0 : 0.0;
+        }
     }
     __syncthreads();
     // Y(i, j), i = ei0 + kq + 4q, j = ej, stored at (j, i), which holds the
@@ -575,8 +608,10 @@ struct Scf {
         hipLaunchKernelGGL(tc2_partials_kernel, dim3(nblk), dim3(256), 0, s, cur, n, pin);
         int extra = -1;
         for (int it = 0; it < 120; ++it) {
+            const int presummed = nblk > TC2_FUSED_MAX || g_scf_density == 2;
+            if (presummed) hipLaunchKernelGGL(tc2_total_kernel, dim3(1), dim3(64), 0, s, pin, nblk, (double)nocc, tc);
             hipLaunchKernelGGL(tc2_step_kernel, dim3(nblk), dim3(256), 0, s, cur, nxt, n, pin, pout, nblk,
-                               (double)nocc, tc);
+                               (double)nocc, tc, presummed);
             std::swap(cur, nxt);
             std::swap(pin, pout);
             if (extra > 0) {
@@ -756,7 +791,7 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     if (eps < EPS10 * n * n) eps = EPS10 * n * n;
     // below ~128 functions the eigensolver is faster than the ~40 GEMMs of
     // the purification (their launches and trace checks dominate)
-    c.purify = g_scf_density == 1 && n >= 128;
+    c.purify = g_scf_density >= 1 && n >= 128;
     const bool log = scf_log();
     double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0, dms = 0.0;
     int iteration = 0, done = 0;
@@ -880,7 +915,7 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     if (eps < EPS10 * n * n) eps = EPS10 * n * n;
     // below ~128 functions the eigensolver is faster than the ~40 GEMMs of
     // the purification (their launches and trace checks dominate)
-    c.purify = g_scf_density == 1 && n >= 128;
+    c.purify = g_scf_density >= 1 && n >= 128;
     const bool log = scf_log();
     double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0;
     int iteration = 0, done = 0;
@@ -988,4 +1023,4 @@ extern "C" int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int 
     return 0;
 }
 
-extern "C" void patmol_hip_set_scf_density(int mode) { pm::g_scf_density = mode == 0 ? 0 : 1; }
+extern "C" void patmol_hip_set_scf_density(int mode) { pm::g_scf_density = (mode == 0 || mode == 2) ? mode : 1; }

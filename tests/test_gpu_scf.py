@@ -144,17 +144,19 @@ def test_sf6_golden_energy_device(golden_dir):
     assert np.abs(ev[: s.nelec // 2] - occ).max() < 2e-5
 
 
-def test_purification_matches_eigensolver_benzene():
-    """The 300-function benzene ring: the SCF with purified densities and
-    the SCF with the eigensolver in every iteration agree."""
+@pytest.mark.parametrize("mode", [1, 2])
+def test_purification_matches_eigensolver_benzene(mode):
+    """The 300-function benzene ring: the SCF with purified densities (mode
+    2: trace totals from their own kernel, the large-N form) and the SCF with
+    the eigensolver in every iteration agree."""
     s = systems.benzene_ring()
     pm.load_system(s)
-    pm.set_options(tau=1e-12, precision=0, scf_density=1)
+    pm.set_options(tau=1e-12, precision=0, scf_density=mode)
     r1, P1, e1 = pm.rhf_scf_device()
     pm.set_options(scf_density=0)
     r0, P0, e0 = pm.rhf_scf_device()
     pm.set_options(scf_density=1)
-    note(f"benzene ring: purified E={r1['energy']:.12f} ({r1['iterations']} it, "
+    note(f"benzene ring (mode {mode}): purified E={r1['energy']:.12f} ({r1['iterations']} it, "
          f"{r1['total_ms'] / (r1['iterations'] + 1):.2f} ms/it), eigensolver E={r0['energy']:.12f} "
          f"({r0['iterations']} it, {r0['total_ms'] / (r0['iterations'] + 1):.2f} ms/it)")
     assert r1["converged"] == 1 and r0["converged"] == 1 and r1["purify_fallbacks"] == 0

@@ -9,6 +9,8 @@ import ctypes as C
 import os
 import subprocess
 
+import math
+
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -334,8 +336,10 @@ def full_to_packed(F: np.ndarray) -> np.ndarray:
 
 # ---------------------------------------------------------------------------
 # CPU mirror of the screened work list (tile quartets) the GPU library
-# builds in two_electron_ints.  Shells sorted by (l, most diffuse exponent
-# descending, centre, index), tiles of
+# builds in two_electron_ints.  Shells sorted by (l, exponent band
+# floor(log_9 of the most diffuse exponent) descending, Morton code of the
+# centre in the centres' bounding box (10 bits per axis), centre, most diffuse
+# exponent descending, index), tiles of
 # 8 (s, p) or 4 (d) shells, tile-pair bound = max Schwarz Q in the tile pair,
 # items = canonical tile-pair pairs with bound product >= tau, bra = higher
 # pair class, sorted by (bra class, ket class, bound) descending.
@@ -345,7 +349,22 @@ def work_items_mirror(system, q_orig, tau, start=0):
     cens = [s.cen for s in system.shells]
     ns = len(ls)
     amin = [min(s.al) for s in system.shells]
-    order = sorted(range(ns), key=lambda i: (ls[i], -amin[i], cens[i], i))
+    band = [math.floor(math.log(a) / math.log(9.0) + 1e-9) for a in amin]
+    xyz = np.asarray(system.coords, dtype=np.float64)
+    pts = xyz[cens]
+    lo, hi = pts.min(axis=0), pts.max(axis=0)
+
+    def morton(c):
+        code = 0
+        for x in range(3):
+            t = float((xyz[c, x] - lo[x]) / (hi[x] - lo[x])) if hi[x] > lo[x] else 0.0
+            g = int(t * 1023.0)
+            for b in range(10):
+                code |= ((g >> b) & 1) << (3 * b + x)
+        return code
+
+    zc = [morton(c) for c in cens]
+    order = sorted(range(ns), key=lambda i: (ls[i], -band[i], zc[i], cens[i], -amin[i], i))
     lsort = [ls[i] for i in order]
 
     def tile_of(l):

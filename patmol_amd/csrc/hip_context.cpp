@@ -250,20 +250,43 @@ static int build_tables(Ctx &c, int start)
     // sort shells (tile order)
     c.sorted_orig.resize(ns);
     std::iota(c.sorted_orig.begin(), c.sorted_orig.end(), 0);
-    // Tiles group shells of one l with similar extent: sorted by l, then by
-    // the most diffuse exponent (descending), then centre.  Shell pairs of a
+    // Tiles group shells of one l with similar extent and position: sorted
+    // by l, then by exponent band (floor of log_9 of the most diffuse
+    // exponent, descending), then along a Morton curve through the centre
+    // positions (10 bits per axis over the centres' bounding box), then by
+    // centre, most diffuse exponent (descending) and index.  Shell pairs of a
     // tile pair then have similar Schwarz bounds, so a kept tile quartet has
-    // few screened-out lanes (about 62% of lanes busy on (H2O)8, against 55%
-    // with centre-major tiles).
+    // few screened-out lanes: 65% of the lanes of the executed steps are busy
+    // on (H2O)8, against 62% with exponent-major and 55% with centre-major
+    // tiles.
     std::vector<double> amin(ns);
+    std::vector<int> band(ns);
+    std::vector<uint32_t> zc(ns);
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int s = 0; s < ns; ++s)
+        for (int x = 0; x < 3; ++x) {
+            const double r = ce[sh[s].cen].r[x];
+            lo[x] = s == 0 ? r : std::min(lo[x], r);
+            hi[x] = s == 0 ? r : std::max(hi[x], r);
+        }
     for (int s = 0; s < ns; ++s) {
         amin[s] = sh[s].al[0];
         for (int p = 1; p < sh[s].npr; ++p) amin[s] = std::min(amin[s], sh[s].al[p]);
+        band[s] = (int)std::floor(std::log(amin[s]) / std::log(9.0) + 1e-9);
+        uint32_t code = 0;
+        for (int x = 0; x < 3; ++x) {
+            const double t = hi[x] > lo[x] ? (ce[sh[s].cen].r[x] - lo[x]) / (hi[x] - lo[x]) : 0.0;
+            const uint32_t g = (uint32_t)(int)(t * 1023.0);
+            for (int b = 0; b < 10; ++b) code |= ((g >> b) & 1u) << (3 * b + x);
+        }
+        zc[s] = code;
     }
     std::stable_sort(c.sorted_orig.begin(), c.sorted_orig.end(), [&](int a, int b) {
         if (sh[a].lsh != sh[b].lsh) return sh[a].lsh < sh[b].lsh;
-        if (amin[a] != amin[b]) return amin[a] > amin[b];
+        if (band[a] != band[b]) return band[a] > band[b];
+        if (zc[a] != zc[b]) return zc[a] < zc[b];
         if (sh[a].cen != sh[b].cen) return sh[a].cen < sh[b].cen;
+        if (amin[a] != amin[b]) return amin[a] > amin[b];
         return a < b;
     });
     c.orig_to_sorted.assign(ns, 0);

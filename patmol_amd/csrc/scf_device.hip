@@ -23,6 +23,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -300,12 +301,24 @@ __device__ __forceinline__ void block_sum2_256(double &a, double &b, double *sh)
 // (b / nt) TC2_T.., nt = ceil(n / TC2_T).  part[2b] = the tile's share of
 // tr X, part[2b + 1] of sum X_ij^2 (= tr X^2 for symmetric X).
 constexpr int TC2_T = 32;
+// Convergence state of a purification, in two slots of st that alternate
+// with the step index (a step reads slot TC2_REM + (step & 1) and writes the
+// other, so no block sees a value written by its own launch): -1 while not
+// converged; once tr X - tr X^2 < 1e-12 and |tr X - nocc| < 1e-8 the number
+// of updates still to do (2, 1), then 0, after which steps only copy X.
+constexpr int TC2_REM = 6;
+__device__ __forceinline__ double tc2_next_rem(double rem, double tr, double sq, double nocc)
+{
+    if (rem >= 0.0) return rem - 1.0;
+    return (tr - sq < 1e-12 && fabs(tr - nocc) < 1e-8) ? 2.0 : -1.0;
+}
 constexpr int TC2_KB = 14;   // k-steps of 4 whose operands a wave loads before their MFMAs
 typedef double tc2_d4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void tc2_partials_kernel(const double *__restrict__ X, int n,
-                                                          double *__restrict__ part)
+                                                          double *__restrict__ part, double *__restrict__ st)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st[TC2_REM] = -1.0;   // step 0's state: not converged
     __shared__ double sh[8];
     const int nt = (n + TC2_T - 1) / TC2_T;
     const int i0 = (blockIdx.x % nt) * TC2_T, j0 = (blockIdx.x / nt) * TC2_T;
@@ -350,14 +363,20 @@ __device__ __forceinline__ bool tc2_choice(const double *__restrict__ pin, int n
 // would grow as the square of the tile count).
 constexpr int TC2_FUSED_MAX = 1024;
 __global__ __launch_bounds__(64) void tc2_total_kernel(const double *__restrict__ pin, int nblk, double nocc,
-                                                       double *__restrict__ st)
+                                                       double *__restrict__ st, int sidx)
 {
+    const double rem = st[TC2_REM + (sidx & 1)];
+    if (rem == 0.0) {
+        if (threadIdx.x == 0) st[TC2_REM + ((sidx + 1) & 1)] = 0.0;
+        return;
+    }
     double tr, sq;
     const bool square = tc2_choice(pin, nblk, nocc, tr, sq);
     if (threadIdx.x == 0) {
         st[0] = tr;
         st[1] = sq;
         st[2] = square ? 1.0 : 0.0;
+        st[TC2_REM + ((sidx + 1) & 1)] = tc2_next_rem(rem, tr, sq, nocc);
     }
 }
 
@@ -377,8 +396,25 @@ __global__ __launch_bounds__(64) void tc2_total_kernel(const double *__restrict_
 // writes the new X to Xn (another buffer); tile partials go to pout.
 __global__ __launch_bounds__(256) void tc2_step_kernel(const double *__restrict__ X, double *__restrict__ Xn, int n,
                                                       const double *__restrict__ pin, double *__restrict__ pout,
-                                                      int nblk, double nocc, double *__restrict__ st, int presummed)
+                                                      int nblk, double nocc, double *__restrict__ st, int presummed,
+                                                      int sidx)
 {
+    const int nt0 = (n + TC2_T - 1) / TC2_T;
+    const double rem = st[TC2_REM + (sidx & 1)];
+    if (rem == 0.0) {
+        // converged and finished: the step copies X (tile and partials)
+        const int ti0 = (blockIdx.x % nt0) * TC2_T, tj0 = (blockIdx.x / nt0) * TC2_T;
+        for (int e = threadIdx.x; e < TC2_T * TC2_T; e += 256) {
+            const int i = ti0 + e % TC2_T, j = tj0 + e / TC2_T;
+            if (i < n && j < n) Xn[(size_t)j * n + i] = X[(size_t)j * n + i];
+        }
+        if (threadIdx.x == 0) {
+            pout[2 * blockIdx.x] = pin[2 * blockIdx.x];
+            pout[2 * blockIdx.x + 1] = pin[2 * blockIdx.x + 1];
+            if (!presummed && blockIdx.x == 0) st[TC2_REM + ((sidx + 1) & 1)] = 0.0;
+        }
+        return;
+    }
     __shared__ double sh[8];
     __shared__ tc2_d4 sacc[4][4][64];   // [wave][sub-tile][lane]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -433,6 +469,7 @@ __global__ __launch_bounds__(256) void tc2_step_kernel(const double *__restrict_
             st[0] = tr;
             st[1] = sq;
             st[2] = square ? 1.0 : 0.0;
+            st[TC2_REM + ((sidx + 1) & 1)] = tc2_next_rem(rem, tr, sq, nocc);
         }
     }
     __syncthreads();
@@ -605,28 +642,29 @@ struct Scf {
         // into the other buffer
         double *cur = T, *nxt = X2, *pin = part, *pout = part + 2 * nblk;
         hipLaunchKernelGGL(tc2_init_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, F, cur, n, tc + 4);
-        hipLaunchKernelGGL(tc2_partials_kernel, dim3(nblk), dim3(256), 0, s, cur, n, pin);
-        int extra = -1;
-        for (int it = 0; it < 120; ++it) {
-            const int presummed = nblk > TC2_FUSED_MAX || g_scf_density == 2;
-            if (presummed) hipLaunchKernelGGL(tc2_total_kernel, dim3(1), dim3(64), 0, s, pin, nblk, (double)nocc, tc);
-            hipLaunchKernelGGL(tc2_step_kernel, dim3(nblk), dim3(256), 0, s, cur, nxt, n, pin, pout, nblk,
-                               (double)nocc, tc, presummed);
-            std::swap(cur, nxt);
-            std::swap(pin, pout);
-            if (extra > 0) {
-                if (--extra == 0) {
-                    converged = true;
-                    break;
-                }
-                continue;
+        hipLaunchKernelGGL(tc2_partials_kernel, dim3(nblk), dim3(256), 0, s, cur, n, pin, tc);
+        // steps in batches (32, then 8 at a time, at most 120): the device
+        // tracks convergence (TC2_REM) and turns the steps after it into
+        // copies; the host reads the state once per batch
+        const int presummed = nblk > TC2_FUSED_MAX || g_scf_density == 2;
+        int sidx = 0;
+        while (sidx < 120) {
+            const int batch = std::min(sidx == 0 ? 32 : 8, 120 - sidx);
+            for (int b = 0; b < batch; ++b, ++sidx) {
+                if (presummed)
+                    hipLaunchKernelGGL(tc2_total_kernel, dim3(1), dim3(64), 0, s, pin, nblk, (double)nocc, tc, sidx);
+                hipLaunchKernelGGL(tc2_step_kernel, dim3(nblk), dim3(256), 0, s, cur, nxt, n, pin, pout, nblk,
+                                   (double)nocc, tc, presummed, sidx);
+                std::swap(cur, nxt);
+                std::swap(pin, pout);
             }
-            if (it % 4 == 3) {
-                double t[2];
-                if (hipMemcpyAsync(t, tc, sizeof(t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-                    hipStreamSynchronize(s) != hipSuccess)
-                    return fail("device failure in the purification");
-                if (t[0] - t[1] < 1e-12 && fabs(t[0] - nocc) < 1e-8) extra = 2;
+            double rem = -1.0;
+            if (hipMemcpyAsync(&rem, tc + TC2_REM + (sidx & 1), sizeof(rem), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return fail("device failure in the purification");
+            if (rem == 0.0) {
+                converged = true;
+                break;
             }
         }
         if (!converged) return true;

@@ -342,7 +342,9 @@ def full_to_packed(F: np.ndarray) -> np.ndarray:
 # exponent descending, index), tiles of
 # 8 (s, p) or 4 (d) shells, tile-pair bound = max Schwarz Q in the tile pair,
 # items = canonical tile-pair pairs with bound product >= tau, bra = higher
-# pair class, sorted by (bra class, ket class, bound) descending.
+# pair class (for two tile pairs of one class other than (dd|dd): the one
+# whose orientation executes fewer kernel steps), sorted by (bra class, ket
+# class, bound) descending.
 # ---------------------------------------------------------------------------
 def work_items_mirror(system, q_orig, tau, start=0):
     ls = [s.l for s in system.shells]
@@ -403,6 +405,39 @@ def work_items_mirror(system, q_orig, tau, start=0):
                     q = max(q, qs(I, J))
             la, lb = tl[ta], tl[tb]
             tps.append((ta, tb, la * (la + 1) // 2 + lb, q, max(tmaxorig[ta], tmaxorig[tb])))
+    def steps_of(br, kt):
+        # steps of the kernel for this orientation: one per ket shell K and
+        # 64 / (TI TJ) consecutive L, executed when any lane holds a quartet
+        # with Q_IJ Q_KL >= tau
+        TI, TJ, TL = tile_of(tl[br[0]]), tile_of(tl[br[1]]), tile_of(tl[kt[1]])
+        ksub = 64 // (TI * TJ)
+        fI, nI, fJ, nJ = tf[br[0]], tsz[br[0]], tf[br[1]], tsz[br[1]]
+        fK, nK, fL, nL = tf[kt[0]], tsz[kt[0]], tf[kt[1]], tsz[kt[1]]
+        steps = 0
+        for kq in range(nK):
+            for l0 in range(0, TL, ksub):
+                hit = False
+                for lq in range(l0, min(l0 + ksub, nL)):
+                    K, L = fK + kq, fL + lq
+                    if kt[0] == kt[1] and K < L:
+                        continue
+                    qk = qs(K, L)
+                    for i in range(nI):
+                        for j in range(nJ):
+                            I, J = fI + i, fJ + j
+                            if br[0] == br[1] and I < J:
+                                continue
+                            if qs(I, J) * qk >= tau:
+                                hit = True
+                                break
+                        if hit:
+                            break
+                    if hit:
+                        break
+                steps += 1 if hit else 0
+        return steps
+
+    dd = 2 * 3 // 2 + 2
     items = []
     for p1 in range(len(tps)):
         A = tps[p1]
@@ -414,6 +449,9 @@ def work_items_mirror(system, q_orig, tau, start=0):
             if start > 0 and max(A[4], B[4]) < start:
                 continue
             bra, ket = (A, B) if A[2] >= B[2] else (B, A)
+            # two tile pairs of one class (not (dd|dd)): fewer steps first
+            if A[2] == B[2] and p1 != p2 and A[2] != dd and steps_of(ket, bra) < steps_of(bra, ket):
+                bra, ket = ket, bra
             items.append(((bra[0], bra[1], ket[0], ket[1]), bra[2], ket[2], bound))
     items.sort(key=lambda x: (-x[1], -x[2], -x[3]))
     tiles = np.array([it[0] for it in items], dtype=np.int32).reshape(-1, 4)

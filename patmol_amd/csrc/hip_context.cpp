@@ -430,6 +430,36 @@ static int build_tables(Ctx &c, int start)
         int bpc, kpc;
         double bound;
     };
+    // Steps jk_kernel3 executes for one orientation of a tile quartet: one
+    // step per ket shell K and KSUB = 64 / (TI TJ) consecutive L, executed
+    // when any of its lanes holds a quartet with Q_IJ Q_KL >= tau.
+    auto qpair = [&](int I, int J) {
+        return I >= J ? c.pair_q[(int64_t)I * (I + 1) / 2 + J] : c.pair_q[(int64_t)J * (J + 1) / 2 + I];
+    };
+    auto steps_of = [&](const TP &br, const TP &kt) {
+        const int TI = tile_of(c.tile_l[br.a]), TJ = tile_of(c.tile_l[br.b]), TL = tile_of(c.tile_l[kt.b]);
+        const int ksub = 64 / (TI * TJ);
+        const int fI = c.tile_first[br.a], nI = c.tile_size[br.a], fJ = c.tile_first[br.b], nJ = c.tile_size[br.b];
+        const int fK = c.tile_first[kt.a], nK = c.tile_size[kt.a], fL = c.tile_first[kt.b], nL = c.tile_size[kt.b];
+        int steps = 0;
+        for (int kq = 0; kq < nK; ++kq)
+            for (int l0 = 0; l0 < TL; l0 += ksub) {
+                bool any = false;
+                for (int lq = l0; lq < std::min(l0 + ksub, nL) && !any; ++lq) {
+                    const int K = fK + kq, L = fL + lq;
+                    if (kt.a == kt.b && K < L) continue;
+                    const double qk = qpair(K, L);
+                    for (int i = 0; i < nI && !any; ++i)
+                        for (int j = 0; j < nJ && !any; ++j) {
+                            const int I = fI + i, J = fJ + j;
+                            if (br.a == br.b && I < J) continue;
+                            any = qpair(I, J) * qk >= c.tau;
+                        }
+                }
+                steps += any ? 1 : 0;
+            }
+        return steps;
+    };
     std::vector<Item> items;
     for (size_t p1 = 0; p1 < tps.size(); ++p1)
         for (size_t p2 = 0; p2 <= p1; ++p2) {
@@ -437,9 +467,13 @@ static int build_tables(Ctx &c, int start)
             double bound = A.q * B.q;
             if (bound < c.tau) continue;
             if (start > 0 && std::max(A.maxorig, B.maxorig) < start) continue;
-            const TP &bra = (A.pc >= B.pc) ? A : B;
-            const TP &ket = (A.pc >= B.pc) ? B : A;
-            items.push_back({make_int4(bra.a, bra.b, ket.a, ket.b), bra.pc, ket.pc, bound});
+            const TP *bra = (A.pc >= B.pc) ? &A : &B;
+            const TP *ket = (A.pc >= B.pc) ? &B : &A;
+            // two tile pairs of one class (other than (dd|dd), which runs
+            // the ket-pair-per-lane kernel): the orientation with fewer steps
+            if (A.pc == B.pc && p1 != p2 && A.pc != pair_class(2, 2) && steps_of(*ket, *bra) < steps_of(*bra, *ket))
+                std::swap(bra, ket);
+            items.push_back({make_int4(bra->a, bra->b, ket->a, ket->b), bra->pc, ket->pc, bound});
         }
     std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
         if (x.bpc != y.bpc) return x.bpc > y.bpc;

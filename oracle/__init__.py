@@ -342,9 +342,10 @@ def full_to_packed(F: np.ndarray) -> np.ndarray:
 # exponent descending, index), tiles of
 # 8 (s, p) or 4 (d) shells, tile-pair bound = max Schwarz Q in the tile pair,
 # items = canonical tile-pair pairs with bound product >= tau, bra = higher
-# pair class (for two tile pairs of one class other than (dd|dd): the one
-# whose orientation executes fewer kernel steps), sorted by (bra class, ket
-# class, bound) descending.
+# pair class, except that two tile pairs of one class (other than (dd|dd)) or
+# of the class pairs (ps|ss), (ds|ss), (ds|ps) take the orientation that
+# executes fewer kernel steps, sorted by (bra class, ket class, bound)
+# descending.
 # ---------------------------------------------------------------------------
 def work_items_mirror(system, q_orig, tau, start=0):
     ls = [s.l for s in system.shells]
@@ -438,6 +439,7 @@ def work_items_mirror(system, q_orig, tau, start=0):
         return steps
 
     dd = 2 * 3 // 2 + 2
+    ds = 2 * 3 // 2 + 0
     items = []
     for p1 in range(len(tps)):
         A = tps[p1]
@@ -449,8 +451,12 @@ def work_items_mirror(system, q_orig, tau, start=0):
             if start > 0 and max(A[4], B[4]) < start:
                 continue
             bra, ket = (A, B) if A[2] >= B[2] else (B, A)
-            # two tile pairs of one class (not (dd|dd)): fewer steps first
-            if A[2] == B[2] and p1 != p2 and A[2] != dd and steps_of(ket, bra) < steps_of(bra, ket):
+            # two tile pairs of one class (not (dd|dd)), or of the class
+            # pairs (ps|ss), (ds|ss), (ds|ps): the orientation with fewer steps
+            hi, lo = max(A[2], B[2]), min(A[2], B[2])
+            reversible = (hi == 1 and lo == 0) or (hi == ds and lo <= 1)
+            orientable = (A[2] == B[2] and p1 != p2 and A[2] != dd) or reversible
+            if orientable and steps_of(ket, bra) < steps_of(bra, ket):
                 bra, ket = ket, bra
             items.append(((bra[0], bra[1], ket[0], ket[1]), bra[2], ket[2], bound))
     items.sort(key=lambda x: (-x[1], -x[2], -x[3]))

@@ -1,5 +1,6 @@
 // eri_dispatch.cpp -- class switch over the per-class kernel objects
-// (generated list: every (bra pair class >= ket pair class) up to d shells).
+// (every (bra pair class >= ket pair class) up to d shells, and the
+// reversed classes (ss|ps), (ss|ds), (ps|ds) for the J/K build).
 #include "eri_kernels.h"
 
 namespace pm {
@@ -53,11 +54,21 @@ int cls_22_22_jk(const JKArgs &, int, int, hipStream_t);
 int cls_22_22_out(const EriOutArgs &, hipStream_t);
 int cls_22_22_schwarz(const SchwarzArgs &, hipStream_t);
 
+// J/K kernels of the reversed classes (bra pair class below the ket's)
+// (ss|ps), (ss|ds), (ps|ds): items of those class pairs run in the
+// orientation with fewer executed steps (hip_context.cpp builds the list)
+int cls_00_10_jk(const JKArgs &, int, int, hipStream_t);
+int cls_00_20_jk(const JKArgs &, int, int, hipStream_t);
+int cls_10_20_jk(const JKArgs &, int, int, hipStream_t);
+
 int launch_jk(int bra_pc, int ket_pc, const JKArgs &args, int prec, int nk, hipStream_t s)
 {
     switch (bra_pc * PM_NPC + ket_pc) {
     case 0: return cls_00_00_jk(args, prec, nk, s);
+    case 1: return cls_00_10_jk(args, prec, nk, s);
+    case 3: return cls_00_20_jk(args, prec, nk, s);
     case 6: return cls_10_00_jk(args, prec, nk, s);
+    case 9: return cls_10_20_jk(args, prec, nk, s);
     case 7: return cls_10_10_jk(args, prec, nk, s);
     case 12: return cls_11_00_jk(args, prec, nk, s);
     case 13: return cls_11_10_jk(args, prec, nk, s);

@@ -54,6 +54,9 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
 #endif
     switch (c) {
     case 0: return 1 * 2 + 1;    // (ss|ss)
+    case 3: return 2 * 2 + 1;    // (ss|ps), reversed (ps|ss)
+    case 6: return 2 * 2 + 0;    // (ss|ds), reversed (ds|ss)
+    case 33: return 2 * 2 + 1;   // (ps|ds), reversed (ds|ps)
     case 27: return 2 * 2 + 1;   // (ps|ss)
     case 30: return 2 * 2 + 0;   // (ps|ps)
     case 36: return 2 * 2 + 0;   // (pp|ss)

@@ -470,9 +470,15 @@ static int build_tables(Ctx &c, int start)
             const TP *bra = (A.pc >= B.pc) ? &A : &B;
             const TP *ket = (A.pc >= B.pc) ? &B : &A;
             // two tile pairs of one class (other than (dd|dd), which runs
-            // the ket-pair-per-lane kernel): the orientation with fewer steps
-            if (A.pc == B.pc && p1 != p2 && A.pc != pair_class(2, 2) && steps_of(*ket, *bra) < steps_of(*bra, *ket))
-                std::swap(bra, ket);
+            // the ket-pair-per-lane kernel), or of the class pairs whose
+            // reversed classes have kernels too ((ps|ss), (ds|ss), (ds|ps);
+            // the reversal measured slower for (pp|ss), (pp|ps), (ds|pp)):
+            // the orientation with fewer steps
+            const int hi = std::max(A.pc, B.pc), lo = std::min(A.pc, B.pc);
+            const bool reversible = (hi == pair_class(1, 0) && lo == pair_class(0, 0)) ||
+                                    (hi == pair_class(2, 0) && lo <= pair_class(1, 0));
+            const bool orientable = (A.pc == B.pc && p1 != p2 && A.pc != pair_class(2, 2)) || reversible;
+            if (orientable && steps_of(*ket, *bra) < steps_of(*bra, *ket)) std::swap(bra, ket);
             items.push_back({make_int4(bra->a, bra->b, ket->a, ket->b), bra->pc, ket->pc, bound});
         }
     std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {

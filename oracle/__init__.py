@@ -408,11 +408,11 @@ def work_items_mirror(system, q_orig, tau, start=0):
             tps.append((ta, tb, la * (la + 1) // 2 + lb, q, max(tmaxorig[ta], tmaxorig[tb])))
     def steps_of(br, kt):
         # steps of the kernel for this orientation: one per ket shell K and
-        # 64 / (TI TJ) consecutive L, executed when any lane holds a quartet
-        # with Q_IJ Q_KL >= tau
+        # 64 / (TI TJ) consecutive L, executed when some lane holds a quartet,
+        # i.e. when (the bra tile pair's largest bound) x Q_KL >= tau for one
+        # of its (K, L)
         TI, TJ, TL = tile_of(tl[br[0]]), tile_of(tl[br[1]]), tile_of(tl[kt[1]])
         ksub = 64 // (TI * TJ)
-        fI, nI, fJ, nJ = tf[br[0]], tsz[br[0]], tf[br[1]], tsz[br[1]]
         fK, nK, fL, nL = tf[kt[0]], tsz[kt[0]], tf[kt[1]], tsz[kt[1]]
         steps = 0
         for kq in range(nK):
@@ -422,18 +422,8 @@ def work_items_mirror(system, q_orig, tau, start=0):
                     K, L = fK + kq, fL + lq
                     if kt[0] == kt[1] and K < L:
                         continue
-                    qk = qs(K, L)
-                    for i in range(nI):
-                        for j in range(nJ):
-                            I, J = fI + i, fJ + j
-                            if br[0] == br[1] and I < J:
-                                continue
-                            if qs(I, J) * qk >= tau:
-                                hit = True
-                                break
-                        if hit:
-                            break
-                    if hit:
+                    if br[3] * qs(K, L) >= tau:
+                        hit = True
                         break
                 steps += 1 if hit else 0
         return steps

@@ -436,10 +436,12 @@ static int build_tables(Ctx &c, int start)
     auto qpair = [&](int I, int J) {
         return I >= J ? c.pair_q[(int64_t)I * (I + 1) / 2 + J] : c.pair_q[(int64_t)J * (J + 1) / 2 + I];
     };
+    // Some lane of a step holds a quartet iff (largest bra bound) x Q_KL >=
+    // tau for one of its (K, L): rounding is monotonic, so the largest
+    // product is the product with the largest bra bound (tile pair q).
     auto steps_of = [&](const TP &br, const TP &kt) {
         const int TI = tile_of(c.tile_l[br.a]), TJ = tile_of(c.tile_l[br.b]), TL = tile_of(c.tile_l[kt.b]);
         const int ksub = 64 / (TI * TJ);
-        const int fI = c.tile_first[br.a], nI = c.tile_size[br.a], fJ = c.tile_first[br.b], nJ = c.tile_size[br.b];
         const int fK = c.tile_first[kt.a], nK = c.tile_size[kt.a], fL = c.tile_first[kt.b], nL = c.tile_size[kt.b];
         int steps = 0;
         for (int kq = 0; kq < nK; ++kq)
@@ -448,13 +450,7 @@ static int build_tables(Ctx &c, int start)
                 for (int lq = l0; lq < std::min(l0 + ksub, nL) && !any; ++lq) {
                     const int K = fK + kq, L = fL + lq;
                     if (kt.a == kt.b && K < L) continue;
-                    const double qk = qpair(K, L);
-                    for (int i = 0; i < nI && !any; ++i)
-                        for (int j = 0; j < nJ && !any; ++j) {
-                            const int I = fI + i, J = fJ + j;
-                            if (br.a == br.b && I < J) continue;
-                            any = qpair(I, J) * qk >= c.tau;
-                        }
+                    any = br.q * qpair(K, L) >= c.tau;
                 }
                 steps += any ? 1 : 0;
             }

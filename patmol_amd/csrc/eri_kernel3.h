@@ -255,16 +255,25 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             sKq[t] = a.pair_q[pk];
             sKok[t] = ok ? 1 : 0;
         }
-        // exclusive prefix sum and maximum of the primitive counts
-        int incl = r.npp, mx = r.npp;
+        // exclusive prefix sum and maximum of the primitive counts; with one
+        // primitive pair per ket pair (uncontracted shells) they are known
+        int off, tot, mx;
+        if (__all(!here || r.npp == 1)) {
+            off = t;
+            tot = KLP;
+            mx = 1;
+        } else {
+            int incl = r.npp;
+            mx = r.npp;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int u = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += u;
-            mx = max(mx, __shfl_xor(mx, d, 64));
+            for (int d = 1; d < 64; d <<= 1) {
+                const int u = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += u;
+                mx = max(mx, __shfl_xor(mx, d, 64));
+            }
+            tot = __shfl(incl, 63, 64);
+            off = incl - r.npp;
         }
-        const int tot = __shfl(incl, 63, 64);
-        const int off = incl - r.npp;
         const bool staged_w = tot <= S::PPCAP;
         if (here) {
             sPPoff[t] = off;

@@ -7,7 +7,10 @@
   on 16 threads), element by element;
 * linearity in the density, G(P1 + 2 P2) = G(P1) + 2 G(P2);
 * the symmetry of the two-electron form, tr(P1 G(P2)) = tr(P2 G(P1)), which
-  a wrong index or component order in any one class kernel would break.
+  a wrong index or component order in any one class kernel would break;
+* the mixed-precision build (work items whose bound x max|P| is under the
+  threshold run the FP32 kernels, FP64 accumulation) against the FP64 build,
+  at the benchmark's threshold 1e-6 and at 1e-3 (most items in FP32).
 
 Tolerances (measured: 5e-14, 5e-15, 2e-16): 1e-11 absolute on G elements
 against the oracle (|G| up to ~3, sums of up to 1e7 integrals in another
@@ -78,3 +81,23 @@ def test_water8_gmatrix_linear_and_symmetric(water8_store):
     note(f"(H2O)8 linearity max|dG| {lin:.2e}; tr(P1 G(P2)) {t12:.12f} vs tr(P2 G(P1)) {t21:.12f} (rel {rel:.1e})")
     assert lin < 1e-12
     assert rel < 1e-13
+
+
+@pytest.mark.parametrize("thr,tol", [(1e-6, 1e-9), (1e-3, 1e-5)])
+def test_water8_mixed_precision_vs_fp64(water8_store, thr, tol):
+    """Tolerances: an FP32 item's integrals carry ~1e-7 relative error on
+    contributions bounded by thr x |P| each (measured: 9.6e-11 with 23,873
+    FP32 items at 1e-6, 1.4e-7 with 86,880 at 1e-3)."""
+    s, st = water8_store
+    P = _density(s.norb, 14)
+    G64 = st.rhf_gmatrix(P)
+    pm.set_options(precision=1, mixed_threshold=thr)
+    try:
+        Gmx = st.rhf_gmatrix(P)
+        n32 = st.stats()["work_items_fp32"]
+    finally:
+        pm.set_options(precision=0)
+    d = float(np.abs(Gmx - G64).max())
+    note(f"(H2O)8 mixed precision, threshold {thr:g}: {n32} FP32 work items, max|G - G_fp64| {d:.2e}")
+    assert n32 > 0
+    assert d < tol

@@ -338,6 +338,59 @@ __global__ __launch_bounds__(256) void tc2_partials_kernel(const double *__restr
     }
 }
 
+// Spectral bounds for a purification from the last F' of the channel (spin)
+// whose eigenvalues are known (the guess, or a diagonalised iteration): by
+// Weyl's inequality every eigenvalue moves by at most ||F - Fref||_2 <=
+// ||F - Fref||_F, so [lo - d, hi + d] bounds the spectrum of F when [lo, hi]
+// bounded that of Fref (d = the Frobenius norm of the change, with a relative
+// margin for rounding).  The interval is intersected with F's Gershgorin
+// interval, which alone is about ten times too wide for an orthogonalised
+// Fock matrix (at (H2O)8: Gershgorin width 5700 Eh, spectrum 540 Eh, the
+// Weyl interval about 1400 Eh).
+__global__ __launch_bounds__(256) void diff2_part_kernel(const double *__restrict__ A, const double *__restrict__ B,
+                                                        int64_t nn, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int64_t chunk = (nn + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < nn ? lo + chunk : nn;
+    double a = 0.0;
+    for (int64_t k = lo + threadIdx.x; k < hi; k += 256) {
+        const double d = A[k] - B[k];
+        a += d * d;
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// gb[0..1]: F's Gershgorin interval in, the bounds used for F out; ref[0..1]:
+// the bounds of Fref
+__global__ void tc2_bounds_kernel(const double *__restrict__ part, int nb, const double *__restrict__ ref,
+                                  double *__restrict__ gb)
+{
+    if (threadIdx.x != 0) return;
+    double s2 = 0.0;
+    for (int b = 0; b < nb; ++b) s2 += part[b];
+    const double w = fabs(ref[0]) + fabs(ref[1]);
+    const double m = sqrt(s2) * (1.0 + 1e-12) + 1e-12 * w;
+    gb[0] = fmax(gb[0], ref[0] - m);
+    gb[1] = fmin(gb[1], ref[1] + m);
+}
+
+// bounds from the eigenvalues of F (ascending), widened by 1e-10 of their
+// magnitude for the eigensolver's error
+__global__ void tc2_exact_bounds_kernel(const double *__restrict__ ev, int n, double *__restrict__ prev)
+{
+    if (threadIdx.x != 0) return;
+    const double w = fabs(ev[0]) + fabs(ev[n - 1]);
+    prev[0] = ev[0] - 1e-10 * w;
+    prev[1] = ev[n - 1] + 1e-10 * w;
+}
+
 // tr X and tr X^2 from the tile partials: lane-strided sums, then xor
 // butterflies (every lane, and every block that does this, gets the same
 // bits); the TC2 choice X^2 when |tr X^2 - nocc| < |2 tr X - tr X^2 - nocc|.
@@ -524,6 +577,10 @@ struct Scf {
     double *X2 = nullptr, *glo = nullptr, *ghi = nullptr, *tc = nullptr;
     double *part = nullptr, *spart = nullptr;   // tile partials of the purification, block partials of the sums
     int nblk = 0;                               // purification tiles
+    // per channel (RHF: 0; UHF: 0 alpha, 1 beta): the last diagonalised F'
+    // and its spectral bounds bnd[2 ch], bnd[2 ch + 1]
+    double *Fprev[2] = {nullptr, nullptr}, *bnd = nullptr;
+    bool have_prev[2] = {false, false};
     bool purify = false;   // density by purification inside the iteration loop
     int purify_fallbacks = 0;
     rocblas_int *info = nullptr;
@@ -563,7 +620,11 @@ struct Scf {
         nblk = nt * nt;
         part = mem.get<double>(4 * (size_t)nblk);
         spart = mem.get<double>(3 * SUMS_BLOCKS);
-        if (!X || !F || !T || !C || !ev || !off || !sums || !info || !X2 || !glo || !ghi || !tc || !part || !spart)
+        Fprev[0] = mem.get<double>(nn);
+        Fprev[1] = mem.get<double>(nn);
+        bnd = mem.get<double>(4);
+        if (!X || !F || !T || !C || !ev || !off || !sums || !info || !X2 || !glo || !ghi || !tc || !part || !spart ||
+            !Fprev[0] || !Fprev[1] || !bnd)
             return fail("device allocation failed");
         return true;
     }
@@ -630,7 +691,7 @@ struct Scf {
     // eigenvalues) is below 1e-12, then two more steps.  Returns false with
     // `converged` unset when 120 steps do not get there (a degenerate or
     // vanishing gap at the Fermi level); the caller then diagonalises.
-    bool purify_density(int nocc, double *P, bool &converged)
+    bool purify_density(int nocc, double *P, bool &converged, int ch)
     {
         const int64_t nn = (int64_t)n * n;
         const double one = 1.0, zero = 0.0;
@@ -638,6 +699,10 @@ struct Scf {
         if (nocc <= 0 || nocc >= n) return true;   // the caller diagonalises these
         hipLaunchKernelGGL(gersh_row_kernel, dim3(n), dim3(256), 0, s, F, n, glo, ghi);
         hipLaunchKernelGGL(gersh_reduce_kernel, dim3(1), dim3(256), 0, s, glo, ghi, n, tc + 4);
+        if (have_prev[ch]) {
+            hipLaunchKernelGGL(diff2_part_kernel, dim3(SUMS_BLOCKS), dim3(256), 0, s, F, Fprev[ch], nn, spart);
+            hipLaunchKernelGGL(tc2_bounds_kernel, dim3(1), dim3(64), 0, s, spart, SUMS_BLOCKS, bnd + 2 * ch, tc + 4);
+        }
         // X0 into cur with its tile partials; each step writes the next X
         // into the other buffer
         double *cur = T, *nxt = X2, *pin = part, *pout = part + 2 * nblk;
@@ -680,15 +745,15 @@ struct Scf {
 
     // the density of (a + b): purification inside the loop when enabled,
     // otherwise (and as its fallback) the eigensolver path
-    bool density(const double *a, const double *b, int nocc, double *P)
+    bool density(const double *a, const double *b, int nocc, double *P, int ch)
     {
-        if (!purify || nocc <= 0 || nocc >= n) return diag_density(a, b, nocc, P);
+        if (!purify || nocc <= 0 || nocc >= n) return diag_density(a, b, nocc, P, ch);
         if (!fprime(a, b)) return false;
         bool conv = false;
-        if (!purify_density(nocc, P, conv)) return false;
+        if (!purify_density(nocc, P, conv, ch)) return false;
         if (conv) return true;
         ++purify_fallbacks;
-        return diag_density(a, b, nocc, P);
+        return diag_density(a, b, nocc, P, ch);
     }
 
     // orbitals of (a + b): eigenvalues into ev, C = X V
@@ -702,12 +767,19 @@ struct Scf {
         return true;
     }
 
-    bool diag_density(const double *a, const double *b, int nocc, double *P)
+    bool diag_density(const double *a, const double *b, int nocc, double *P, int ch)
     {
         const int64_t nn = (int64_t)n * n;
         const double one = 1.0, zero = 0.0;
         if (!fprime(a, b)) return false;
+        // F' (overwritten by the eigenvectors) and its exact spectral bounds
+        // become the channel's reference for the bounds of later
+        // purifications
+        if (hipMemcpyAsync(Fprev[ch], F, sizeof(double) * nn, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return fail("copy failed");
         if (!eigen()) return false;
+        hipLaunchKernelGGL(tc2_exact_bounds_kernel, dim3(1), dim3(64), 0, s, ev, n, bnd + 2 * ch);
+        have_prev[ch] = true;
         // C = X V, D = C_occ C_occ^T (lower), P = pack(D)
         if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, F, n, &zero, C, n) !=
             rocblas_status_success)
@@ -822,7 +894,7 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     if (Pguess) {
         if (hipMemcpyAsync(P, Pguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
             return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: upload failed");
-    } else if (!c.diag_density(dH, nullptr, nocc, P)) {
+    } else if (!c.diag_density(dH, nullptr, nocc, P, 0)) {
         return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
     }
     double eps = scf_eps;
@@ -848,7 +920,7 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         P2 = P1;
         P1 = P;
         P = fresh;
-        if (!c.density(dH, dG, nocc, P)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        if (!c.density(dH, dG, nocc, P, 0)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
         double sm[3];
         if (!c.sums3(P, P1, dH, dG, sm)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
         const float g_ms = c.elapsed_ms();
@@ -938,7 +1010,7 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         if (hipMemcpyAsync(PA, PAguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
             hipMemcpyAsync(PB, PBguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
             return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: upload failed");
-    } else if (!c.diag_density(dH, nullptr, nocca, PA)) {
+    } else if (!c.diag_density(dH, nullptr, nocca, PA, 0)) {
         return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
     } else if (noccb > 0) {
         const double one = 1.0, zero = 0.0;
@@ -976,9 +1048,9 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         PB1 = PB;
         PB = fb;
         double sa[3], sb[3];
-        if (!c.density(dH, GA, nocca, PA) ||
+        if (!c.density(dH, GA, nocca, PA, 0) ||
             hipMemcpyAsync(evA, c.ev, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s) != hipSuccess ||
-            !c.sums3(PA, PA1, dH, GA, sa) || !c.density(dH, GB, noccb, PB) || !c.sums3(PB, PB1, dH, GB, sb))
+            !c.sums3(PA, PA1, dH, GA, sa) || !c.density(dH, GB, noccb, PB, 1) || !c.sums3(PB, PB1, dH, GB, sb))
             return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
         gms += c.elapsed_ms();
         pdiff = sqrt(2.0 * sa[2]) / n + sqrt(2.0 * sb[2]) / n;
@@ -1052,7 +1124,7 @@ extern "C" int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int 
     if (hipMemcpyAsync(dS, Smat, sizeof(double) * c.no2, hipMemcpyHostToDevice, c.s) != hipSuccess ||
         hipMemcpyAsync(dF, Fmat, sizeof(double) * c.no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
         return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock: upload failed");
-    if (!c.form_x(dS) || !c.diag_density(dF, nullptr, nocc, dP)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock");
+    if (!c.form_x(dS) || !c.diag_density(dF, nullptr, nocc, dP, 0)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_diag_fock");
     bool ok = true;
     if (evals) ok = ok && hipMemcpy(evals, c.ev, sizeof(double) * n, hipMemcpyDeviceToHost) == hipSuccess;
     if (Cmat) ok = ok && download_rowmajor(c, c.C, Cmat);

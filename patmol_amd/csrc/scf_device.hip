@@ -342,8 +342,9 @@ __global__ __launch_bounds__(256) void tc2_partials_kernel(const double *__restr
 // whose eigenvalues are known (the guess, or a diagonalised iteration): by
 // Weyl's inequality every eigenvalue moves by at most ||F - Fref||_2 <=
 // ||F - Fref||_F, so [lo - d, hi + d] bounds the spectrum of F when [lo, hi]
-// bounded that of Fref (d = the Frobenius norm of the change, with a relative
-// margin for rounding).  The interval is intersected with F's Gershgorin
+// bounded that of Fref (d = the smaller of the Frobenius norm and the largest
+// absolute column sum of the change, both bounds of its 2-norm, with a
+// relative margin for rounding).  The interval is intersected with F's Gershgorin
 // interval, which alone is about ten times too wide for an orthogonalised
 // Fock matrix (at (H2O)8: Gershgorin width 5700 Eh, spectrum 540 Eh, the
 // Weyl interval about 1400 Eh).
@@ -367,16 +368,38 @@ __global__ __launch_bounds__(256) void diff2_part_kernel(const double *__restric
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// gb[0..1]: F's Gershgorin interval in, the bounds used for F out; ref[0..1]:
-// the bounds of Fref
-__global__ void tc2_bounds_kernel(const double *__restrict__ part, int nb, const double *__restrict__ ref,
-                                  double *__restrict__ gb)
+// absolute column sums of A - B (column-major n x n), one block per column
+__global__ __launch_bounds__(256) void diffcol_kernel(const double *__restrict__ A, const double *__restrict__ B, int n,
+                                                     double *__restrict__ col)
 {
+    __shared__ double red[256];
+    const size_t c0 = (size_t)blockIdx.x * n;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) a += fabs(A[c0 + i] - B[c0 + i]);
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) col[blockIdx.x] = red[0];
+}
+
+// gb[0..1]: F's Gershgorin interval in, the bounds used for F out; ref[0..1]:
+// the bounds of Fref; part: Frobenius partials, col: column sums of F - Fref
+// (one 64-thread wave)
+__global__ __launch_bounds__(64) void tc2_bounds_kernel(const double *__restrict__ part, int nb,
+                                                        const double *__restrict__ col, int n,
+                                                        const double *__restrict__ ref, double *__restrict__ gb)
+{
+    double cmax = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) cmax = fmax(cmax, col[i]);
+    for (int d = 32; d > 0; d >>= 1) cmax = fmax(cmax, __shfl_xor(cmax, d, 64));
     if (threadIdx.x != 0) return;
     double s2 = 0.0;
     for (int b = 0; b < nb; ++b) s2 += part[b];
     const double w = fabs(ref[0]) + fabs(ref[1]);
-    const double m = sqrt(s2) * (1.0 + 1e-12) + 1e-12 * w;
+    const double m = fmin(sqrt(s2), cmax) * (1.0 + 1e-12) + 1e-12 * w;
     gb[0] = fmax(gb[0], ref[0] - m);
     gb[1] = fmin(gb[1], ref[1] + m);
 }
@@ -701,7 +724,9 @@ struct Scf {
         hipLaunchKernelGGL(gersh_reduce_kernel, dim3(1), dim3(256), 0, s, glo, ghi, n, tc + 4);
         if (have_prev[ch]) {
             hipLaunchKernelGGL(diff2_part_kernel, dim3(SUMS_BLOCKS), dim3(256), 0, s, F, Fprev[ch], nn, spart);
-            hipLaunchKernelGGL(tc2_bounds_kernel, dim3(1), dim3(64), 0, s, spart, SUMS_BLOCKS, bnd + 2 * ch, tc + 4);
+            hipLaunchKernelGGL(diffcol_kernel, dim3(n), dim3(256), 0, s, F, Fprev[ch], n, glo);
+            hipLaunchKernelGGL(tc2_bounds_kernel, dim3(1), dim3(64), 0, s, spart, SUMS_BLOCKS, glo, n, bnd + 2 * ch,
+                               tc + 4);
         }
         // X0 into cur with its tile partials; each step writes the next X
         // into the other buffer

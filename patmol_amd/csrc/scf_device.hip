@@ -604,6 +604,7 @@ struct Scf {
     // and its spectral bounds bnd[2 ch], bnd[2 ch + 1]
     double *Fprev[2] = {nullptr, nullptr}, *bnd = nullptr;
     bool have_prev[2] = {false, false};
+    int last_steps[2] = {0, 0};   // steps the channel's last purification launched
     bool purify = false;   // density by purification inside the iteration loop
     int purify_fallbacks = 0;
     rocblas_int *info = nullptr;
@@ -733,13 +734,15 @@ struct Scf {
         double *cur = T, *nxt = X2, *pin = part, *pout = part + 2 * nblk;
         hipLaunchKernelGGL(tc2_init_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, F, cur, n, tc + 4);
         hipLaunchKernelGGL(tc2_partials_kernel, dim3(nblk), dim3(256), 0, s, cur, n, pin, tc);
-        // steps in batches (32, then 8 at a time, at most 120): the device
-        // tracks convergence (TC2_REM) and turns the steps after it into
-        // copies; the host reads the state once per batch
+        // steps in batches (the first as long as the channel's last
+        // purification less 4, or 32; then 4 at a time; at most 120): the
+        // device tracks convergence (TC2_REM) and turns the steps after it
+        // into copies; the host reads the state once per batch
         const int presummed = nblk > TC2_FUSED_MAX || g_scf_density == 2;
+        const int first = last_steps[ch] > 0 ? std::max(8, last_steps[ch] - 4) : 32;
         int sidx = 0;
         while (sidx < 120) {
-            const int batch = std::min(sidx == 0 ? 32 : 8, 120 - sidx);
+            const int batch = std::min(sidx == 0 ? first : 4, 120 - sidx);
             for (int b = 0; b < batch; ++b, ++sidx) {
                 if (presummed)
                     hipLaunchKernelGGL(tc2_total_kernel, dim3(1), dim3(64), 0, s, pin, nblk, (double)nocc, tc, sidx);
@@ -754,6 +757,7 @@ struct Scf {
                 return fail("device failure in the purification");
             if (rem == 0.0) {
                 converged = true;
+                last_steps[ch] = sidx;
                 break;
             }
         }

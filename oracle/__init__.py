@@ -92,9 +92,6 @@ def lib():
         L.or_check_phase.argtypes = [C.c_int, _dp]
         L.or_diag_fock.restype = C.c_int
         L.or_diag_fock.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
-        L.or_quartet_tasks.restype = C.c_int64
-        L.or_quartet_tasks.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
-                                       C.c_double, C.c_int, _ip, _ip, _ip, _ip, C.c_int64]
         _lib = L
     return _lib
 
@@ -139,17 +136,6 @@ def diag_fock(S: np.ndarray, F: np.ndarray, nocc: int):
                             nocc, ev, Cm.reshape(-1), P)
     assert rc == 0, "overlap matrix has an eigenvalue <= DBL_EPSILON"
     return ev, Cm, P
-
-
-def quartet_tasks(qbra, qket, same, tau, tau32=0.0, chunk=64):
-    qb = np.ascontiguousarray(qbra, dtype=np.float64)
-    qk = np.ascontiguousarray(qket, dtype=np.float64)
-    L = lib()
-    z = np.zeros(1, np.int32)
-    n = L.or_quartet_tasks(len(qb), qb, len(qk), qk, int(same), tau, tau32, chunk, z, z, z, z, 0)
-    tb, t0, t1, tp = (np.zeros(max(n, 1), np.int32) for _ in range(4))
-    L.or_quartet_tasks(len(qb), qb, len(qk), qk, int(same), tau, tau32, chunk, tb, t0, t1, tp, n)
-    return tb[:n], t0[:n], t1[:n], tp[:n]
 
 
 class Basis:

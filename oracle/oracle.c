@@ -31,8 +31,6 @@
  *                                                         incl. a dominant negative first component
  *   or_diag_fock        rhfscf.c:432-471, scfaux.c:5-41   reference spack_trans/spack_diag/
  *                                                         form_cmatrix/form_pmatrix (_ref)
- *   or_quartet_tasks    (no reference counterpart: the reference has no Schwarz list, K4)
- *                                                         integer brute force over all pairs
  */
 #include "oracle.h"
 #include <math.h>
@@ -1763,52 +1761,4 @@ int or_uhf_scf(const OBasis *b, int nelec, int mult, int maxits, double eps, int
     free(PA); free(PA1); free(PA2); free(PB); free(PB1); free(PB2);
     free(X); free(C); free(ev);
     return 0;
-}
-
-/* ------------------------------------------------------------------ */
-/* Screened quartet task list (CPU mirror).  Bra and ket pair lists are  */
-/* sorted by Schwarz bound, descending.  For bra b the surviving kets    */
-/* form the prefix [0, n(b)) with qbra[b]*qket[k] >= tau; same-list      */
-/* (same class) keeps k <= b.  Kets with qbra*qket < tau32 are FP32      */
-/* (tprec 1), the rest FP64 (tprec 0); tasks never straddle that split  */
-/* and hold at most `chunk` kets.  Returns the task count (writes at    */
-/* most cap).                                                           */
-/* ------------------------------------------------------------------ */
-static int prefix_count(const double *q, int n, double qb, double thr)
-{
-    int lo = 0, hi = n; /* first index with qb*q[k] < thr */
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (qb * q[mid] >= thr)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-int64_t or_quartet_tasks(int nbra, const double *qbra, int nket, const double *qket, int same,
-                         double tau, double tau32, int chunk, int *tb, int *tk0, int *tk1, int *tprec,
-                         int64_t cap)
-{
-    int64_t nt = 0;
-    for (int b = 0; b < nbra; ++b) {
-        int n = prefix_count(qket, nket, qbra[b], tau);
-        if (same && n > b + 1) n = b + 1;
-        int n64 = tau32 > 0.0 ? prefix_count(qket, nket, qbra[b], tau32) : n;
-        if (n64 > n) n64 = n;
-        for (int seg = 0; seg < 2; ++seg) {
-            int lo = seg ? n64 : 0, hi = seg ? n : n64;
-            for (int k0 = lo; k0 < hi; k0 += chunk) {
-                if (nt < cap) {
-                    tb[nt] = b;
-                    tk0[nt] = k0;
-                    tk1[nt] = (k0 + chunk < hi) ? k0 + chunk : hi;
-                    tprec[nt] = seg;
-                }
-                ++nt;
-            }
-        }
-    }
-    return nt;
 }

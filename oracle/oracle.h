@@ -110,11 +110,6 @@ int or_eigh(int n, const double *A, double *evals, double *evecs);
 void or_check_phase(int n, double *z);
 int or_diag_fock(int n, const double *S, const double *F, int nocc, double *ev, double *C, double *P);
 
-/* ---- screened quartet list mirror ---- */
-int64_t or_quartet_tasks(int nbra, const double *qbra, int nket, const double *qket,
-                         int same, double tau, double tau32, int chunk, int *tb, int *tk0,
-                         int *tk1, int *tprec, int64_t cap);
-
 #ifdef __cplusplus
 }
 #endif

@@ -8,9 +8,9 @@
 
 namespace pm {
 
-int PM_NAME(PM_LA, PM_LB, PM_LC, PM_LD, jk)(const JKArgs &args, int prec, int nk, hipStream_t s)
+int PM_NAME(PM_LA, PM_LB, PM_LC, PM_LD, jk)(const JKArgs &args, int prec, hipStream_t s)
 {
-    launch_jk_t<PM_LA, PM_LB, PM_LC, PM_LD>(args, prec, nk, s);
+    launch_jk_t<PM_LA, PM_LB, PM_LC, PM_LD>(args, prec, s);
     return 0;
 }
 

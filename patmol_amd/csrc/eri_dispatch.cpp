@@ -5,89 +5,89 @@
 
 namespace pm {
 
-int cls_00_00_jk(const JKArgs &, int, int, hipStream_t);
+int cls_00_00_jk(const JKArgs &, int, hipStream_t);
 int cls_00_00_out(const EriOutArgs &, hipStream_t);
 int cls_00_00_schwarz(const SchwarzArgs &, hipStream_t);
-int cls_10_00_jk(const JKArgs &, int, int, hipStream_t);
+int cls_10_00_jk(const JKArgs &, int, hipStream_t);
 int cls_10_00_out(const EriOutArgs &, hipStream_t);
-int cls_10_10_jk(const JKArgs &, int, int, hipStream_t);
+int cls_10_10_jk(const JKArgs &, int, hipStream_t);
 int cls_10_10_out(const EriOutArgs &, hipStream_t);
 int cls_10_10_schwarz(const SchwarzArgs &, hipStream_t);
-int cls_11_00_jk(const JKArgs &, int, int, hipStream_t);
+int cls_11_00_jk(const JKArgs &, int, hipStream_t);
 int cls_11_00_out(const EriOutArgs &, hipStream_t);
-int cls_11_10_jk(const JKArgs &, int, int, hipStream_t);
+int cls_11_10_jk(const JKArgs &, int, hipStream_t);
 int cls_11_10_out(const EriOutArgs &, hipStream_t);
-int cls_11_11_jk(const JKArgs &, int, int, hipStream_t);
+int cls_11_11_jk(const JKArgs &, int, hipStream_t);
 int cls_11_11_out(const EriOutArgs &, hipStream_t);
 int cls_11_11_schwarz(const SchwarzArgs &, hipStream_t);
-int cls_20_00_jk(const JKArgs &, int, int, hipStream_t);
+int cls_20_00_jk(const JKArgs &, int, hipStream_t);
 int cls_20_00_out(const EriOutArgs &, hipStream_t);
-int cls_20_10_jk(const JKArgs &, int, int, hipStream_t);
+int cls_20_10_jk(const JKArgs &, int, hipStream_t);
 int cls_20_10_out(const EriOutArgs &, hipStream_t);
-int cls_20_11_jk(const JKArgs &, int, int, hipStream_t);
+int cls_20_11_jk(const JKArgs &, int, hipStream_t);
 int cls_20_11_out(const EriOutArgs &, hipStream_t);
-int cls_20_20_jk(const JKArgs &, int, int, hipStream_t);
+int cls_20_20_jk(const JKArgs &, int, hipStream_t);
 int cls_20_20_out(const EriOutArgs &, hipStream_t);
 int cls_20_20_schwarz(const SchwarzArgs &, hipStream_t);
-int cls_21_00_jk(const JKArgs &, int, int, hipStream_t);
+int cls_21_00_jk(const JKArgs &, int, hipStream_t);
 int cls_21_00_out(const EriOutArgs &, hipStream_t);
-int cls_21_10_jk(const JKArgs &, int, int, hipStream_t);
+int cls_21_10_jk(const JKArgs &, int, hipStream_t);
 int cls_21_10_out(const EriOutArgs &, hipStream_t);
-int cls_21_11_jk(const JKArgs &, int, int, hipStream_t);
+int cls_21_11_jk(const JKArgs &, int, hipStream_t);
 int cls_21_11_out(const EriOutArgs &, hipStream_t);
-int cls_21_20_jk(const JKArgs &, int, int, hipStream_t);
+int cls_21_20_jk(const JKArgs &, int, hipStream_t);
 int cls_21_20_out(const EriOutArgs &, hipStream_t);
-int cls_21_21_jk(const JKArgs &, int, int, hipStream_t);
+int cls_21_21_jk(const JKArgs &, int, hipStream_t);
 int cls_21_21_out(const EriOutArgs &, hipStream_t);
 int cls_21_21_schwarz(const SchwarzArgs &, hipStream_t);
-int cls_22_00_jk(const JKArgs &, int, int, hipStream_t);
+int cls_22_00_jk(const JKArgs &, int, hipStream_t);
 int cls_22_00_out(const EriOutArgs &, hipStream_t);
-int cls_22_10_jk(const JKArgs &, int, int, hipStream_t);
+int cls_22_10_jk(const JKArgs &, int, hipStream_t);
 int cls_22_10_out(const EriOutArgs &, hipStream_t);
-int cls_22_11_jk(const JKArgs &, int, int, hipStream_t);
+int cls_22_11_jk(const JKArgs &, int, hipStream_t);
 int cls_22_11_out(const EriOutArgs &, hipStream_t);
-int cls_22_20_jk(const JKArgs &, int, int, hipStream_t);
+int cls_22_20_jk(const JKArgs &, int, hipStream_t);
 int cls_22_20_out(const EriOutArgs &, hipStream_t);
-int cls_22_21_jk(const JKArgs &, int, int, hipStream_t);
+int cls_22_21_jk(const JKArgs &, int, hipStream_t);
 int cls_22_21_out(const EriOutArgs &, hipStream_t);
-int cls_22_22_jk(const JKArgs &, int, int, hipStream_t);
+int cls_22_22_jk(const JKArgs &, int, hipStream_t);
 int cls_22_22_out(const EriOutArgs &, hipStream_t);
 int cls_22_22_schwarz(const SchwarzArgs &, hipStream_t);
 
 // J/K kernels of the reversed classes (bra pair class below the ket's)
 // (ss|ps), (ss|ds), (ps|ds): items of those class pairs run in the
 // orientation with fewer executed steps (hip_context.cpp builds the list)
-int cls_00_10_jk(const JKArgs &, int, int, hipStream_t);
-int cls_00_20_jk(const JKArgs &, int, int, hipStream_t);
-int cls_10_20_jk(const JKArgs &, int, int, hipStream_t);
+int cls_00_10_jk(const JKArgs &, int, hipStream_t);
+int cls_00_20_jk(const JKArgs &, int, hipStream_t);
+int cls_10_20_jk(const JKArgs &, int, hipStream_t);
 
-int launch_jk(int bra_pc, int ket_pc, const JKArgs &args, int prec, int nk, hipStream_t s)
+int launch_jk(int bra_pc, int ket_pc, const JKArgs &args, int prec, hipStream_t s)
 {
     switch (bra_pc * PM_NPC + ket_pc) {
-    case 0: return cls_00_00_jk(args, prec, nk, s);
-    case 1: return cls_00_10_jk(args, prec, nk, s);
-    case 3: return cls_00_20_jk(args, prec, nk, s);
-    case 6: return cls_10_00_jk(args, prec, nk, s);
-    case 9: return cls_10_20_jk(args, prec, nk, s);
-    case 7: return cls_10_10_jk(args, prec, nk, s);
-    case 12: return cls_11_00_jk(args, prec, nk, s);
-    case 13: return cls_11_10_jk(args, prec, nk, s);
-    case 14: return cls_11_11_jk(args, prec, nk, s);
-    case 18: return cls_20_00_jk(args, prec, nk, s);
-    case 19: return cls_20_10_jk(args, prec, nk, s);
-    case 20: return cls_20_11_jk(args, prec, nk, s);
-    case 21: return cls_20_20_jk(args, prec, nk, s);
-    case 24: return cls_21_00_jk(args, prec, nk, s);
-    case 25: return cls_21_10_jk(args, prec, nk, s);
-    case 26: return cls_21_11_jk(args, prec, nk, s);
-    case 27: return cls_21_20_jk(args, prec, nk, s);
-    case 28: return cls_21_21_jk(args, prec, nk, s);
-    case 30: return cls_22_00_jk(args, prec, nk, s);
-    case 31: return cls_22_10_jk(args, prec, nk, s);
-    case 32: return cls_22_11_jk(args, prec, nk, s);
-    case 33: return cls_22_20_jk(args, prec, nk, s);
-    case 34: return cls_22_21_jk(args, prec, nk, s);
-    case 35: return cls_22_22_jk(args, prec, nk, s);
+    case 0: return cls_00_00_jk(args, prec, s);
+    case 1: return cls_00_10_jk(args, prec, s);
+    case 3: return cls_00_20_jk(args, prec, s);
+    case 6: return cls_10_00_jk(args, prec, s);
+    case 9: return cls_10_20_jk(args, prec, s);
+    case 7: return cls_10_10_jk(args, prec, s);
+    case 12: return cls_11_00_jk(args, prec, s);
+    case 13: return cls_11_10_jk(args, prec, s);
+    case 14: return cls_11_11_jk(args, prec, s);
+    case 18: return cls_20_00_jk(args, prec, s);
+    case 19: return cls_20_10_jk(args, prec, s);
+    case 20: return cls_20_11_jk(args, prec, s);
+    case 21: return cls_20_20_jk(args, prec, s);
+    case 24: return cls_21_00_jk(args, prec, s);
+    case 25: return cls_21_10_jk(args, prec, s);
+    case 26: return cls_21_11_jk(args, prec, s);
+    case 27: return cls_21_20_jk(args, prec, s);
+    case 28: return cls_21_21_jk(args, prec, s);
+    case 30: return cls_22_00_jk(args, prec, s);
+    case 31: return cls_22_10_jk(args, prec, s);
+    case 32: return cls_22_11_jk(args, prec, s);
+    case 33: return cls_22_20_jk(args, prec, s);
+    case 34: return cls_22_21_jk(args, prec, s);
+    case 35: return cls_22_22_jk(args, prec, s);
     default: return -1;
     }
 }

@@ -1,18 +1,21 @@
 // eri_kernel3.h -- J/K kernel with the tile quartet staged in LDS.
 //
-// Same work decomposition and lane layout as jk_kernel2 (one workgroup per
-// tile quartet; lane = (ket sub-slot, bra shell I, bra shell J), bra pair
-// fixed per lane; wave w owns ket shells K = w, w+4, ...; each wave steps
-// over L in chunks of KSUB), but everything a step reads or writes lives in
-// LDS for the duration of the item:
+// One workgroup per tile quartet; lane = (ket sub-slot, bra shell I, bra
+// shell J), the bra pair fixed per lane for the whole item; wave w owns ket
+// shells K = w, w+4, ...; each wave steps over L in chunks of KSUB.
+// Everything a step reads or writes lives in LDS for the duration of the
+// item:
 //   * the six density blocks D(IJ), D(KL), D(IK), D(IL), D(JK), D(JL) of the
 //     tile quartet (gathered once, coalesced),
 //   * the ket pair table (pair reference, Schwarz bound, validity, centres)
 //     and, when they fit, the ket primitive pairs,
-//   * the six output blocks J'(IJ), J'(KL), K'(IK), K'(IL), K'(JK), K'(JL):
-//     cross-lane reductions land there (plain adds where one wave owns the
-//     element, LDS atomics where waves share it), and one pass of global
-//     atomics at the end of the item moves them to the per-XCD J'/K' copies.
+//   * the six output blocks J'(IJ), J'(KL), K'(IK), K'(IL), K'(JK), K'(JL)
+//     (FP64): cross-lane reductions land there, every wave that can write an
+//     element writing its own copy of the block; at the end of the item the
+//     copies are summed in copy order, converted to fixed point and moved by
+//     one pass of global integer atomics into the per-XCD J'/K' copies.
+//     Neither the order in which waves run nor the order of the atomics
+//     changes the sums, so a G build is reproducible bit for bit.
 // Classes whose 2-D tables for all roots would not fit the register file
 // digest one Rys root at a time (primitive_quartet_rootwise); that choice, the
 // density-read fence and the register budget per class are measured
@@ -36,22 +39,16 @@ struct K3 {
 // Per-class launch shape, from timing every class on (H2O)8 with both
 // digestion orders, with and without the density-read fence, at 1 and 2
 // waves per SIMD, with and without splitting the bra components over the
-// waves: code = rys_lds * 128 + ksplit * 64 + (split_b - 1) * 32 +
-// (split_a - 1) * 16 + fence * 8 + waves * 2 + rootwise (rys_lds: the Rys
-// coefficients of the class's root count are staged in LDS also above three
-// roots; ksplit: every item is two workgroups, each taking half of the ket
+// waves: code = rys_global * 256 + rys_lds * 128 + ksplit * 64 +
+// (split_b - 1) * 32 + (split_a - 1) * 16 + fence * 8 + waves * 2 + rootwise
+// (rys_lds: the Rys coefficients of the class's root count are staged in LDS
+// also above three roots; rys_global: they are read from global memory even
+// at three roots or fewer, which keeps two workgroups' LDS within a CU's
+// 160 KB; ksplit: every item is two workgroups, each taking half of the ket
 // tile's K shells).
 constexpr int jk3_shape(int la, int lb, int lc, int ld)
 {
     const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
-#if defined(PM_SHAPE_CLS) && defined(PM_SHAPE_VAL)
-    if (c == PM_SHAPE_CLS) return PM_SHAPE_VAL;   // timing experiments
-#endif
-#ifdef PM_SHAPE_TABLE
-    constexpr int ovr[][2] = {PM_SHAPE_TABLE};   // timing experiments: {class, code}, ...
-    for (const auto &o : ovr)
-        if (o[0] == c) return o[1];
-#endif
     switch (c) {
     case 0: return 1 * 2 + 1;    // (ss|ss)
     case 3: return 2 * 2 + 1;    // (ss|ps), reversed (ps|ss)
@@ -64,7 +61,7 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 40: return 1 * 2 + 1;   // (pp|pp)
     case 54: return 2 * 2 + 0;   // (ds|ss)
     case 57: return 2 * 2 + 1;   // (ds|ps)
-    case 58: return 16 + 2 * 2 + 1;   // (ds|pp)
+    case 58: return 256 + 16 + 2 * 2 + 1;   // (ds|pp)
     case 60: return 1 * 2 + 0;   // (ds|ds)
     case 63: return 2 * 2 + 1;   // (dp|ss)
     case 66: return 1 * 2 + 0;   // (dp|ps)
@@ -76,11 +73,7 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
     case 76: return 128 + 64 + 32 + 16 + 1 * 2 + 1;   // (dd|pp)
     case 78: return 128 + 32 + 1 * 2 + 1;   // (dd|ds)
     case 79: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dp)
-#ifdef PM_SHAPE_DDDD
-    case 80: return PM_SHAPE_DDDD;             // (dd|dd)
-#else
-    case 80: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dd)
-#endif
+    case 80: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dd) (runs jk_kernel)
     default: return 1 * 2 + 1;
     }
 }
@@ -88,23 +81,18 @@ constexpr int jk3_shape(int la, int lb, int lc, int ld)
 template <int LA, int LB, int LC, int LD>
 constexpr bool jk3_rootwise()
 {
-#ifdef PM_K3_ROOTWISE
-    return PM_K3_ROOTWISE;
-#else
     return jk3_shape(LA, LB, LC, LD) & 1;
-#endif
 }
 
 // The Rys coefficient section of the class's root count is kept in LDS for
-// the classes with up to three roots (7 / 17 / 30 KB) and for the classes
-// whose shape asks for it (4 roots: 47 KB)
-#ifndef PM_RYS_LDS_MAXN
-#define PM_RYS_LDS_MAXN 3
-#endif
+// the classes with up to three roots (7 / 17 / 30 KB) unless their shape
+// says otherwise, and for the classes whose shape asks for it (4 roots:
+// 47 KB)
 template <int LA, int LB, int LC, int LD>
 constexpr bool jk3_rys_lds()
 {
-    return (LA + LB + LC + LD) / 2 + 1 <= PM_RYS_LDS_MAXN || ((jk3_shape(LA, LB, LC, LD) >> 7) & 1);
+    constexpr int code = jk3_shape(LA, LB, LC, LD);
+    return ((LA + LB + LC + LD) / 2 + 1 <= 3 && !((code >> 8) & 1)) || ((code >> 7) & 1);
 }
 
 // Workgroups per work item (jk3_shape's ksplit): the items of the largest
@@ -119,55 +107,45 @@ constexpr int jk3_wg_per_item()
 template <int LA, int LB, int LC, int LD>
 constexpr bool jk3_block_fence()
 {
-#ifdef PM_K3_FENCE
-    return PM_K3_FENCE;
-#else
     return (jk3_shape(LA, LB, LC, LD) >> 3) & 1;
-#endif
 }
 
 template <int LA, int LB, int LC, int LD>
 constexpr int jk3_split_a()
 {
-#ifdef PM_K3_SPLIT_A
-    return (ncart(LA) % PM_K3_SPLIT_A == 0) ? PM_K3_SPLIT_A : 1;
-#else
     return ((jk3_shape(LA, LB, LC, LD) >> 4) & 1) + 1;
-#endif
 }
 
 template <int LA, int LB, int LC, int LD>
 constexpr int jk3_split_b()
 {
-#ifdef PM_K3_SPLIT_B
-    return (ncart(LB) % PM_K3_SPLIT_B == 0) ? PM_K3_SPLIT_B : 1;
-#else
     return ((jk3_shape(LA, LB, LC, LD) >> 5) & 1) + 1;
-#endif
 }
 
 template <int LA, int LB, int LC, int LD>
 constexpr int jk3_waves_per_eu()
 {
-#ifdef PM_JK_WAVES
-    return PM_JK_WAVES;
-#else
     return (jk3_shape(LA, LB, LC, LD) >> 1) & 3;
-#endif
 }
 
 // Cross-lane sums: lane bits 0-3 by DPP inside rows of 16; bits 4-5 (other
 // rows) are left to LDS atomics from one leader per row, which costs less
 // than the permlane swaps for the few values a step produces.
-constexpr int red_lo(int mask)
-{
-#ifdef PM_K3_FULLREDUCE
-    return mask;
-#else
-    return mask & 15;
-#endif
-}
+constexpr int red_lo(int mask) { return mask & 15; }
 constexpr bool red_hi(int mask) { return (mask & ~red_lo(mask)) != 0; }
+
+// v into an LDS output element: with ATOMIC lanes of this wave's instruction
+// share the element (one LDS atomic instruction), otherwise this lane is its
+// only writer
+template <bool ATOMIC>
+__device__ __forceinline__ void lds_acc(double *o, double v)
+{
+    if constexpr (ATOMIC) {
+        if (v != 0.0) __hip_atomic_fetch_add(o, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        *o += v;
+    }
+}
 
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(jk3_waves_per_eu<LA, LB, LC, LD>()))) void jk_kernel3(JKArgs a)
@@ -184,11 +162,31 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     constexpr int NLSTEP = (TL + KSUB - 1) / KSUB;
     static_assert(IJN * KSUB == 64, "bra tile pair must divide a wavefront");
 
+    // Bra-component split: SA x SB waves share every K, each taking a block
+    // of NAH bra-a components x NBH bra-b components of every quartet (and
+    // only that block's accumulators); without a split the four waves own
+    // different K.  Waves w with the same w / SPLIT (the K group) share K.
+    constexpr int SA = jk3_split_a<LA, LB, LC, LD>(), SB = jk3_split_b<LA, LB, LC, LD>();
+    constexpr int SPLIT = SA * SB, KSTRIDE = 4 / SPLIT;
+    constexpr int NAH = NA / SA, NBH = NB / SB;
+    static_assert(NA % SA == 0 && NB % SB == 0 && 4 % SPLIT == 0, "bra component split");
+
     // block rows padded to an odd number of doubles so that rows of
     // different bra shells fall into different LDS banks
     constexpr int SJ = FJ | 1, SK = FK | 1, SL = FL | 1;
-    __shared__ double sD[FI * SJ + FK * SL + FI * SK + FI * SL + FJ * SK + FJ * SL];
-    __shared__ double sO[FI * SJ + FK * SL + FI * SK + FI * SL + FJ * SK + FJ * SL];
+    constexpr int ZIJ = FI * SJ, ZKL = FK * SL, ZIK = FI * SK, ZIL = FI * SL, ZJK = FJ * SK, ZJL = FJ * SL;
+    __shared__ double sD[ZIJ + ZKL + ZIK + ZIL + ZJK + ZJL];
+    // Output blocks.  An element that several waves can write gets one copy
+    // per writing wave: J'(KL) one per wave of a K group (SPLIT), K'(IK) per
+    // b block (SB), K'(JK) per a block (SA), K'(IL) per (K group, b block),
+    // K'(JL) per (K group, a block).  J'(IJ) has one copy; the K groups add
+    // into it one after the other at the end of the item.  Lanes of one wave
+    // that share an element add with one LDS atomic instruction, whose lanes
+    // the LDS applies in a fixed order.
+    constexpr int CKL = SPLIT, CIK = SB, CJK = SA, CIL = KSTRIDE * SB, CJL = KSTRIDE * SA;
+    constexpr int pIJ = 0, pKL = pIJ + ZIJ, pIK = pKL + CKL * ZKL, pIL = pIK + CIK * ZIK, pJK = pIL + CIL * ZIL,
+                  pJL = pJK + CJK * ZJK, pEND = pJL + CJL * ZJL;
+    __shared__ double sO[pEND];
     __shared__ int sOff[TI + TJ + TK + TL];
     __shared__ double sXK[TK][3], sXL[TL][3];
     __shared__ PairRef sKref[KLP];
@@ -204,9 +202,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     // only the NR-root section of the table (rys_eval<NR, true>)
     constexpr int RYS_LDS_N = RYS_LDS ? rys_tmax(S::NR) * rys_stride(S::NR) : 2;
     __shared__ __attribute__((aligned(16))) double sRys[RYS_LDS_N];
-    // block offsets inside sD / sO
-    constexpr int oIJ = 0, oKL = oIJ + FI * SJ, oIK = oKL + FK * SL, oIL = oIK + FI * SK, oJK = oIL + FI * SL,
-                  oJL = oJK + FJ * SK, oEND = oJL + FJ * SL;
+    // density block offsets inside sD
+    constexpr int dIJ = 0, dKL = dIJ + ZIJ, dIK = dKL + ZKL, dIL = dIK + ZIK, dJK = dIL + ZIL, dJL = dJK + ZJK;
 
     // with KS workgroups per item, workgroups KS i .. KS i + KS - 1 share
     // item i and each takes one range of the K shells of the ket tile
@@ -306,15 +303,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 double v = 0.0;
                 if (ex) v = src[(size_t)(sOff[rsh + rs] + r % nrc) * N + sOff[csh + cs] + c % ncc];
                 sD[o + e] = v;
-                sO[o + e] = 0.0;
             }
         };
-        gather(oIJ, FI, SJ, 0, nI, NA, TI, nJ, NB, a.DJ);
-        gather(oKL, FK, SL, TI + TJ, nK, NC, TI + TJ + TK, nL, ND, a.DJ);
-        gather(oIK, FI, SK, 0, nI, NA, TI + TJ, nK, NC, a.DK0);
-        gather(oIL, FI, SL, 0, nI, NA, TI + TJ + TK, nL, ND, a.DK0);
-        gather(oJK, FJ, SK, TI, nJ, NB, TI + TJ, nK, NC, a.DK0);
-        gather(oJL, FJ, SL, TI, nJ, NB, TI + TJ + TK, nL, ND, a.DK0);
+        for (int e = tid; e < pEND; e += 256) sO[e] = 0.0;
+        gather(dIJ, FI, SJ, 0, nI, NA, TI, nJ, NB, a.DJ);
+        gather(dKL, FK, SL, TI + TJ, nK, NC, TI + TJ + TK, nL, ND, a.DJ);
+        gather(dIK, FI, SK, 0, nI, NA, TI + TJ, nK, NC, a.DK);
+        gather(dIL, FI, SL, 0, nI, NA, TI + TJ + TK, nL, ND, a.DK);
+        gather(dJK, FJ, SK, TI, nJ, NB, TI + TJ, nK, NC, a.DK);
+        gather(dJL, FJ, SL, TI, nJ, NB, TI + TJ + TK, nL, ND, a.DK);
     }
     __syncthreads();
     const bool staged = sStaged != 0;
@@ -342,22 +339,20 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     // bra primitive pairs: contraction 1 is the common case, keep it in registers
     const PrimPair bp0 = a.pp[bref.pp0];
 
-    // Bra-component split: SA x SB waves share every K, each taking a block
-    // of NAH bra-a components x NBH bra-b components of every quartet (and
-    // only that block's accumulators); without a split the four waves own
-    // different K.
-    constexpr int SA = jk3_split_a<LA, LB, LC, LD>(), SB = jk3_split_b<LA, LB, LC, LD>();
-    constexpr int SPLIT = SA * SB, KSTRIDE = 4 / SPLIT;
-    constexpr int NAH = NA / SA, NBH = NB / SB;
-    static_assert(NA % SA == 0 && NB % SB == 0 && 4 % SPLIT == 0, "bra component split");
-    const int sub = wave % SPLIT;
+    const int sub = wave % SPLIT, kgrp = wave / SPLIT;
     const int ha = sub % SA, hb = sub / SA;   // wave-uniform
     const int a0 = ha * NAH, b0 = hb * NBH;
+    // this wave's copies of the shared output blocks
+    double *const oKL = sO + pKL + sub * ZKL;
+    double *const oIK = sO + pIK + hb * ZIK;
+    double *const oJK = sO + pJK + ha * ZJK;
+    double *const oIL = sO + pIL + (kgrp * SB + hb) * ZIL;
+    double *const oJL = sO + pJL + (kgrp * SA + ha) * ZJL;
 
     double jab[NAH * NBH];
 #pragma unroll
     for (int i = 0; i < NAH * NBH; ++i) jab[i] = 0.0;
-    const double *__restrict__ DIJ = sD + oIJ + (ii * NA) * SJ + jj * NB;
+    const double *__restrict__ DIJ = sD + dIJ + (ii * NA) * SJ + jj * NB;
 
     constexpr int KPER = (TK + KS - 1) / KS;
     const int kq_end = min(TK, (ksub + 1) * KPER);
@@ -405,11 +400,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             for (int i = 0; i < NAH * ND; ++i) kad[i] = 0.0;
 #pragma unroll
             for (int i = 0; i < NBH * ND; ++i) kbd[i] = 0.0;
-            const double *__restrict__ DKL = sD + oKL + (kq * NC) * SL + lq * ND;
-            const double *__restrict__ DJL = sD + oJL + (jj * NB) * SL + lq * ND;
-            const double *__restrict__ DIL = sD + oIL + (ii * NA) * SL + lq * ND;
-            const double *__restrict__ DJK = sD + oJK + (jj * NB) * SK + kq * NC;
-            const double *__restrict__ DIK = sD + oIK + (ii * NA) * SK + kq * NC;
+            const double *__restrict__ DKL = sD + dKL + (kq * NC) * SL + lq * ND;
+            const double *__restrict__ DJL = sD + dJL + (jj * NB) * SL + lq * ND;
+            const double *__restrict__ DIL = sD + dIL + (ii * NA) * SL + lq * ND;
+            const double *__restrict__ DJK = sD + dJK + (jj * NB) * SK + kq * NC;
+            const double *__restrict__ DIK = sD + dIK + (ii * NA) * SK + kq * NC;
             const int nkp = sKmax;  // uniform bound of the ket primitive loop
             const PrimPair *kpp = staged ? (sPP + sPPoff[t]) : (a.pp + kref.pp0);
             // the primitive loops for one (a, b) component block
@@ -481,11 +476,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 for (int c = 0; c < NC; ++c)
 #pragma unroll
                     for (int d = 0; d < ND; ++d) {
-                        double *o = &sO[oKL + (kq * NC + c) * SL + lq * ND + d];
-                        if constexpr (SPLIT == 1 && !red_hi(MASK_I | MASK_J)) *o += 2.0 * jcd[c * ND + d];
-                        else if (jcd[c * ND + d] != 0.0)
-                            __hip_atomic_fetch_add(o, 2.0 * jcd[c * ND + d], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        lds_acc<red_hi(MASK_I | MASK_J)>(&oKL[(kq * NC + c) * SL + lq * ND + d], 2.0 * jcd[c * ND + d]);
                     }
             }
             if ((lane & red_lo(MASK_J)) == 0 && iexist && lexist) {
@@ -493,18 +484,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 for (int x = 0; x < NAH; ++x)
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
-                        if (kad[x * ND + d] != 0.0)
-                            __hip_atomic_fetch_add(&sO[oIL + (ii * NA + a0 + x) * SL + lq * ND + d], kad[x * ND + d],
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        lds_acc<red_hi(MASK_J)>(&oIL[(ii * NA + a0 + x) * SL + lq * ND + d], kad[x * ND + d]);
             }
             if ((lane & red_lo(MASK_I)) == 0 && jexist && lexist) {
 #pragma unroll
                 for (int b = 0; b < NBH; ++b)
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
-                        if (kbd[b * ND + d] != 0.0)
-                            __hip_atomic_fetch_add(&sO[oJL + (jj * NB + b0 + b) * SL + lq * ND + d], kbd[b * ND + d],
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        lds_acc<red_hi(MASK_I)>(&oJL[(jj * NB + b0 + b) * SL + lq * ND + d], kbd[b * ND + d]);
             }
         }
         // per-K results: K'(IK), K'(JK) (owned by the waves of this K)
@@ -515,10 +502,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             for (int x = 0; x < NAH; ++x)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    double *o = &sO[oIK + (ii * NA + a0 + x) * SK + kq * NC + c];
-                    if constexpr (SB == 1 && !red_hi(MASK_J | MASK_S)) *o += kac[x * NC + c];
-                    else if (kac[x * NC + c] != 0.0)
-                        __hip_atomic_fetch_add(o, kac[x * NC + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    lds_acc<red_hi(MASK_J | MASK_S)>(&oIK[(ii * NA + a0 + x) * SK + kq * NC + c], kac[x * NC + c]);
                 }
         }
         if ((lane & red_lo(MASK_I | MASK_S)) == 0 && jexist) {
@@ -526,45 +510,45 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             for (int b = 0; b < NBH; ++b)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    double *o = &sO[oJK + (jj * NB + b0 + b) * SK + kq * NC + c];
-                    if constexpr (SA == 1 && !red_hi(MASK_I | MASK_S)) *o += kbc[b * NC + c];
-                    else if (kbc[b * NC + c] != 0.0)
-                        __hip_atomic_fetch_add(o, kbc[b * NC + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    lds_acc<red_hi(MASK_I | MASK_S)>(&oJK[(jj * NB + b0 + b) * SK + kq * NC + c], kbc[b * NC + c]);
                 }
         }
     }
-    // per-item result: J'(IJ) (shared by the waves of different K)
+    // per-item result: J'(IJ), added by one K group after the other (the
+    // waves of one K group hold disjoint (a, b) blocks)
     lane_reduce_n<red_lo(MASK_S)>(jab);
-    if ((lane & red_lo(MASK_S)) == 0 && iexist && jexist) {
+    for (int g = 0; g < KSTRIDE; ++g) {
+        if (kgrp == g && (lane & red_lo(MASK_S)) == 0 && iexist && jexist) {
 #pragma unroll
-        for (int x = 0; x < NAH; ++x)
+            for (int x = 0; x < NAH; ++x)
 #pragma unroll
-            for (int b = 0; b < NBH; ++b)
-                if (jab[x * NBH + b] != 0.0)
-                    __hip_atomic_fetch_add(&sO[oIJ + (ii * NA + a0 + x) * SJ + jj * NB + b0 + b], 2.0 * jab[x * NBH + b],
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (int b = 0; b < NBH; ++b)
+                    lds_acc<red_hi(MASK_S)>(&sO[pIJ + (ii * NA + a0 + x) * SJ + jj * NB + b0 + b], 2.0 * jab[x * NBH + b]);
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
-    // ---- flush the output blocks to this XCD's J'/K' copy ----
+    // ---- flush: copies summed in copy order, fixed point, this XCD's J'/K' ----
     const size_t xoff = (size_t)xcd_slot() * N * N;
-    double *const Jx = a.J + xoff;
-    double *const Kx = a.K0 + xoff;
-    auto flush = [&](int o, int rows, int stride, int cols, int rsh, int nrc, int csh, int ncc, double *dst) {
+    FxAcc *const Jx = a.J + xoff;
+    FxAcc *const Kx = a.K + xoff;
+    auto flush = [&](int o, int copies, int size, int rows, int stride, int cols, int rsh, int nrc, int csh, int ncc,
+                     FxAcc *dst) {
         for (int e = tid; e < rows * stride; e += 256) {
             const int r = e / stride, c = e % stride;
             if (c >= cols) continue;
-            const double v = sO[o + e];
+            double v = sO[o + e];
+            for (int k = 1; k < copies; ++k) v += sO[o + k * size + e];
             if (v == 0.0) continue;
-            atomic_add(&dst[(size_t)(sOff[rsh + r / nrc] + r % nrc) * N + sOff[csh + c / ncc] + c % ncc], v);
+            fx_flush(&dst[(size_t)(sOff[rsh + r / nrc] + r % nrc) * N + sOff[csh + c / ncc] + c % ncc], fx_split(v));
         }
     };
-    flush(oIJ, FI, SJ, FJ, 0, NA, TI, NB, Jx);
-    flush(oKL, FK, SL, FL, TI + TJ, NC, TI + TJ + TK, ND, Jx);
-    flush(oIK, FI, SK, FK, 0, NA, TI + TJ, NC, Kx);
-    flush(oIL, FI, SL, FL, 0, NA, TI + TJ + TK, ND, Kx);
-    flush(oJK, FJ, SK, FK, TI, NB, TI + TJ, NC, Kx);
-    flush(oJL, FJ, SL, FL, TI, NB, TI + TJ + TK, ND, Kx);
+    flush(pIJ, 1, ZIJ, FI, SJ, FJ, 0, NA, TI, NB, Jx);
+    flush(pKL, CKL, ZKL, FK, SL, FL, TI + TJ, NC, TI + TJ + TK, ND, Jx);
+    flush(pIK, CIK, ZIK, FI, SK, FK, 0, NA, TI + TJ, NC, Kx);
+    flush(pIL, CIL, ZIL, FI, SL, FL, 0, NA, TI + TJ + TK, ND, Kx);
+    flush(pJK, CJK, ZJK, FJ, SK, FK, TI, NB, TI + TJ, NC, Kx);
+    flush(pJL, CJL, ZJL, FJ, SL, FL, TI, NB, TI + TJ + TK, ND, Kx);
 }
 
 }  // namespace pm

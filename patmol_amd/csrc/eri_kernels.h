@@ -14,6 +14,51 @@ constexpr int PM_XCD_SLOTS = 8;
 // TK*TL must divide 64).
 __host__ __device__ constexpr int tile_of(int l) { return l <= 1 ? 8 : 4; }
 
+// A J'/K' accumulator element in fixed point: the value is
+// hi * 2^-FX_HI_BITS + lo * 2^-FX_LO_BITS.  Every contribution is rounded
+// once onto that grid (2^-65 absolute) and then added as integers, which are
+// associative: the sums do not depend on the order in which lanes, waves
+// and workgroups add them, so a G build is bitwise reproducible.
+constexpr int FX_HI_BITS = 24;
+constexpr int FX_LO_BITS = 64;
+struct __attribute__((aligned(16))) FxAcc {
+    long long hi, lo;
+};
+
+// Fixed-point words of v (FxAcc): hi = v 2^24 rounded to the nearest
+// integer, lo = the remainder in units of 2^-64, rounded to nearest.  The
+// roundings use the 1.5 2^52 bias: for |x| < 2^51, x + 1.5 2^52 is rounded
+// to an integer by the FP64 add itself and that integer sits in the low
+// mantissa bits, so the difference of the bit patterns is round(x) as an
+// int64.  Every step is exact in FP64 (power-of-two scalings, x - round(x)),
+// so the split is exact up to the final rounding of lo; valid for
+// |v| < 2^27.
+constexpr double FX_HI_SCALE = 16777216.0;           // 2^FX_HI_BITS
+constexpr double FX_LO_SCALE = 1099511627776.0;      // 2^(FX_LO_BITS - FX_HI_BITS)
+constexpr double FX_BIAS = 6755399441055744.0;       // 1.5 2^52
+static_assert(FX_LO_BITS - FX_HI_BITS == 40, "FX_LO_SCALE");
+__host__ __device__ inline long long fx_bits(double x)
+{
+    union {
+        double d;
+        long long i;
+    } u = {x};
+    return u.i;
+}
+__host__ __device__ inline FxAcc fx_split(double v)
+{
+    const double t = v * FX_HI_SCALE;
+    const double bt = t + FX_BIAS;              // round(t) in the low mantissa bits
+    const double h = bt - FX_BIAS;              // round(t) as a double
+    const double bl = (t - h) * FX_LO_SCALE + FX_BIAS;   // |(t - h) 2^40| <= 2^39
+    return FxAcc{fx_bits(bt) - fx_bits(FX_BIAS), fx_bits(bl) - fx_bits(FX_BIAS)};
+}
+
+__host__ __device__ inline double fx_value(long long hi, long long lo)
+{
+    return (double)hi * (1.0 / FX_HI_SCALE) + (double)lo * (1.0 / (FX_HI_SCALE * FX_LO_SCALE));
+}
+
 struct JKArgs {
     const int4 *items;         // (tI, tJ, tK, tL): bra tile pair first
     int nitems;
@@ -26,8 +71,8 @@ struct JKArgs {
     const double *pair_q;      // Schwarz bound of each pair
     const PrimPair *pp;
     const double *DJ;          // density contracted into J (full N x N)
-    const double *DK0, *DK1;   // densities contracted into K
-    double *J, *K0, *K1;       // accumulators J', K': PM_XCD_SLOTS copies of N x N each
+    const double *DK;          // density contracted into K
+    FxAcc *J, *K;              // accumulators J', K': PM_XCD_SLOTS copies of N x N each
     int N;
     double tau;                // Schwarz threshold on Q_IJ Q_KL
     int start_shell;           // keep quartets whose largest original shell index >= start_shell
@@ -55,7 +100,9 @@ struct SchwarzArgs {
     RysTables rys;
 };
 
-int launch_jk(int bra_pc, int ket_pc, const JKArgs &args, int prec, int nk, hipStream_t s);
+// prec 0: FP64 instantiation, 1: FP32 (FP64 accumulation).  Returns nonzero
+// for a class without a kernel.
+int launch_jk(int bra_pc, int ket_pc, const JKArgs &args, int prec, hipStream_t s);
 int launch_eri_out(int bra_pc, int ket_pc, const EriOutArgs &args, hipStream_t s);
 int launch_schwarz(int pc, const SchwarzArgs &args, hipStream_t s);
 

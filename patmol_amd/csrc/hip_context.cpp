@@ -205,7 +205,8 @@ struct Ctx {
     DevBuf<PrimPair> d_pp;
     DevBuf<int4> d_items;
     // work buffers
-    DevBuf<double> d_P, d_PB, d_D, d_D2, d_DJ, d_J, d_K0, d_K1, d_G, d_G2, d_dmax;
+    DevBuf<double> d_P, d_PB, d_D, d_D2, d_DJ, d_G, d_G2, d_dmax;
+    DevBuf<FxAcc> d_J, d_K;   // PM_XCD_SLOTS fixed-point copies of J', K' (N x N each)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> cls_ev;
     // side streams for concurrent class launches (default priority)
@@ -515,9 +516,10 @@ __global__ void add_kernel(const double *__restrict__ a, const double *__restric
 }
 
 // G[ij] += cj (J'ij + J'ji) - (K'ij + K'ji), packed i >= j, with J' and K'
-// the sums of the per-XCD copies, held in sorted function order (inv maps a
-// function to its sorted index)
-__global__ void finalize_kernel(const double *__restrict__ J, const double *__restrict__ K, double *__restrict__ G,
+// the sums of the per-XCD fixed-point copies (integer sums, then one
+// conversion each), held in sorted function order (inv maps a function to
+// its sorted index)
+__global__ void finalize_kernel(const FxAcc *__restrict__ J, const FxAcc *__restrict__ K, double *__restrict__ G,
                                 const int *__restrict__ inv, int N, double cj)
 {
     int64_t ij = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -529,12 +531,23 @@ __global__ void finalize_kernel(const double *__restrict__ J, const double *__re
     int j = (int)(ij - (int64_t)i * (i + 1) / 2);
     const int si = inv[i], sj = inv[j];
     const int64_t NN = (int64_t)N * N, a = (int64_t)si * N + sj, b = (int64_t)sj * N + si;
-    double jj = 0.0, kk = 0.0;
+    long long jh = 0, jl = 0, kh = 0, kl = 0;
     for (int x = 0; x < PM_XCD_SLOTS; ++x) {
-        jj += J[x * NN + a] + J[x * NN + b];
-        kk += K[x * NN + a] + K[x * NN + b];
+        const FxAcc ja = J[x * NN + a], jb = J[x * NN + b], ka = K[x * NN + a], kb = K[x * NN + b];
+        jh += ja.hi + jb.hi;
+        jl += ja.lo + jb.lo;
+        kh += ka.hi + kb.hi;
+        kl += ka.lo + kb.lo;
     }
-    G[ij] += cj * jj - kk;
+    G[ij] += cj * fx_value(jh, jl) - fx_value(kh, kl);
+}
+
+// p[0..n) = 0 (a kernel rather than hipMemsetAsync, so that a caller may
+// capture the build in a HIP graph)
+__global__ void zero_fx_kernel(FxAcc *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = FxAcc{0, 0};
 }
 
 __global__ void absmax_kernel(const double *__restrict__ p, int64_t n, double *out)
@@ -551,16 +564,18 @@ __global__ void absmax_kernel(const double *__restrict__ p, int64_t n, double *o
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
-// Launch every class over the device densities already in c.d_D (+ d_D2).
-// nk = 1: K from d_D; nk = 2: K0 from d_D, K1 from d_D2, J from d_D + d_D2
-// (passed as DJ).
-static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1, int nk, double dmax, hipStream_t s)
+// Launch every class: J' from density DJ, K' from density DK (both full
+// N x N in sorted function order), into the fixed-point copies J, K (zeroed
+// here).  nside side streams (0: everything on s).
+static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K, double dmax, int nside,
+                  bool timed, hipStream_t s)
 {
     const int64_t NN = (int64_t)c.N * c.N * PM_XCD_SLOTS;
-    HIPCHK(hipMemsetAsync(c.d_J.p, 0, sizeof(double) * NN, s));
-    HIPCHK(hipMemsetAsync(c.d_K0.p, 0, sizeof(double) * NN, s));
-    if (nk == 2) HIPCHK(hipMemsetAsync(c.d_K1.p, 0, sizeof(double) * NN, s));
-    if (g_timing) {
+    hipLaunchKernelGGL(zero_fx_kernel, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, s, J, NN);
+    hipLaunchKernelGGL(zero_fx_kernel, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, s, K, NN);
+    HIPCHK(hipGetLastError());
+    timed = timed && g_timing;
+    if (timed) {
         if (!c.ev0) {
             HIPCHK(hipEventCreate(&c.ev0));
             HIPCHK(hipEventCreate(&c.ev1));
@@ -573,7 +588,7 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1
     // Classes are independent: spread them over side streams so small grids
     // overlap.  With timing on, launches stay on the caller's stream so each
     // class's events bracket only its own kernels.
-    const int nside = g_timing ? 0 : 4;
+    if (timed) nside = 0;
     if (nside && c.side.empty()) {
         c.side.resize(nside);
         c.side_ev.resize(nside);
@@ -608,28 +623,26 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1
         a.pair_q = c.d_q.p;
         a.pp = c.d_pp.p;
         a.DJ = DJ;
-        a.DK0 = DK0;
-        a.DK1 = DK1 ? DK1 : DK0;
-        a.J = c.d_J.p;
-        a.K0 = c.d_K0.p;
-        a.K1 = nk == 2 ? c.d_K1.p : c.d_K0.p;
+        a.DK = DK;
+        a.J = J;
+        a.K = K;
         a.N = c.N;
         a.tau = c.tau;
         a.start_shell = c.start;
         a.rys = g_rys;
-        if (g_timing && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
+        if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
         if (n64 > 0) {
             a.items = c.d_items.p + r.first;
             a.nitems = n64;
-            if (launch_jk(r.bra_pc, r.ket_pc, a, 0, nk, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 0, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
         if (r.count > n64) {
             a.items = c.d_items.p + r.first + n64;
             a.nitems = r.count - n64;
             c.items_fp32 += r.count - n64;
-            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, nk, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
-        if (g_timing && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci + 1], s));
+        if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci + 1], s));
         HIPCHK(hipGetLastError());
         ++ci;
     }
@@ -639,7 +652,7 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK0, const double *DK1
             HIPCHK(hipStreamWaitEvent(s, c.side_ev[i], 0));
         }
     }
-    if (g_timing) HIPCHK(hipEventRecord(c.ev1, s));
+    if (timed) HIPCHK(hipEventRecord(c.ev1, s));
     return 0;
 }
 
@@ -662,13 +675,13 @@ static int collect_timing(Ctx &c)
     return 0;
 }
 
-static int alloc_work(Ctx &c, int nk)
+static int alloc_work(Ctx &c, bool uhf)
 {
     const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
-    if (c.d_D.alloc(NN) || c.d_J.alloc(NN * PM_XCD_SLOTS) || c.d_K0.alloc(NN * PM_XCD_SLOTS) || c.d_P.alloc(no2) ||
+    if (c.d_D.alloc(NN) || c.d_J.alloc(NN * PM_XCD_SLOTS) || c.d_K.alloc(NN * PM_XCD_SLOTS) || c.d_P.alloc(no2) ||
         c.d_G.alloc(no2) || c.d_dmax.alloc(1))
         return g_err;
-    if (nk == 2 && (c.d_D2.alloc(NN) || c.d_DJ.alloc(NN) || c.d_PB.alloc(no2) || c.d_G2.alloc(no2))) return g_err;
+    if (uhf && (c.d_D2.alloc(NN) || c.d_DJ.alloc(NN) || c.d_PB.alloc(no2) || c.d_G2.alloc(no2))) return g_err;
     return 0;
 }
 
@@ -681,18 +694,26 @@ static double device_absmax(Ctx &c, const double *p, int64_t n, hipStream_t s)
     return m;
 }
 
-static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
+// G += 2J[P] - K[P] on stream s with the work buffers D (N x N doubles) and
+// J, K (PM_XCD_SLOTS x N x N fixed-point each) given by the caller.
+static int rhf_build(Ctx &c, const double *dP, double *dG, double *D, FxAcc *J, FxAcc *K, double dmax, int nside,
+                     bool timed, hipStream_t s)
 {
-    if (alloc_work(c, 1)) return g_err;
     const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
-    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dP, c.d_D.p, c.d_fperm.p, c.N);
-    double dmax = 0.0;
-    if (g_prec_mode == 1) dmax = device_absmax(c, dP, no2, s);
-    if (run_jk(c, c.d_D.p, c.d_D.p, nullptr, 1, dmax, s)) return g_err;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, c.d_J.p, c.d_K0.p, dG, c.d_finv.p,
-                       c.N, 2.0);
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dP, D, c.d_fperm.p, c.N);
+    if (run_jk(c, D, D, J, K, dmax, nside, timed, s)) return g_err;
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, J, K, dG, c.d_finv.p, c.N, 2.0);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
+{
+    if (alloc_work(c, false)) return g_err;
+    const int64_t no2 = (int64_t)c.N * (c.N + 1) / 2;
+    double dmax = 0.0;
+    if (g_prec_mode == 1) dmax = device_absmax(c, dP, no2, s);
+    return rhf_build(c, dP, dG, c.d_D.p, c.d_J.p, c.d_K.p, dmax, 4, true, s);
 }
 
 // GA += J[PA+PB] - K[PA], GB += J[PA+PB] - K[PB] (scfaux.c:554-714), all on
@@ -700,18 +721,18 @@ static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
 // with J = J' + J'^T.
 static int uhf_device(Ctx &x, const double *dPA, const double *dPB, double *dGA, double *dGB, hipStream_t s)
 {
-    if (alloc_work(x, 2)) return g_err;
+    if (alloc_work(x, true)) return g_err;
     const int64_t NN = (int64_t)x.N * x.N, no2 = (int64_t)x.N * (x.N + 1) / 2;
     hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPA, x.d_D.p, x.d_fperm.p, x.N);
     hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dPB, x.d_D2.p, x.d_fperm.p, x.N);
     hipLaunchKernelGGL(add_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, x.d_D.p, x.d_D2.p, x.d_DJ.p, NN);
     double dmax = 0.0;
     if (g_prec_mode == 1) dmax = std::max(device_absmax(x, dPA, no2, s), device_absmax(x, dPB, no2, s));
-    if (run_jk(x, x.d_DJ.p, x.d_D.p, nullptr, 1, dmax, s)) return g_err;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGA, x.d_finv.p,
+    if (run_jk(x, x.d_DJ.p, x.d_D.p, x.d_J.p, x.d_K.p, dmax, 4, true, s)) return g_err;
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K.p, dGA, x.d_finv.p,
                        x.N, 1.0);
-    if (run_jk(x, x.d_DJ.p, x.d_D2.p, nullptr, 1, dmax, s)) return g_err;
-    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K0.p, dGB, x.d_finv.p,
+    if (run_jk(x, x.d_DJ.p, x.d_D2.p, x.d_J.p, x.d_K.p, dmax, 4, true, s)) return g_err;
+    hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, x.d_J.p, x.d_K.p, dGB, x.d_finv.p,
                        x.N, 1.0);
     HIPCHK(hipGetLastError());
     return 0;
@@ -784,9 +805,11 @@ void clean_ERIFS(ERIFS *e)
         }
     }
     if (c) {
-        for (auto *b : {&c->d_xyz, &c->d_q, &c->d_bound, &c->d_P, &c->d_PB, &c->d_D, &c->d_D2, &c->d_DJ, &c->d_J,
-                        &c->d_K0, &c->d_K1, &c->d_G, &c->d_G2, &c->d_dmax})
+        for (auto *b : {&c->d_xyz, &c->d_q, &c->d_bound, &c->d_P, &c->d_PB, &c->d_D, &c->d_D2, &c->d_DJ, &c->d_G,
+                        &c->d_G2, &c->d_dmax})
             b->release();
+        c->d_J.release();
+        c->d_K.release();
         for (auto *b : {&c->d_off, &c->d_soff, &c->d_fperm, &c->d_finv, &c->d_orig, &c->d_tile_first, &c->d_tile_size})
             b->release();
         c->d_pairs.release();
@@ -835,7 +858,7 @@ void rhf_form_gmatrix(const ERIFS *e, const double *P, double *G)
         return;
     }
     const int64_t no2 = (int64_t)c->N * (c->N + 1) / 2;
-    if (alloc_work(*c, 1)) return;
+    if (alloc_work(*c, false)) return;
     if (hipMemcpy(c->d_P.p, P, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->d_G.p, 0, sizeof(double) * no2) != hipSuccess) {
         fail(PATMOL_HIP_EDEVICE, "rhf_form_gmatrix: transfer failed");
@@ -865,7 +888,7 @@ void uhf_form_gmatrix(const ERIFS *e, const double *PA, const double *PB, double
     }
     Ctx &x = *c;
     const int64_t no2 = (int64_t)x.N * (x.N + 1) / 2;
-    if (alloc_work(x, 2)) return;
+    if (alloc_work(x, true)) return;
     if (hipMemcpy(x.d_P.p, PA, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(x.d_PB.p, PB, sizeof(double) * no2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(x.d_G.p, 0, sizeof(double) * no2) != hipSuccess ||

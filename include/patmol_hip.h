@@ -74,6 +74,23 @@ void patmol_hip_set_precision(int mode, double threshold);
  * Asynchronous with respect to the host. */
 int patmol_hip_rhf_gmatrix_device(const ERIFS *tfiles, const double *d_P, double *d_G, void *stream);
 
+/* ---- allocation-free device G build ---- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the same G += 2J[P] - K[P] as
+ * patmol_hip_rhf_gmatrix_device (bitwise identical in FP64), with every
+ * work buffer in a fixed device array of the library (17,301,504 bytes per
+ * device, held until the process ends) instead of allocated memory.  No
+ * hipMalloc/hipFree and no allocating library call; one stream (the
+ * caller's), no events.  Always FP64 (patmol_hip_set_precision does not
+ * apply: its split needs max|P| on the host).  Bases above
+ * PATMOL_HIP_NOALLOC_MAX_NORB functions fail with PATMOL_HIP_EUNSUPPORTED.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams or threads) race and must be serialised by the caller.
+ * The handle's tables come from two_electron_ints, which does allocate. */
+#define PATMOL_HIP_NOALLOC_MAX_NORB 256
+int patmol_hip_rhf_gmatrix_device_noalloc(const ERIFS *tfiles, const double *d_P, double *d_G, void *stream);
+
 /* ---- ERINT record interop (erifs.h:19-22) ---- */
 /* Writes every canonical integral of the basis with |v| > 1e-14 as ERINT
  * records, in the order of two_electron_ints (twoints.c:228-379), shells

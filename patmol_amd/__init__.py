@@ -36,6 +36,9 @@ class PatmolError(RuntimeError):
     pass
 
 
+NOALLOC_MAX_NORB = 256   # PATMOL_HIP_NOALLOC_MAX_NORB (include/patmol_hip.h)
+
+
 class HipStats(C.Structure):
     _fields_ = [("nshell", C.c_int), ("norb", C.c_int), ("start", C.c_int),
                 ("pairs", C.c_int64), ("shell_quartets_total", C.c_int64),
@@ -111,6 +114,8 @@ def lib():
         G.patmol_hip_set_precision.argtypes = [C.c_int, C.c_double]
         G.patmol_hip_rhf_gmatrix_device.restype = C.c_int
         G.patmol_hip_rhf_gmatrix_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        G.patmol_hip_rhf_gmatrix_device_noalloc.restype = C.c_int
+        G.patmol_hip_rhf_gmatrix_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         G.patmol_hip_stats.restype = C.c_int
         G.patmol_hip_stats.argtypes = [C.c_void_p, C.POINTER(HipStats)]
         G.patmol_hip_class_stats.restype = C.c_int
@@ -171,7 +176,7 @@ def exported_symbols():
     return ["init_ERIFS", "clean_ERIFS", "two_electron_ints", "rhf_form_gmatrix", "uhf_form_gmatrix",
             "dpm_ints", "one_electron_ints", "patmol_hip_last_error", "patmol_hip_error_string", "patmol_hip_set_strict",
             "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision",
-            "patmol_hip_rhf_gmatrix_device", "patmol_hip_stats", "patmol_hip_class_stats",
+            "patmol_hip_rhf_gmatrix_device", "patmol_hip_rhf_gmatrix_device_noalloc", "patmol_hip_stats", "patmol_hip_class_stats",
             "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
@@ -267,6 +272,14 @@ class EriStore:
         rc = L.patmol_hip_rhf_gmatrix_device(self.h, C.c_void_p(d_P), C.c_void_p(d_G), C.c_void_p(stream))
         if rc:
             _check("patmol_hip_rhf_gmatrix_device")
+
+    def rhf_gmatrix_device_noalloc(self, d_P: int, d_G: int, stream: int = 0) -> None:
+        """The allocation-free form of :meth:`rhf_gmatrix_device` (FP64,
+        bases of at most NOALLOC_MAX_NORB functions)."""
+        _, L = lib()
+        rc = L.patmol_hip_rhf_gmatrix_device_noalloc(self.h, C.c_void_p(d_P), C.c_void_p(d_G), C.c_void_p(stream))
+        if rc:
+            _check("patmol_hip_rhf_gmatrix_device_noalloc")
 
     def stats(self) -> dict:
         _, L = lib()

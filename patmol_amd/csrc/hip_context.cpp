@@ -716,6 +716,61 @@ static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
     return rhf_build(c, dP, dG, c.d_D.p, c.d_J.p, c.d_K.p, dmax, 4, true, s);
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free G build (patmol_hip_rhf_gmatrix_device_noalloc): the work
+// buffers of rhf_build live in one fixed device array, sized for bases of
+// up to PATMOL_HIP_NOALLOC_MAX_NORB functions:
+//   D                 N^2 doubles                  (offset 0)
+//   J', K' copies     PM_XCD_SLOTS N^2 FxAcc each
+// 17,301,504 bytes at N = 256, held by every device the library runs on
+// until the process ends.  Host code never uses the array's name as an
+// address: the device address comes from hipGetSymbolAddress, per device.
+// ---------------------------------------------------------------------------
+constexpr size_t noalloc_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NOALLOC_N = PATMOL_HIP_NOALLOC_MAX_NORB;
+constexpr size_t NOALLOC_D_BYTES = noalloc_align(sizeof(double) * NOALLOC_N * NOALLOC_N);
+constexpr size_t NOALLOC_FX_BYTES = noalloc_align(sizeof(FxAcc) * PM_XCD_SLOTS * NOALLOC_N * NOALLOC_N);
+constexpr size_t NOALLOC_BYTES = NOALLOC_D_BYTES + 2 * NOALLOC_FX_BYTES;
+static_assert(NOALLOC_BYTES == 17301504, "allocation-free scratch size");
+static __device__ __attribute__((aligned(256))) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+static int scratch_address(void **out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PATMOL_HIP_EDEVICE, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(PATMOL_HIP_EDEVICE, "hipGetSymbolAddress returned no address");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+static int rhf_device_noalloc(Ctx &c, const double *dP, double *dG, hipStream_t s)
+{
+    if (c.N > PATMOL_HIP_NOALLOC_MAX_NORB)
+        return fail(PATMOL_HIP_EUNSUPPORTED, "%d functions: the allocation-free G build takes at most %d", c.N,
+                    PATMOL_HIP_NOALLOC_MAX_NORB);
+    void *base = nullptr;
+    if (scratch_address(&base)) return g_err;
+    char *b = static_cast<char *>(base);
+    double *D = reinterpret_cast<double *>(b);
+    FxAcc *J = reinterpret_cast<FxAcc *>(b + NOALLOC_D_BYTES);
+    FxAcc *K = reinterpret_cast<FxAcc *>(b + NOALLOC_D_BYTES + NOALLOC_FX_BYTES);
+    // FP64 throughout (the FP32 split needs max|P| on the host), everything
+    // on the caller's stream, no events
+    return rhf_build(c, dP, dG, D, J, K, 0.0, 0, false, s);
+}
+
 // GA += J[PA+PB] - K[PA], GB += J[PA+PB] - K[PB] (scfaux.c:554-714), all on
 // stream s: two single-density passes (J[PA+PB], K[PA]) and (J[PA+PB], K[PB])
 // with J = J' + J'^T.
@@ -917,6 +972,15 @@ int patmol_hip_rhf_gmatrix_device(const ERIFS *e, const double *dP, double *dG, 
     if (!c || !dP || !dG) return fail(PATMOL_HIP_EBADARG, "bad handle or null pointer");
     if (!c->built) return fail(PATMOL_HIP_ESTATE, "two_electron_ints was not run for this handle");
     return rhf_device(*c, dP, dG, (hipStream_t)stream);
+}
+
+int patmol_hip_rhf_gmatrix_device_noalloc(const ERIFS *e, const double *dP, double *dG, void *stream)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *c = ctx_of(e);
+    if (!c || !dP || !dG) return fail(PATMOL_HIP_EBADARG, "bad handle or null pointer");
+    if (!c->built) return fail(PATMOL_HIP_ESTATE, "two_electron_ints was not run for this handle");
+    return rhf_device_noalloc(*c, dP, dG, (hipStream_t)stream);
 }
 
 int patmol_hip_uhf_gmatrix_device(const ERIFS *e, const double *dPA, const double *dPB, double *dGA, double *dGB,

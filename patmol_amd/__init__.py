@@ -160,6 +160,8 @@ def lib():
         H.patmol_rhf_moments.argtypes = [_dp, _dp, _dp, _dp]
         H.patmol_uhf_moments.argtypes = [_dp, _dp, _dp, _dp, _dp]
         H.patmol_rhf_finite_field.argtypes = [C.c_double, _dp, C.POINTER(ScfResult)]
+        H.patmol_dpm_calculation.restype = C.c_int
+        H.patmol_dpm_calculation.argtypes = [C.c_int, C.c_int, _dp, C.c_void_p, C.POINTER(ScfResult)]
         _host, _hip = H, G
     return _host, _hip
 
@@ -242,6 +244,13 @@ class EriStore:
         self.nshell = H.number_of_shells()
         self.h = G.init_ERIFS(0, self.norb, name.encode())
         _check("init_ERIFS")
+        G.two_electron_ints(self.h, start)
+        _check("two_electron_ints")
+
+    def rebuild(self, start: int = 0) -> None:
+        """two_electron_ints again on this handle (after the basis moved,
+        e.g. set_center_position of the DPM centre)."""
+        _, G = lib()
         G.two_electron_ints(self.h, start)
         _check("two_electron_ints")
 
@@ -562,3 +571,38 @@ def uhf_gmatrix_erints(recs, PA, PB, path: Optional[str] = None):
     if rc:
         _check("patmol_hip_uhf_gmatrix_erints")
     return GA, GB
+
+
+class DpmPoint(C.Structure):
+    _fields_ = [("vpol", C.c_double), ("alfa", C.c_double), ("e_first", C.c_double), ("e_conv", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def add_dpm_center() -> None:
+    """Add the DPM centre (charge +1) and its 7 shells to the current basis
+    (basis.c:220-241, 280-284); they take part once dpm_calculation (or
+    dpm_augment) makes them visible."""
+    H, _ = lib()
+    if H.patmol_basis_add_dpm_center():
+        raise PatmolError("patmol_basis_add_dpm_center failed")
+
+
+def dpm_calculation(points, uhf: bool = False):
+    """The DPM grid loop (rhf_dpm_calculation, rhfscf.c:217-263, or its UHF
+    form) on the device SCF for the current basis, whose DPM centre has been
+    added (:func:`add_dpm_center`) but not made visible.  points: (npts, 3)
+    positron positions in bohr.  Returns (per-point dicts with vpol, alfa,
+    e_first, e_conv, iterations, converged; ground-state result).  The basis
+    is left augmented, as the reference leaves it."""
+    H, _ = lib()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    out = (DpmPoint * max(1, len(pts)))()
+    gs = ScfResult()
+    rc = H.patmol_dpm_calculation(int(uhf), len(pts), pts.reshape(-1), C.addressof(out), C.byref(gs))
+    if rc:
+        _check("patmol_dpm_calculation")
+        raise PatmolError(f"patmol_dpm_calculation failed ({rc})")
+    return [out[i].as_dict() for i in range(len(pts))], gs.as_dict()

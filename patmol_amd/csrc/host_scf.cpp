@@ -10,6 +10,7 @@
 // resets the iteration counter (rhfscf.c:85-88).  The one-electron
 // integrals and the G builds go through the library's own device
 // one_electron_ints and rhf_form_gmatrix / uhf_form_gmatrix.
+#include <algorithm>
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -560,4 +561,93 @@ extern "C" int patmol_rhf_finite_field(double efx, double *alfa, PatmolScfResult
     alfa[3] = (alfa[0] + alfa[1] + alfa[2]) / 3.0;
     alfa[4] = (2.0 * alfa[2] - alfa[1] - alfa[2]) / 3.0;
     return 0;
+}
+
+// DPM grid loop (rhfscf.c:217-263, uhfscf.c:213-262) on the device SCF.
+extern "C" int patmol_dpm_calculation(int uhf, int npts, const double *pts, PatmolDpmPoint *out, PatmolScfResult *gs)
+{
+    const int ono = number_of_orbitals(), onsh = number_of_shells(), ne = number_of_electrons();
+    const int mult = spin_multiplicity();
+    if (npts < 0 || (npts > 0 && (!pts || !out)) || ono <= 0) return 1;
+    if (!uhf && (ne % 2 || mult != 1)) return 1;
+    const int noccb = (ne - (mult - 1)) / 2, nocca = ne - noccb;
+    const int maxits = max_scf_iterations(), accel = scf_acceleration();
+    const double eps = scf_convergence();
+    // ground state in the electron basis (rhf_energy / uhf_energy)
+    const size_t ono2 = (size_t)ono * (ono + 1) / 2;
+    std::vector<double> S(ono2), T(ono2), H(ono2), PgA(ono2), PgB(ono2);
+    one_electron_ints(S.data(), T.data(), H.data());
+    if (patmol_hip_last_error()) return 2;
+    ERIFS *mints = init_ERIFS(0, ono, "MINTS.DAT");
+    two_electron_ints(mints, 0);
+    if (patmol_hip_last_error()) {
+        clean_ERIFS(mints);
+        return 2;
+    }
+    PatmolHipScfResult g{};
+    int rc = uhf ? patmol_hip_uhf_scf(mints, nullptr, S.data(), H.data(), nocca, noccb, maxits, eps, accel,
+                                      nuclear_repulsion(), nullptr, nullptr, PgA.data(), PgB.data(), nullptr, nullptr, &g)
+                 : patmol_hip_rhf_scf(mints, nullptr, S.data(), H.data(), nocca, maxits, eps, accel, nuclear_repulsion(),
+                                      nullptr, PgA.data(), nullptr, nullptr, &g);
+    if (rc) {
+        clean_ERIFS(mints);
+        return 2;
+    }
+    if (gs) {
+        gs->energy = g.energy;
+        gs->eelec = g.eelec;
+        gs->enuc = g.enuc;
+        gs->ediff = g.ediff;
+        gs->pdiff = g.pdiff;
+        gs->iterations = g.iterations;
+        gs->converged = g.converged;
+        gs->gbuild_seconds = g.gbuild_ms * 1e-3;
+        gs->diag_seconds = g.other_ms * 1e-3;
+    }
+    // the DPM shells become part of the basis (rhfscf.c:226-229)
+    const int poscen = skip_center();
+    dpm_augment();
+    const int tnorb = number_of_orbitals();
+    if (tnorb <= ono || poscen < 0 || poscen >= number_of_centers()) {
+        clean_ERIFS(mints);
+        return 3;   // no DPM centre and shells were added
+    }
+    const size_t tno2 = (size_t)tnorb * (tnorb + 1) / 2;
+    ERIFS *xints = init_ERIFS(ono, tnorb, "XINTS.DAT");
+    std::vector<double> St(tno2), Tt(tno2), Ht(tno2), QA(tno2, 0.0), QB(tno2, 0.0);
+    // ground-state densities on the packed prefix of the augmented basis
+    // (the DPM functions are numbered after the electron functions)
+    std::copy(PgA.begin(), PgA.end(), QA.begin());
+    std::copy(PgB.begin(), PgB.end(), QB.begin());
+    for (int ip = 0; ip < npts; ++ip) {
+        const double *rp = pts + 3 * ip;
+        set_center_position(poscen, rp);
+        one_electron_ints(St.data(), Tt.data(), Ht.data());
+        if (!patmol_hip_last_error()) dpm_ints(Ht.data());
+        if (!patmol_hip_last_error()) two_electron_ints(xints, onsh);
+        if (patmol_hip_last_error()) {
+            rc = 2;
+            break;
+        }
+        PatmolHipScfResult r{};
+        rc = uhf ? patmol_hip_uhf_scf(mints, xints, St.data(), Ht.data(), nocca, noccb, maxits, eps, accel,
+                                      nuclear_repulsion(), QA.data(), QB.data(), nullptr, nullptr, nullptr, nullptr, &r)
+                 : patmol_hip_rhf_scf(mints, xints, St.data(), Ht.data(), nocca, maxits, eps, accel, nuclear_repulsion(),
+                                      QA.data(), nullptr, nullptr, nullptr, &r);
+        if (rc) {
+            rc = 2;
+            break;
+        }
+        const double r2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
+        PatmolDpmPoint &o = out[ip];
+        o.e_first = r.energy0;
+        o.e_conv = r.energy;
+        o.vpol = r.energy - r.energy0;
+        o.alfa = -2.0 * o.vpol * r2 * r2;
+        o.iterations = r.iterations;
+        o.converged = r.converged;
+    }
+    clean_ERIFS(xints);
+    clean_ERIFS(mints);
+    return rc;
 }

@@ -66,6 +66,26 @@ int patmol_uhf_moments(const double *PA, const double *PB, double *elec, double 
  * alfa[5] = xx, yy, zz, mean, and the reference's alfa2 */
 int patmol_rhf_finite_field(double efx, double *alfa, PatmolScfResult *gs);
 
+/* DPM grid loop, rhf_dpm_calculation / uhf_dpm_calculation (rhfscf.c:217-263,
+ * uhfscf.c:213-262), on the device SCF.  The DPM centre and shells must have
+ * been added (patmol_basis_add_dpm_center, or init_Basis with the dpm flag)
+ * and not yet made visible: the ground state is computed in the electron
+ * basis, then dpm_augment() and, per grid point pts[3i..3i+2] (bohr): the
+ * centre moved (set_center_position), S, T, H of the augmented basis with
+ * the DPM potential subtracted (one_electron_ints, dpm_ints), the ERIs with
+ * a DPM shell (two_electron_ints(xints, number of electron shells)), the SCF
+ * over mints + xints from the ground-state density (zero over the DPM
+ * functions), and vpol = E_conv - E_first with E_first the energy after the
+ * first update (rhfscf.c:245-253).  alfa = -2 vpol |r|^4 as rhfscf.c:256.
+ * uhf != 0 runs the UHF form (uhfscf.c) with the basis's multiplicity.
+ * The reference's vstat (rhfscf.c:254) reads an uninitialised variable and
+ * is not reproduced. */
+typedef struct {
+    double vpol, alfa, e_first, e_conv;
+    int iterations, converged;
+} PatmolDpmPoint;
+int patmol_dpm_calculation(int uhf, int npts, const double *pts, PatmolDpmPoint *out, PatmolScfResult *gs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,139 @@
+"""The DPM / positron path on the GPU against the oracle.
+
+rhf_dpm_calculation (rhfscf.c:217-263): the DPM centre and its 7 shells are
+added to the basis, and per positron position the ERIs with at least one DPM
+shell come from two_electron_ints(xints, number of electron shells) (the
+start_shell > 0 branch of the J/K kernels), re-run after every
+set_center_position; the SCF runs over mints + xints from the ground-state
+density, and vpol = E_conv - E_first.  Compared here: the xints G build
+(start > 0) and its rebuild after the centre moved against the oracle's
+two_electron_ints(start) records and scatter; mints + xints against the
+oracle's G of the whole augmented basis; the grid loop's energies, vpol and
+iteration counts against the oracle's loop; closed-shell UHF against RHF;
+and the mirror symmetry of a planar ring (z -> -z).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import patmol_amd as pm
+from patmol_amd import systems
+from tests.gpu_util import note, packed, random_density
+
+pytestmark = pytest.mark.gpu
+
+G_TOL = 1e-10
+E_TOL = 1e-8
+
+
+def _augment(s, Q):
+    """Electron basis of s with the DPM centre added, made visible and moved to Q."""
+    pm.load_system(s)
+    pm.add_dpm_center()
+    H, _ = pm.lib()
+    H.dpm_augment()
+    H.set_center_position(s.ncen, np.ascontiguousarray(Q, dtype=np.float64))
+
+
+def test_xints_start_gmatrix_and_rebuild_vs_oracle():
+    s = systems.water(d_shells=True)
+    Q1, Q2 = [0.3, -0.4, 2.0], [-1.0, 0.5, -2.5]
+    pm.set_options(tau=0.0, precision=0, strict=False)
+    _augment(s, Q1)
+    st = pm.EriStore(start=s.nshell, name="XINTS.DAT")
+    H, _ = pm.lib()
+    for k, Q in enumerate((Q1, Q2)):
+        if k:
+            H.set_center_position(s.ncen, np.ascontiguousarray(Q, dtype=np.float64))
+            st.rebuild(s.nshell)
+        bx = oracle.dpm_basis(s, Q)
+        assert st.norb == bx.norb == s.norb + 13
+        ints = bx.two_electron_ints(start=s.nshell)
+        P = packed(random_density(bx.norb, 41 + k))
+        G = st.rhf_gmatrix(P)
+        d = float(np.abs(G - ints.rhf_gmatrix(P)).max())
+        PA, PB = packed(random_density(bx.norb, 51 + k)), packed(random_density(bx.norb, 61 + k))
+        GA, GB = st.uhf_gmatrix(PA, PB)
+        RA, RB = ints.uhf_gmatrix(PA, PB)
+        du = max(float(np.abs(GA - RA).max()), float(np.abs(GB - RB).max()))
+        note(f"xints at Q{k + 1}: {ints.n} records, max |dG| rhf {d:.3e} uhf {du:.3e}")
+        assert d < G_TOL and du < G_TOL
+    st.close()
+
+
+def test_mints_plus_xints_is_the_augmented_g():
+    s = systems.water(d_shells=True)
+    Q = [0.8, 0.2, -1.7]
+    pm.set_options(tau=0.0, precision=0, strict=False)
+    pm.load_system(s)
+    mints = pm.EriStore(start=0)           # electron basis only (rhf_init)
+    pm.add_dpm_center()
+    H, _ = pm.lib()
+    H.dpm_augment()
+    H.set_center_position(s.ncen, np.ascontiguousarray(Q, dtype=np.float64))
+    xints = pm.EriStore(start=s.nshell, name="XINTS.DAT")
+    bx = oracle.dpm_basis(s, Q)
+    P = packed(random_density(bx.norb, 71))
+    G = mints.rhf_gmatrix(P)               # touches the electron-basis prefix only
+    G = xints.rhf_gmatrix(P, G)
+    ref = bx.two_electron_ints(start=0).rhf_gmatrix(P)
+    d = float(np.abs(G - ref).max())
+    note(f"mints + xints vs augmented G: {d:.3e}")
+    assert d < G_TOL
+    mints.close()
+    xints.close()
+
+
+POINTS = [[0.0, 0.0, 3.0], [0.5, -0.4, 2.5], [1.5, 1.0, -2.0]]
+
+
+def test_dpm_grid_loop_vs_oracle():
+    s = systems.water(d_shells=True)
+    pm.load_system(s)
+    pm.set_options(tau=0.0, precision=0, strict=False, scf_density=1)
+    pm.add_dpm_center()
+    got, gs = pm.dpm_calculation(POINTS)
+    ogs, ref = oracle.rhf_dpm_calculation(s, POINTS)
+    note(f"ground state: gpu {gs['energy']:.12f} ({gs['iterations']} it), oracle {ogs.energy:.12f} "
+         f"({ogs.iterations} it)")
+    assert abs(gs["energy"] - ogs.energy) < E_TOL
+    for p, g, r in zip(POINTS, got, ref):
+        note(f"  r={p}: vpol gpu {g['vpol']:.10f} oracle {r['vpol']:.10f}; E {g['e_conv']:.12f} / "
+             f"{r['e_conv']:.12f}; it {g['iterations']} / {r['iterations']}")
+        assert g["converged"] == 1 and r["converged"] == 1
+        for k in ("e_first", "e_conv", "vpol"):
+            assert abs(g[k] - r[k]) < E_TOL, k
+        assert abs(g["alfa"] - r["alfa"]) < E_TOL * 2.0 * float(np.dot(p, p)) ** 2
+        assert abs(g["iterations"] - r["iterations"]) <= 3
+
+
+def test_dpm_closed_shell_uhf_equals_rhf():
+    s = systems.water(d_shells=True)
+    pm.set_options(tau=1e-12, precision=0, strict=False)
+    pm.load_system(s)
+    pm.add_dpm_center()
+    r, _ = pm.dpm_calculation(POINTS[:2])
+    pm.load_system(s)
+    pm.add_dpm_center()
+    u, _ = pm.dpm_calculation(POINTS[:2], uhf=True)
+    for a, b in zip(r, u):
+        note(f"  vpol rhf {a['vpol']:.10f} uhf {b['vpol']:.10f}")
+        assert abs(a["e_conv"] - b["e_conv"]) < E_TOL and abs(a["vpol"] - b["vpol"]) < E_TOL
+
+
+def test_dpm_ring_mirror_symmetry():
+    """Positron at the centre of the planar C6H6-shaped ring and 2 bohr above
+    and below it: the two off-plane points are mirror images (z -> -z), so
+    their energies and vpol agree."""
+    s = systems.benzene_ring()
+    pm.load_system(s)
+    pm.set_options(tau=1e-12, precision=0, strict=False, scf_density=1)
+    pm.add_dpm_center()
+    pts = [[0.0, 0.0, 0.0], [0.0, 0.0, 2.0], [0.0, 0.0, -2.0]]
+    got, gs = pm.dpm_calculation(pts)
+    for p, g in zip(pts, got):
+        note(f"  ring r={p}: vpol {g['vpol']:.10f} E {g['e_conv']:.10f} it {g['iterations']}")
+        assert g["converged"] == 1
+    assert abs(got[1]["e_conv"] - got[2]["e_conv"]) < E_TOL
+    assert abs(got[1]["vpol"] - got[2]["vpol"]) < E_TOL
+    assert got[0]["vpol"] < 0.0 and got[1]["vpol"] < 0.0

@@ -188,14 +188,14 @@ def test_rhf_scf_water_vs_oracle():
     assert abs(res["energy"] - ref.energy) < 1e-8
     assert res["converged"] == 1 and res["pdiff"] < 1e-8
     # The reference's stopping rule (rhfscf.c:52-59) stops on |dE| < 10 eps
-    # (2.2e-15 Eh) and damps on the sign of dE (rhfscf.c:60-69); near
-    # convergence both react to summation-order noise of ~1e-14 in G (the
-    # J/K atomics and the oracle's threads order sums differently from run
-    # to run), so the stop moves by several iterations while E is fixed to
-    # 1e-12 Eh: observed 38-47 against the oracle's 46, the early end being
-    # a step whose energy rounded to exactly the previous one (dE = 0).  The
-    # SF6 golden shows the same spread (43 vs 41-42).
-    assert abs(res["iterations"] - ref.iterations) <= 10
+    # (2.2e-15 Eh, an unchanged energy) or pdiff < eps and damps on the sign
+    # of dE (rhfscf.c:60-69); near convergence both react to rounding noise
+    # of ~1e-14 in G.  The device G build is bitwise reproducible (fixed-point
+    # J/K sums), and so is the oracle's single-thread scatter, so each takes
+    # one iteration count for this input; they can differ only through the
+    # rounding of their differently ordered sums (observed 44 vs 45; the DPM
+    # points and the device SCF differ by at most 2).
+    assert abs(res["iterations"] - ref.iterations) <= 2
 
 
 def test_sf6_golden_energy(golden_dir):
@@ -209,4 +209,4 @@ def test_sf6_golden_energy(golden_dir):
     # the reference's own Rys fits carry ~5e-9 relative error (see oracle
     # tests); with exact roots the energy sits 5.1e-6 Eh below the golden
     assert abs(res["energy"] - gold["energy"]) < 2e-5
-    assert abs(res["iterations"] - gold["iterations"]) <= 3
+    assert abs(res["iterations"] - gold["iterations"]) <= 1

@@ -74,9 +74,9 @@ def test_rhf_scf_device_water_vs_oracle(density_mode):
     assert abs(res["energy"] - ref.energy) < 1e-8
     assert np.abs(P - Pr).max() < 1e-6
     assert np.abs(ev - evr).max() < 1e-6
-    # the stopping rule reacts to ~1e-14 summation-order noise (see the
-    # host-driver SCF test): the iteration count may move by a few
-    assert abs(res["iterations"] - ref.iterations) <= 10
+    # the stopping rule reacts to ~1e-14 rounding noise (see the host-driver
+    # SCF test): device and oracle sum in different orders, each reproducibly
+    assert abs(res["iterations"] - ref.iterations) <= 2
     # same answer as the host driver with CPU diagonalisation
     hres, hP, hev = pm.rhf_scf()
     assert abs(hres["energy"] - res["energy"]) < 1e-8
@@ -139,7 +139,7 @@ def test_sf6_golden_energy_device(golden_dir):
     # the reference's own Rys fits carry ~5e-9 relative error: with exact
     # roots the energy sits 5.1e-6 Eh below the golden
     assert abs(res["energy"] - gold["energy"]) < 2e-5
-    assert abs(res["iterations"] - gold["iterations"]) <= 3
+    assert abs(res["iterations"] - gold["iterations"]) <= 1
     occ = np.array(gold["orbital_energies"][: s.nelec // 2])
     assert np.abs(ev[: s.nelec // 2] - occ).max() < 2e-5
 

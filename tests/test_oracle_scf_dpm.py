@@ -235,3 +235,19 @@ def test_dpm_s_type_closed_form():
                 v += ck * 2 * math.pi ** 2.5 / (p * ak * math.sqrt(p + ak)) * Kab * F0(T)
             i, j = off[I], off[J]
             assert abs(-h[i * (i + 1) // 2 + j] - na * nb * v) < 1e-13, (I, J)
+
+
+def test_dpm_grid_loop_mirror_symmetry():
+    """The oracle's DPM grid loop (rhfscf.c:217-263) on water, which lies in
+    the yz plane with the H atoms at +/-y: positron positions related by the
+    mirrors x -> -x and y -> -y give the same first-update and converged
+    energies and the same vpol; a position far from the molecule polarises it
+    less than a near one."""
+    s = systems.water()
+    pts = [[0.6, 0.4, 2.2], [-0.6, 0.4, 2.2], [0.6, -0.4, 2.2], [0.0, 0.0, 12.0]]
+    gs, out = oracle.rhf_dpm_calculation(s, pts, nthreads=4)
+    assert gs.converged and all(o["converged"] for o in out)
+    for o in out[1:3]:
+        for k in ("e_first", "e_conv", "vpol"):
+            assert abs(o[k] - out[0][k]) < 1e-9, k
+    assert out[0]["vpol"] < out[3]["vpol"] < 0.0

@@ -1,21 +1,26 @@
 #!/usr/bin/env python3
 """ERI + Fock (G) build per SCF iteration on one MI355X.
 
-Workload: (H2O)8 on a 2x2x2 lattice (2.9 A spacing, waters rigidly rotated
-by seeded uniform rotations, seed 1001), even-tempered s/p/d Cartesian basis:
-232 shells, 448 functions, 80 electrons.  One step is one direct G build
-G = 2J[P] - K[P] (every surviving shell quartet's integrals recomputed and
-contracted), with the density P and G already resident in HBM, through the
-library's device entry point.  The step is timed in FP64 and in mixed
-precision (work items whose Schwarz bound times max|P| is below 1e-6 run in
-FP32, accumulated in FP64); |dE| of the mixed build against the FP64 build
-is reported.  value = surviving shell quartets per second of the mixed
-build (of the FP64 build if the mixed one misses |dE| <= 1e-6, runs no FP32
-work item or is not faster).
+Headline workload: (H2O)16 on a 4x2x2 lattice (2.9 A spacing, waters rigidly
+rotated by seeded uniform rotations, seed 1004), even-tempered s/p/d
+Cartesian basis: 464 shells, 896 functions, 160 electrons -- the largest
+single-GPU system among the benchmark's configurations.  One step is one
+direct G build G = 2J[P] - K[P] (every surviving shell quartet's integrals
+recomputed and contracted), with the density P and G already resident in
+HBM, through the library's device entry point.  The step is timed in FP64
+and in mixed precision (work items whose Schwarz bound times max|P| is below
+1e-6 run in FP32, accumulated in fixed point); the SCF is also converged in
+both modes on the device and |dE| of the converged energies is reported.
+value = surviving canonical shell quartets per second of the mixed build
+(of the FP64 build if the mixed one misses |dE| <= 1e-6 Eh, runs no FP32
+work item or is not faster).  A secondary block times (H2O)8 (448
+functions) the same way.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
        python bench.py --calibrate-copy BYTES   (plain device copy of BYTES,
        three launches, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)
+       python bench.py --profile-build SYSTEM   (one FP64 build with classes
+       serialised, for rocprofv3 runs; SYSTEM water8 or water16)
 """
 import argparse
 import json
@@ -32,7 +37,9 @@ METRIC = "ERI+Fock s/SCF-iter, mixed vs FP64 on 1 GPU at |dE|<=1e-6 Eh; quartets
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix)
 TAU = 1e-12
 MIXED_THR = 1e-6
-CPU_SAMPLE_MOD = 1
+# the oracle's direct build keeps one canonical tuple in CPU_SAMPLE_MOD
+# ((I + 3J + 7K + 13L) mod CPU_SAMPLE_MOD == 0): about 10-20 s of CPU work
+CPU_SAMPLE_MOD = 7
 
 
 def log(msg):
@@ -57,17 +64,10 @@ def core_guess_density(S, H, n, nocc):
     return P[il].copy()
 
 
-def elec_energy(P, H, G, n):
-    """2 * calc_energy (scfaux.c:99-118): sum_full P (2H + G) / ... as the reference."""
-    diag = np.array([i * (i + 1) // 2 + i for i in range(n)])
-    w = np.ones_like(P)
-    w[diag] = 0.5
-    s1 = float(np.sum(w * P * H))
-    s2 = float(np.sum(w * P * G))
-    return 2.0 * (s1 + s1 + s2)
-
-
 def time_builds(torch, store, dP, dG, steps, warmup, stream):
+    """Mean ms per G build over `steps` builds after `warmup`, bracketed by
+    events on the launch stream (the library forks its side streams from it
+    and joins them back)."""
     for _ in range(warmup):
         dG.zero_()
         store.rhf_gmatrix_device(dP.data_ptr(), dG.data_ptr(), stream.cuda_stream)
@@ -83,14 +83,12 @@ def time_builds(torch, store, dP, dG, steps, warmup, stream):
     return ev0.elapsed_time(ev1) / steps
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_water8_fp64.json")
-
-
-def dominant_traffic(kernel):
+def traffic_of(workload_key, kernel):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), if it lists it."""
+    FETCH_SIZE/WRITE_SIZE summary of this workload (copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_key}_fp64.json")
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(path) as fh:
             data = json.load(fh)
     except (OSError, ValueError):
         return None
@@ -100,25 +98,124 @@ def dominant_traffic(kernel):
     return None
 
 
+CLASS_NAMES = ["ss", "ps", "pp", "ds", "dp", "dd"]
+
+
+def measure(torch, pm, system, key, steps, warmup, scf):
+    """FP64 and mixed G builds of `system` (core-Hamiltonian guess density),
+    per-class kernel times, the dominant class kernel's roofline, and
+    optionally the converged device SCF in both modes."""
+    n, nocc = system.norb, system.nelec // 2
+    pm.load_system(system)
+    pm.set_options(tau=TAU, precision=0, timing=True, strict=False)
+    S, T, H = pm.one_electron_ints()
+    P = core_guess_density(S, H, n, nocc)
+    log(f"workload {system.name}: {system.nshell} shells, {n} functions")
+    store = pm.EriStore()
+    st = store.stats()
+    q36, f36 = store.class_stats()
+    log(f"  kept shell quartets {st['shell_quartets_kept']} of {st['shell_quartets_total']}, "
+        f"algorithmic FLOPs/build {st['flops_per_build']:.4g}, work items {st['work_items']}")
+    stream = torch.cuda.current_stream()
+    dP = torch.tensor(P, dtype=torch.float64, device="cuda")
+    dG = torch.zeros_like(dP)
+    # per-class pass: classes serialised on one stream, each bracketed by
+    # HIP events inside the library
+    time_builds(torch, store, dP, dG, 1, 1, stream)
+    tot_ms, cls_ms = store.last_timing()
+    pm.set_options(timing=False)
+    ms64 = time_builds(torch, store, dP, dG, steps, warmup, stream)
+    G64 = dG.cpu().numpy().copy()
+    k = int(np.argmax(cls_ms))
+    achieved = f36[k] / (cls_ms[k] * 1e-3) / 1e12 if cls_ms[k] > 0 else 0.0
+    dom = f"({CLASS_NAMES[k // 6]}|{CLASS_NAMES[k % 6]})"
+    dom_kernel = store.class_kernel_name(k // 6, k % 6)
+    traffic = traffic_of(key, dom_kernel)
+    per_class = {f"({CLASS_NAMES[i // 6]}|{CLASS_NAMES[i % 6]})":
+                 {"ms": round(float(cls_ms[i]), 4), "quartets": int(q36[i]),
+                  "tflops": round(f36[i] / (cls_ms[i] * 1e-3) / 1e12, 3)}
+                 for i in range(36) if q36[i] > 0 and cls_ms[i] > 0}
+    log(f"  FP64: {ms64:.3f} ms/build (classes serialised: {tot_ms:.3f} ms); dominant {dom} "
+        f"{cls_ms[k]:.3f} ms, {achieved:.2f} TFLOP/s")
+    pm.set_options(precision=1, mixed_threshold=MIXED_THR)
+    msmx = time_builds(torch, store, dP, dG, steps, warmup, stream)
+    Gmx = dG.cpu().numpy().copy()
+    stm = store.stats()
+    pm.set_options(precision=0)
+    log(f"  mixed: {msmx:.3f} ms/build, max |G_mixed - G_fp64| {np.abs(Gmx - G64).max():.3e}, "
+        f"FP32 items {stm['work_items_fp32']}/{stm['work_items']}")
+    out = {
+        "workload": key,
+        "shells": system.nshell,
+        "functions": n,
+        "shell_quartets_kept": st["shell_quartets_kept"],
+        "shell_quartets_total": st["shell_quartets_total"],
+        "flops_per_build": st["flops_per_build"],
+        "ms_per_build_fp64": ms64,
+        "ms_per_build_mixed": msmx,
+        "mixed_speedup": ms64 / msmx if msmx > 0 else None,
+        "fp32_items": stm["work_items_fp32"],
+        "work_items": stm["work_items"],
+        "max_abs_dG_mixed": float(np.abs(Gmx - G64).max()),
+        "quartets_per_s_fp64": st["shell_quartets_kept"] / (ms64 * 1e-3),
+        "build_tflops": st["flops_per_build"] / (ms64 * 1e-3) / 1e12,
+        "serialised_class_ms": tot_ms,
+        "per_class_fp64": per_class,
+        "dominant": {"class": dom, "kernel_symbol": dom_kernel, "ms": float(cls_ms[k]),
+                     "flops": float(f36[k]), "tflops": achieved, "traffic": traffic},
+    }
+    if scf:
+        # the whole RHF iteration on the device, FP64 and mixed, with Pulay
+        # DIIS (scf_accel 3: the reference's damping leaves (H2O)16
+        # oscillating from the core guess); a run makes iterations + 1
+        # updates (the first is not counted, rhfscf.c:85-88)
+        system.scf_a = 3
+        pm.load_system(system)
+        runs = {}
+        for mode in (0, 1):
+            pm.set_options(precision=mode, mixed_threshold=MIXED_THR)
+            r, _, _ = pm.rhf_scf_device(store=store)
+            nupd = r["iterations"] + 1
+            runs[mode] = {"energy": r["energy"], "iterations": r["iterations"], "converged": r["converged"],
+                          "ms_per_iteration": r["total_ms"] / nupd, "gbuild_ms_per_iteration": r["gbuild_ms"] / nupd,
+                          "other_ms_per_iteration": r["other_ms"] / nupd, "purify_fallbacks": r["purify_fallbacks"]}
+            log(f"  device SCF, DIIS ({'mixed' if mode else 'FP64'}): E {r['energy']:.10f} in {r['iterations']} "
+                f"iterations, {runs[mode]['ms_per_iteration']:.2f} ms/iteration (G "
+                f"{runs[mode]['gbuild_ms_per_iteration']:.2f})")
+        pm.set_options(precision=0)
+        out["scf_fp64"] = runs[0]
+        out["scf_mixed"] = runs[1]
+        out["dE_scf_mixed_vs_fp64"] = abs(runs[1]["energy"] - runs[0]["energy"])
+    store.close()
+    return out, P
+
+
+def cpu_threads():
+    """Threads of the host's share: OMP_NUM_THREADS where the environment
+    sets it (16 per GPU on the GPU boxes), else every core."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
 def cpu_baseline(system, P, kept_total):
-    """The oracle's direct G build (the reference's Rys algorithm, canonical
-    loop, in-memory scatter) on host cores, over the same Schwarz-screened
-    quartet set as the device build, timed over one full build
-    (CPU_SAMPLE_MOD = 1, about 10 s on 16 host threads; a larger value times
-    the class-representative subset of one canonical tuple in CPU_SAMPLE_MOD)."""
+    """The oracle's direct G build (the reference's Rys algorithm, its
+    canonical loop, in-memory scatter) on the host threads, over the same
+    Schwarz-screened quartet set as the device build, on the sample of one
+    canonical tuple in CPU_SAMPLE_MOD; counted in canonical shell quartets,
+    the unit of the device's kept count."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     b = oracle.Basis(system)
     q = b.pair_schwarz(nthreads=threads)
     t0 = time.time()
     _, done = b.direct_gmatrix_sampled(P, q, TAU, CPU_SAMPLE_MOD, nthreads=threads)
     dt = time.time() - t0
     rate = done / dt
-    which = ("all of them: one full G build" if CPU_SAMPLE_MOD == 1 else
-             f"every canonical tuple with (I+3J+7K+13L) % {CPU_SAMPLE_MOD} == 0 of the same G build")
-    return {"value": rate, "unit": "quartets/s", "cores": threads, "kind": "port",
-            "sample": f"{done} Schwarz-kept shell quartets (tau {TAU:g}; {which}) in {dt:.1f} s",
+    return {"value": rate, "unit": "quartets/s", "cores": threads, "nproc": os.cpu_count(), "kind": "port",
+            "sample": f"{done} Schwarz-kept canonical shell quartets (tau {TAU:g}) of the {system.name} build, "
+                      f"every canonical tuple with (I+3J+7K+13L) % {CPU_SAMPLE_MOD} == 0, in {dt:.1f} s on "
+                      f"{threads} threads",
             "s_per_build_est": kept_total / rate}
 
 
@@ -163,18 +260,6 @@ def erint_interop(pm, systems, with_cpu):
     return out
 
 
-def cpu_diag_ms(S, H, nocc):
-    """One SCF diagonalisation step by the oracle's single-thread
-    restatement: X = U s^-1/2, X^T F X, eigenvectors, C and P, i.e. two
-    eigenproblems where an SCF iteration needs one (X is formed once per
-    run); the CPU counterpart of the device SCF's work outside the G build."""
-    import oracle
-
-    t0 = time.time()
-    oracle.diag_fock(S, H, nocc)
-    return (time.time() - t0) * 1e3
-
-
 def calibrate_copy(nbytes):
     """Three launches of a plain device-to-device copy of nbytes: the
     reference reading for the HBM traffic counters."""
@@ -191,6 +276,29 @@ def calibrate_copy(nbytes):
     return 0
 
 
+def profile_build(name):
+    """One FP64 G build of `name` with the class kernels serialised on one
+    stream (after one warm-up build), for rocprofv3 kernel-trace and counter
+    runs."""
+    import torch
+    import patmol_amd as pm
+    from patmol_amd import systems
+
+    system = getattr(systems, name)()
+    pm.load_system(system)
+    pm.set_options(tau=TAU, precision=0, timing=True, strict=False)
+    S, T, H = pm.one_electron_ints()
+    P = core_guess_density(S, H, system.norb, system.nelec // 2)
+    store = pm.EriStore()
+    stream = torch.cuda.current_stream()
+    dP = torch.tensor(P, dtype=torch.float64, device="cuda")
+    dG = torch.zeros_like(dP)
+    ms = time_builds(torch, store, dP, dG, 1, 1, stream)
+    store.close()
+    print(json.dumps({"profile_build": name, "ms": ms}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,16 +306,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
-    ap.add_argument("--serial-classes", action="store_true",
-                    help="launch the class kernels of every build on one stream (profiling runs)")
+    ap.add_argument("--profile-build", default="", metavar="SYSTEM")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the G build runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         return 2
-
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
+    if args.profile_build:
+        return profile_build(args.profile_build)
 
     import torch
     import patmol_amd as pm
@@ -216,85 +324,19 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         return 1
-    system = systems.water8()
-    n, nocc = system.norb, system.nelec // 2
-    pm.load_system(system)
-    pm.set_options(tau=TAU, precision=0, timing=True, strict=False)
-    S, T, H = pm.one_electron_ints()
-    t0 = time.perf_counter()
-    for _ in range(5):
-        pm.one_electron_ints()
-    ms_oneints = (time.perf_counter() - t0) * 1e3 / 5
-    P = core_guess_density(S, H, n, nocc)
-    log(f"workload {system.name}: {system.nshell} shells, {n} functions")
-    store = pm.EriStore()
-    st = store.stats()
-    q36, f36 = store.class_stats()
-    log(f"kept quartets {st['shell_quartets_kept']} of {st['shell_quartets_total']}, "
-        f"algorithmic FLOPs/build {st['flops_per_build']:.4g}, work items {st['work_items']}")
-    stream = torch.cuda.current_stream()
-    dP = torch.tensor(P, dtype=torch.float64, device="cuda")
-    dG = torch.zeros_like(dP)
+    head_sys = systems.water16()
+    head, P16 = measure(torch, pm, head_sys, "water16", args.steps, args.warmup, scf=True)
+    sec, _ = measure(torch, pm, systems.water8(), "water8", args.steps, args.warmup, scf=False)
 
-    # FP64, per-class pass: classes serialised on one stream, each bracketed
-    # by HIP events inside the library
-    time_builds(torch, store, dP, dG, 1, 1, stream)
-    tot_ms, cls_ms = store.last_timing()
-    # FP64 timed pass: classes concurrent on side streams
-    pm.set_options(timing=args.serial_classes)
-    ms64 = time_builds(torch, store, dP, dG, args.steps, args.warmup, stream)
-    G64 = dG.cpu().numpy().copy()
-    E64 = elec_energy(P, H, G64, n)
-    log(f"FP64: {ms64:.3f} ms/build (classes serialised: {tot_ms:.3f} ms), E_elec {E64:.10f}")
-    # dominant class kernel (FP64): algorithmic FLOPs / its HIP-event duration
-    k = int(np.argmax(cls_ms))
-    achieved = f36[k] / (cls_ms[k] * 1e-3) / 1e12 if cls_ms[k] > 0 else 0.0
-    names = ["ss", "ps", "pp", "ds", "dp", "dd"]
-    dom = f"({names[k // 6]}|{names[k % 6]})"
-    dom_kernel = store.class_kernel_name(k // 6, k % 6)
-    traffic = dominant_traffic(dom_kernel)
-    per_class = {f"({names[i // 6]}|{names[i % 6]})": {"ms": round(float(cls_ms[i]), 4), "quartets": int(q36[i]),
-                                                      "tflops": round(f36[i] / (cls_ms[i] * 1e-3) / 1e12, 3)}
-                 for i in range(36) if q36[i] > 0 and cls_ms[i] > 0}
-
-    # mixed precision
-    pm.set_options(precision=1, mixed_threshold=MIXED_THR)
-    msmx = time_builds(torch, store, dP, dG, args.steps, args.warmup, stream)
-    Gmx = dG.cpu().numpy().copy()
-    Emx = elec_energy(P, H, Gmx, n)
-    stm = store.stats()
-    pm.set_options(precision=0)
-    dE = abs(Emx - E64)
-    log(f"mixed: {msmx:.3f} ms/build, |dE| {dE:.3e}, FP32 items {stm['work_items_fp32']}/{stm['work_items']}")
-
-    # reference-compatible entry point: host P in, host G out (PCIe included)
-    nh = max(1, min(args.steps, 5))
-    store.rhf_gmatrix(P)
-    t0 = time.perf_counter()
-    for _ in range(nh):
-        store.rhf_gmatrix(P)
-    ms_host = (time.perf_counter() - t0) * 1e3 / nh
-    log(f"host-array entry (rhf_form_gmatrix, PCIe included): {ms_host:.3f} ms/build")
-
-    # the whole RHF iteration on the device (patmol_hip_rhf_scf), same
-    # basis and screened store, core-Hamiltonian guess; a run makes
-    # iterations + 1 updates (the first is not counted, rhfscf.c:85-88)
-    scf, _, _ = pm.rhf_scf_device(store=store)
-    nupd = scf["iterations"] + 1
-    scf_dev = {"energy": scf["energy"], "iterations": scf["iterations"], "converged": scf["converged"],
-               "ms_per_iteration": scf["total_ms"] / nupd, "gbuild_ms_per_iteration": scf["gbuild_ms"] / nupd,
-               "other_ms_per_iteration": scf["other_ms"] / nupd, "purify_fallbacks": scf["purify_fallbacks"]}
-    log(f"device SCF: E {scf['energy']:.10f} in {scf['iterations']} iterations, "
-        f"{scf_dev['ms_per_iteration']:.2f} ms/iteration (G {scf_dev['gbuild_ms_per_iteration']:.2f}, "
-        f"rest {scf_dev['other_ms_per_iteration']:.2f}), purification fallbacks {scf['purify_fallbacks']}")
-
-    use_mixed = dE <= 1e-6 and stm["work_items_fp32"] > 0 and msmx < ms64
-    ms = msmx if use_mixed else ms64
-    kept = st["shell_quartets_kept"]
-    value = kept / (ms * 1e-3)
+    dE = head["dE_scf_mixed_vs_fp64"]
+    use_mixed = dE <= 1e-6 and head["fp32_items"] > 0 and head["ms_per_build_mixed"] < head["ms_per_build_fp64"]
+    ms = head["ms_per_build_mixed"] if use_mixed else head["ms_per_build_fp64"]
+    kept = head["shell_quartets_kept"]
+    dom = head["dominant"]
+    tr = dom["traffic"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": kept / (ms * 1e-3),
         "unit": "quartets/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -304,47 +346,42 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+f32" if use_mixed else "f64",
-        "data": "synthetic: seeded (H2O)8 geometry, even-tempered basis, core-Hamiltonian guess density",
+        "data": "synthetic: seeded (H2O)16 geometry, even-tempered basis, core-Hamiltonian guess density",
         "config": {
-            "workload": "(H2O)8, 2x2x2 lattice 2.9 A, seed 1001, even-tempered s/p/d (O d 0.3*3^k), "
-                        "232 shells, 448 functions; one direct G build per step",
+            "workload": "(H2O)16, 4x2x2 lattice 2.9 A, seed 1004, even-tempered s/p/d (O d 0.3*3^k), "
+                        "464 shells, 896 functions; one direct G build per step",
             "schwarz_tau": TAU,
             "mixed_threshold": MIXED_THR,
             "shell_quartets_kept": kept,
-            "shell_quartets_total": st["shell_quartets_total"],
+            "shell_quartets_total": head["shell_quartets_total"],
             "parallelism": "single GPU",
         },
-        "s_per_iter_fp64": ms64 * 1e-3,
-        "s_per_iter_mixed": msmx * 1e-3,
-        "mixed_speedup": ms64 / msmx if msmx > 0 else None,
-        "dE_mixed_vs_fp64": dE,
-        "quartets_per_s_fp64": kept / (ms64 * 1e-3),
-        "fp32_items": stm["work_items_fp32"],
-        "ms_per_step_host_entry": ms_host,
-        "one_electron_ints_ms": ms_oneints,
-        "scf_device": scf_dev,
+        "s_per_iter_fp64": head["ms_per_build_fp64"] * 1e-3,
+        "s_per_iter_mixed": head["ms_per_build_mixed"] * 1e-3,
+        "mixed_speedup": head["mixed_speedup"],
+        "dE_scf_mixed_vs_fp64": dE,
         "roofline": {
             "bound": "mfma",
-            "kernel": f"jk_kernel {dom} FP64",
-            "achieved": achieved,
+            "bound_detail": "FP64 arithmetic: MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; "
+                            "the J/K kernels run on the FP64 VALU (no MFMA)",
+            "kernel": f"J/K {dom['class']} FP64",
+            "kernel_symbol": dom["kernel_symbol"],
+            "achieved": dom["tflops"],
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": traffic["bytes"] if traffic else None,
-            "traffic_fetch": traffic["fetch_bytes"] if traffic else None,
-            "traffic_write": traffic["write_bytes"] if traffic else None,
-            "kernel_symbol": dom_kernel,
-            "build_tflops": st["flops_per_build"] / (ms64 * 1e-3) / 1e12,
-            "build_frac": st["flops_per_build"] / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "frac": dom["tflops"] / FP64_PEAK_TFLOPS,
+            "traffic": tr["bytes"] if tr else None,
+            "traffic_fetch": tr["fetch_bytes"] if tr else None,
+            "traffic_write": tr["write_bytes"] if tr else None,
+            "build_tflops": head["build_tflops"],
+            "build_frac": head["build_tflops"] / FP64_PEAK_TFLOPS,
         },
-        "per_class_fp64": per_class,
-        "serialised_class_ms": tot_ms,
+        "headline": head,
+        "secondary_water8": sec,
     }
     if not args.no_cpu:
         log("cpu baseline (oracle direct build, bounded sample) ...")
-        out["cpu_baseline"] = cpu_baseline(system, P, kept)
-        out["scf_device"]["cpu_diag_ms"] = cpu_diag_ms(S, H, nocc)
-    store.close()
+        out["cpu_baseline"] = cpu_baseline(head_sys, P16, kept)
     out["erint_interop"] = erint_interop(pm, systems, not args.no_cpu)
     log(f"ERINT interop: {out['erint_interop']}")
     print(json.dumps(out))

@@ -129,7 +129,11 @@ int patmol_hip_write_momints(const char *path);
  * as rhf_form_xmatrix (rhfscf.c:432-452) fails with PATMOL_HIP_EDEVICE on an
  * overlap eigenvalue <= DBL_EPSILON.  The guess is the core Hamiltonian
  * (rhfscf.c:454-471, uhfscf.c:505-527) unless one is passed.  scf_eps gets
- * the floor 10 DBL_EPSILON N^2 (rhfscf.c:161-164).  S, H, guesses and
+ * the floor 10 DBL_EPSILON N^2 (rhfscf.c:161-164).  scf_accel 0, 1, 2 are the
+ * reference's (none, averaging, scfAccelerator on dE > 0); scf_accel 3 (RHF
+ * only, no counterpart in the reference) extrapolates the Fock matrix by
+ * Pulay DIIS over the last 8 iterations (commutator error X^T (FDS - SDF) X)
+ * instead, for systems the reference's damping does not converge.  S, H, guesses and
  * outputs are host packed arrays (Cmat row-major N x N, MO j in column j);
  * any output may be NULL.  Returns 0 or an error code.  Set
  * PATMOL_HIP_SCF_LOG=1 for the reference's per-iteration log lines. */

@@ -819,7 +819,9 @@ void or_pair_schwarz(const OBasis *bs, double *q, int nthreads)
  * only: q (ns x ns Schwarz bounds) with tau skips shell quartets whose bound
  * product is below tau, and sample_mod > 1 keeps only the tuples with
  * (ish + 3 jsh + 7 ksh + 13 lsh) % sample_mod == 0, a class-representative
- * subset.  done_quartets counts the shell quartets actually computed. */
+ * subset.  done_quartets counts the canonical shell quartets computed
+ * (I >= J, K >= L, IJ >= KL; the loop's (IJ|IL), L > J tuples are computed
+ * but not counted), the unit the device build counts its kept quartets in. */
 void or_rhf_direct_gmatrix_sampled(const OBasis *bs, int start, const double *P, double *G, int nthreads,
                                    int64_t max_quartets, int64_t *done_quartets, const double *q,
                                    double tau, int sample_mod);
@@ -876,7 +878,10 @@ void or_rhf_direct_gmatrix_sampled(const OBasis *bs, int start, const double *P,
                                         if (!computed) {
                                             or_eri_block(bs, ish, jsh, ksh, lsh, blk);
                                             computed = 1;
-                                            ++done;
+                                            /* counted in canonical shell quartets (I >= J, K >= L,
+                                             * IJ >= KL): the (IJ|IL), L > J tuples of the
+                                             * reference loop are computed but not counted */
+                                            if (!(ksh == ish && lsh > jsh)) ++done;
                                         }
                                         double v = blk[((ils * nJ + jls) * nK + kls) * nL + lls];
                                         if (fabs(v) > 1e-14) {

@@ -15,7 +15,9 @@
 //   sums    calc_energy and pdiffNorm (scfaux.c:97-136), and the three dot
 //           products of scfAccelerator (scfaux.c:64-95), each in a fixed
 //           summation order;
-//   damping the averaging / two-point Anderson updates (rhfscf.c:60-69).
+//   damping the averaging / two-point Anderson updates (rhfscf.c:60-69);
+//           or, with scf_accel 3 (no counterpart in the reference), Pulay
+//           DIIS on the Fock matrices (RHF).
 // The host keeps only the iteration logic: per iteration it reads back the
 // energy and density change (two doubles) to take the reference's
 // convergence and damping decisions.
@@ -572,6 +574,60 @@ __global__ __launch_bounds__(256) void tc2_step_kernel(const double *__restrict_
     }
 }
 
+// ---- Pulay DIIS (scf_accel 3) on the orthonormal-basis Fock matrices ----
+// The commutator error e = F D S - S D F = A - A^T with A = F D S (F, D, S
+// symmetric), column-major n x n.
+__global__ void antisym_kernel(const double *__restrict__ A, double *__restrict__ E, int n)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n) return;
+    const int64_t i = idx % n, j = idx / n;
+    E[idx] = A[idx] - A[i * n + j];
+}
+
+// dot products of the error in slot k with the errors of slots 0..m-1
+// (grid SUMS_BLOCKS x m): block b of slot j reduces one contiguous chunk in
+// a fixed tree; diis_dot_final_kernel adds the partials in block order
+__global__ __launch_bounds__(256) void diis_dot_part_kernel(const double *__restrict__ ebuf, int k, int64_t nn,
+                                                           double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int j = blockIdx.y;
+    const double *__restrict__ a = ebuf + (size_t)k * nn;
+    const double *__restrict__ b = ebuf + (size_t)j * nn;
+    const int64_t chunk = (nn + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < nn ? lo + chunk : nn;
+    double v = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) v += a[i] * b[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)j * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void diis_dot_final_kernel(const double *__restrict__ part, int nb, int m, double *__restrict__ out)
+{
+    const int j = threadIdx.x;
+    if (j >= m) return;
+    double v = 0.0;
+    for (int b = 0; b < nb; ++b) v += part[(size_t)j * nb + b];
+    out[j] = v;
+}
+
+// F = sum_j c_j F_j over the m stored Fock matrices
+__global__ void diis_combine_kernel(const double *__restrict__ fbuf, const double *__restrict__ c, int m, int64_t nn,
+                                    double *__restrict__ F)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nn) return;
+    double v = 0.0;
+    for (int j = 0; j < m; ++j) v += c[j] * fbuf[(size_t)j * nn + idx];
+    F[idx] = v;
+}
+
 struct DevMem {
     std::vector<void *> ptrs;
     ~DevMem()
@@ -798,9 +854,14 @@ struct Scf {
 
     bool diag_density(const double *a, const double *b, int nocc, double *P, int ch)
     {
+        return fprime(a, b) && diag_from_fprime(nocc, P, ch);
+    }
+
+    // the density of F' (in F) by the eigensolver
+    bool diag_from_fprime(int nocc, double *P, int ch)
+    {
         const int64_t nn = (int64_t)n * n;
         const double one = 1.0, zero = 0.0;
-        if (!fprime(a, b)) return false;
         // F' (overwritten by the eigenvectors) and its exact spectral bounds
         // become the channel's reference for the bounds of later
         // purifications
@@ -857,6 +918,150 @@ struct Scf {
     void average(double *p, const double *p1, const double *p2)
     {
         hipLaunchKernelGGL(average_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, p, p1, p2, (int64_t)no2);
+    }
+
+    // ---- Pulay DIIS (scf_accel 3) ----
+    // The last DIIS_M orthonormal-basis Fock matrices F' = X^T F X and
+    // commutator errors e' = X^T (F D S - S D F) X of the iteration's input
+    // density D; the Fock matrix the density is taken from is the
+    // combination sum c_j F'_j with sum c_j = 1 that minimises |sum c_j e'_j|
+    // (Pulay, Chem. Phys. Lett. 73, 393 (1980); J. Comput. Chem. 3, 556
+    // (1982)).  The error dot products are fixed-order device sums, the small
+    // linear system is solved on the host, so the sequence is reproducible.
+    static constexpr int DIIS_M = 8;
+    double *Sfull = nullptr, *Dfull = nullptr, *Fao = nullptr, *W = nullptr;
+    double *dfbuf = nullptr, *debuf = nullptr, *dpart = nullptr, *dout = nullptr, *dcoef = nullptr;
+    int diis_cnt = 0, diis_next = 0;
+    double Bm[DIIS_M][DIIS_M] = {};
+    double diis_err = 0.0;   // Frobenius norm of the newest error
+
+    bool diis_init(const double *dS)
+    {
+        const size_t nn = (size_t)n * n;
+        Sfull = mem.get<double>(nn);
+        Dfull = mem.get<double>(nn);
+        Fao = mem.get<double>(nn);
+        W = mem.get<double>(nn);
+        dfbuf = mem.get<double>(DIIS_M * nn);
+        debuf = mem.get<double>(DIIS_M * nn);
+        dpart = mem.get<double>((size_t)DIIS_M * SUMS_BLOCKS);
+        dout = mem.get<double>(DIIS_M);
+        dcoef = mem.get<double>(DIIS_M);
+        if (!Sfull || !Dfull || !Fao || !W || !dfbuf || !debuf || !dpart || !dout || !dcoef)
+            return fail("device allocation failed");
+        hipLaunchKernelGGL(sym_from_packed_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, dS, nullptr, Sfull, n);
+        diis_cnt = diis_next = 0;
+        return hipGetLastError() == hipSuccess || fail("launch failure");
+    }
+
+    // F' of (a + b) extrapolated by DIIS over the input density Pin, into F
+    bool diis_fprime(const double *a, const double *b, const double *Pin)
+    {
+        const size_t nn = (size_t)n * n;
+        const double one = 1.0, zero = 0.0;
+        const dim3 g((unsigned)((nn + 255) / 256));
+        hipLaunchKernelGGL(sym_from_packed_kernel, g, dim3(256), 0, s, a, b, Fao, n);
+        hipLaunchKernelGGL(sym_from_packed_kernel, g, dim3(256), 0, s, Pin, nullptr, Dfull, n);
+        const int k = diis_next;
+        double *Ek = debuf + (size_t)k * nn, *Fk = dfbuf + (size_t)k * nn;
+        auto gemm = [&](rocblas_operation ta, const double *A, const double *B, double *Cm) {
+            return rocblas_dgemm(h, ta, rocblas_operation_none, n, n, n, &one, A, n, B, n, &zero, Cm, n) ==
+                   rocblas_status_success;
+        };
+        // e' = X^T (A - A^T) X with A = F D S; F' = X^T F X
+        if (!gemm(rocblas_operation_none, Fao, Dfull, W) || !gemm(rocblas_operation_none, W, Sfull, T))
+            return fail("rocblas_dgemm failed");
+        hipLaunchKernelGGL(antisym_kernel, g, dim3(256), 0, s, T, W, n);
+        if (!gemm(rocblas_operation_transpose, X, W, T) || !gemm(rocblas_operation_none, T, X, Ek) ||
+            !gemm(rocblas_operation_none, Fao, X, T) || !gemm(rocblas_operation_transpose, X, T, Fk))
+            return fail("rocblas_dgemm failed");
+        diis_next = (diis_next + 1) % DIIS_M;
+        diis_cnt = std::min(diis_cnt + 1, DIIS_M);
+        const int m = diis_cnt;
+        hipLaunchKernelGGL(diis_dot_part_kernel, dim3(SUMS_BLOCKS, m), dim3(256), 0, s, debuf, k, (int64_t)nn, dpart);
+        hipLaunchKernelGGL(diis_dot_final_kernel, dim3(1), dim3(64), 0, s, dpart, SUMS_BLOCKS, m, dout);
+        double d[DIIS_M];
+        if (hipMemcpyAsync(d, dout, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail("device failure in DIIS");
+        for (int j = 0; j < m; ++j) Bm[k][j] = Bm[j][k] = d[j];
+        diis_err = std::sqrt(std::max(0.0, d[k]));
+        // minimise c^T B c subject to sum c = 1 over the stored slots (in slot
+        // order), dropping the oldest slots while the system is singular
+        std::vector<int> use;
+        for (int j = 0; j < m; ++j) use.push_back((k - (m - 1) + j + DIIS_M) % DIIS_M);   // oldest first
+        std::vector<double> c;
+        while (!use.empty()) {
+            std::vector<int> ord(use);
+            std::sort(ord.begin(), ord.end());
+            const int q = (int)ord.size();
+            std::vector<double> A((size_t)(q + 1) * (q + 1), 0.0), r(q + 1, 0.0);
+            double scale = 0.0;
+            for (int i = 0; i < q; ++i) scale = std::max(scale, Bm[ord[i]][ord[i]]);
+            if (scale <= 0.0) scale = 1.0;
+            for (int i = 0; i < q; ++i) {
+                for (int j = 0; j < q; ++j) A[(size_t)i * (q + 1) + j] = Bm[ord[i]][ord[j]] / scale;
+                A[(size_t)i * (q + 1) + q] = -1.0;
+                A[(size_t)q * (q + 1) + i] = -1.0;
+            }
+            r[q] = -1.0;
+            if (solve(q + 1, A, r)) {
+                c.assign(DIIS_M, 0.0);
+                for (int i = 0; i < q; ++i) c[ord[i]] = r[i];
+                break;
+            }
+            use.erase(use.begin());   // drop the oldest
+        }
+        if (c.empty()) {   // cannot happen for a nonzero newest error; take F' itself
+            c.assign(DIIS_M, 0.0);
+            c[k] = 1.0;
+        }
+        if (hipMemcpyAsync(dcoef, c.data(), sizeof(double) * DIIS_M, hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail("upload failed");
+        hipLaunchKernelGGL(diis_combine_kernel, g, dim3(256), 0, s, dfbuf, dcoef, DIIS_M, (int64_t)nn, F);
+        // the host vector c must outlive the copy
+        if (hipStreamSynchronize(s) != hipSuccess) return fail("device failure in DIIS");
+        return hipGetLastError() == hipSuccess || fail("launch failure");
+    }
+
+    // Gaussian elimination with partial pivoting of the dense q x q system
+    // A x = r (x into r); false when a pivot vanishes
+    static bool solve(int q, std::vector<double> &A, std::vector<double> &r)
+    {
+        for (int c = 0; c < q; ++c) {
+            int p = c;
+            for (int i = c + 1; i < q; ++i)
+                if (std::fabs(A[(size_t)i * q + c]) > std::fabs(A[(size_t)p * q + c])) p = i;
+            if (std::fabs(A[(size_t)p * q + c]) < 1e-14) return false;
+            if (p != c) {
+                for (int j = 0; j < q; ++j) std::swap(A[(size_t)p * q + j], A[(size_t)c * q + j]);
+                std::swap(r[p], r[c]);
+            }
+            for (int i = c + 1; i < q; ++i) {
+                const double f = A[(size_t)i * q + c] / A[(size_t)c * q + c];
+                for (int j = c; j < q; ++j) A[(size_t)i * q + j] -= f * A[(size_t)c * q + j];
+                r[i] -= f * r[c];
+            }
+        }
+        for (int i = q - 1; i >= 0; --i) {
+            double v = r[i];
+            for (int j = i + 1; j < q; ++j) v -= A[(size_t)i * q + j] * r[j];
+            r[i] = v / A[(size_t)i * q + i];
+        }
+        return true;
+    }
+
+    // the density of the DIIS-extrapolated Fock matrix of (a + b)
+    bool diis_density(const double *a, const double *b, const double *Pin, int nocc, double *P, int ch)
+    {
+        if (!diis_fprime(a, b, Pin)) return false;
+        if (purify && nocc > 0 && nocc < n) {
+            bool conv = false;
+            if (!purify_density(nocc, P, conv, ch)) return false;
+            if (conv) return true;
+            ++purify_fallbacks;
+        }
+        return diag_from_fprime(nocc, P, ch);
     }
 
     float elapsed_ms()
@@ -920,6 +1125,8 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         hipMemsetAsync(P2, 0, sizeof(double) * no2, c.s) != hipSuccess)
         return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: upload failed");
     if (!c.form_x(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+    const bool diis = scf_accel == 3;
+    if (diis && !c.diis_init(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
     if (Pguess) {
         if (hipMemcpyAsync(P, Pguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
             return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: upload failed");
@@ -949,7 +1156,9 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         P2 = P1;
         P1 = P;
         P = fresh;
-        if (!c.density(dH, dG, nocc, P, 0)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        // P1 is the density this iteration's G was built from
+        const bool dens_ok = diis ? c.diis_density(dH, dG, P1, nocc, P, 0) : c.density(dH, dG, nocc, P, 0);
+        if (!dens_ok) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
         double sm[3];
         if (!c.sums3(P, P1, dH, dG, sm)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
         const float g_ms = c.elapsed_ms();
@@ -974,7 +1183,9 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         } else {
             done = ret;
         }
-        if (log) fprintf(stderr, "ITERATION = %12d EDIFF= %lg\n  energy = %.12f\n", iteration, ediff, enrg + enuc);
+        if (log)
+            fprintf(stderr, "ITERATION = %12d EDIFF= %lg PDIFF= %lg\n  energy = %.12f\n", iteration, ediff, pdiff,
+                    enrg + enuc);
         if (iteration == 0) continue;
         if (done || iteration >= maxits) break;
     }
@@ -1033,6 +1244,7 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         hipMemsetAsync(PB1, 0, sizeof(double) * no2, c.s) != hipSuccess ||
         hipMemsetAsync(PB2, 0, sizeof(double) * no2, c.s) != hipSuccess)
         return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: upload failed");
+    if (scf_accel == 3) return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_uhf_scf: DIIS (scf_accel 3) is RHF only");
     if (!c.form_x(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf");
     // core-Hamiltonian guess: both spins from the same orbitals (uhfscf.c:505-527)
     if (PAguess && PBguess) {
@@ -1105,7 +1317,9 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         } else {
             done = ret;
         }
-        if (log) fprintf(stderr, "ITERATION = %12d EDIFF= %lg\n  energy = %.12f\n", iteration, ediff, enrg + enuc);
+        if (log)
+            fprintf(stderr, "ITERATION = %12d EDIFF= %lg PDIFF= %lg\n  energy = %.12f\n", iteration, ediff, pdiff,
+                    enrg + enuc);
         if (iteration == 0) continue;
         if (done || iteration >= maxits) break;
     }

@@ -163,3 +163,22 @@ def test_purification_matches_eigensolver_benzene(mode):
     assert abs(r1["energy"] - r0["energy"]) < 1e-8
     assert np.abs(P1 - P0).max() < 1e-6
     assert np.abs(e1 - e0).max() < 1e-6
+
+
+def test_rhf_scf_device_diis_water_vs_oracle():
+    """Pulay DIIS (scf_accel 3, no counterpart in the reference) reaches the
+    oracle's converged energy and density in fewer iterations than the
+    reference's damping."""
+    s = systems.water()
+    s.scf_a = 3
+    pm.load_system(s)
+    pm.set_options(tau=0.0, precision=0, scf_density=1)
+    res, P, ev = pm.rhf_scf_device()
+    b = oracle.Basis(s)
+    ref, Pr, evr = oracle.rhf_scf(b, b.two_electron_ints(), s.nelec, maxits=s.maxits, eps=s.scf_eps, scf_a=2)
+    note(f"device RHF water, DIIS: E={res['energy']:.12f} it={res['iterations']} (oracle {ref.energy:.12f}, "
+         f"{ref.iterations} with damping)")
+    assert res["converged"] == 1 and res["iterations"] < ref.iterations
+    assert abs(res["energy"] - ref.energy) < 1e-8
+    assert np.abs(P - Pr).max() < 1e-6
+    assert np.abs(ev - evr).max() < 1e-6

@@ -310,6 +310,8 @@ def test_sampled_direct_build_is_a_subset(water_d):
     d = np.einsum("abab->ab", b.eri_block_unscreened(I, J, I, J))
     assert abs(q[I, J] - np.sqrt(np.abs(d).max())) < 1e-15 and q[I, J] == q[J, I]
     G0, n0 = b.direct_gmatrix(P, nthreads=2)
+    npair = s.nshell * (s.nshell + 1) // 2
+    assert n0 == npair * (npair + 1) // 2      # every canonical shell quartet, counted once
     G1, n1 = b.direct_gmatrix_sampled(P, q, 0.0, 1, nthreads=2)
     assert n0 == n1 and np.abs(G0 - G1).max() < 1e-12
     _, n3 = b.direct_gmatrix_sampled(P, q, 1e-12, 3, nthreads=2)

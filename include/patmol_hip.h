@@ -173,6 +173,16 @@ int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int nocc, doubl
 int patmol_hip_uhf_gmatrix_device(const ERIFS *tfiles, const double *d_PA, const double *d_PB, double *d_GA,
                                   double *d_GB, void *stream);
 
+/* ---- nuclear gradient ---- */
+/* No counterpart in the reference (it has no gradients).  The two-electron
+ * part of the RHF energy gradient for the packed half density P (host,
+ * no2): grad[3 c + x] += sum_{abcd} Gamma_abcd d(ab|cd)/dR_{c,x} with
+ * Gamma = 2 P_ab P_cd - (P_ac P_bd + P_ad P_bc)/2, over the handle's
+ * Schwarz-screened quartets (the energy's own screen), for every centre c of
+ * the basis (grad has 3 * number_of_centers() doubles, accumulated).  FP64,
+ * reproducible bit for bit. */
+int patmol_hip_eri_gradient(const ERIFS *tfiles, const double *Pmat, double *grad);
+
 /* ---- introspection ---- */
 typedef struct {
     int nshell, norb, start;

@@ -160,6 +160,13 @@ def lib():
         H.patmol_rhf_moments.argtypes = [_dp, _dp, _dp, _dp]
         H.patmol_uhf_moments.argtypes = [_dp, _dp, _dp, _dp, _dp]
         H.patmol_rhf_finite_field.argtypes = [C.c_double, _dp, C.POINTER(ScfResult)]
+        H.patmol_rhf_gradient.restype = C.c_int
+        H.patmol_rhf_gradient.argtypes = [C.c_void_p, _dp, _dp]
+        H.patmol_host_onee_gradient.restype = C.c_int
+        H.patmol_host_onee_gradient.argtypes = [_dp, _dp, _dp]
+        H.patmol_nuclear_repulsion_gradient.argtypes = [_dp]
+        G.patmol_hip_eri_gradient.restype = C.c_int
+        G.patmol_hip_eri_gradient.argtypes = [C.c_void_p, _dp, _dp]
         H.patmol_dpm_calculation.restype = C.c_int
         H.patmol_dpm_calculation.argtypes = [C.c_int, C.c_int, _dp, C.c_void_p, C.POINTER(ScfResult)]
         _host, _hip = H, G
@@ -184,7 +191,7 @@ def exported_symbols():
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
             "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
             "patmol_hip_write_erints", "patmol_hip_rhf_gmatrix_erints", "patmol_hip_rhf_gmatrix_erint_file",
-            "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file"]
+            "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file", "patmol_hip_eri_gradient"]
 
 
 def load_system(sys_: "systems.System") -> None:
@@ -606,3 +613,47 @@ def dpm_calculation(points, uhf: bool = False):
         _check("patmol_dpm_calculation")
         raise PatmolError(f"patmol_dpm_calculation failed ({rc})")
     return [out[i].as_dict() for i in range(len(pts))], gs.as_dict()
+
+
+def rhf_gradient(P: np.ndarray, store: Optional["EriStore"] = None) -> np.ndarray:
+    """dE/dR (ncen x 3, hartree/bohr) of the RHF energy at the converged half
+    density P of the current basis: two-electron part on the GPU over
+    `store`'s screened quartets (or a store built for the call), one-electron
+    and nuclear repulsion parts on the host with W = P F P."""
+    H, _ = lib()
+    nc = H.number_of_centers()
+    g = np.zeros(3 * nc)
+    rc = H.patmol_rhf_gradient(None if store is None else store.h, np.ascontiguousarray(P, dtype=np.float64), g)
+    if rc:
+        _check("patmol_rhf_gradient")
+        raise PatmolError(f"patmol_rhf_gradient failed ({rc})")
+    return g.reshape(nc, 3)
+
+
+def eri_gradient(P: np.ndarray, store: "EriStore") -> np.ndarray:
+    """The two-electron part alone (patmol_hip_eri_gradient), ncen x 3."""
+    H, G = lib()
+    nc = H.number_of_centers()
+    g = np.zeros(3 * nc)
+    if G.patmol_hip_eri_gradient(store.h, np.ascontiguousarray(P, dtype=np.float64), g):
+        _check("patmol_hip_eri_gradient")
+    return g.reshape(nc, 3)
+
+
+def host_onee_gradient(P: np.ndarray, W: np.ndarray) -> np.ndarray:
+    """2 sum P dH/dR - 2 sum W dS/dR (packed P, W) for the current basis,
+    computed on the host, ncen x 3."""
+    H, _ = lib()
+    nc = H.number_of_centers()
+    g = np.zeros(3 * nc)
+    if H.patmol_host_onee_gradient(np.ascontiguousarray(P, dtype=np.float64), np.ascontiguousarray(W, dtype=np.float64), g):
+        raise PatmolError("patmol_host_onee_gradient failed")
+    return g.reshape(nc, 3)
+
+
+def nuclear_repulsion_gradient() -> np.ndarray:
+    H, _ = lib()
+    nc = H.number_of_centers()
+    g = np.zeros(3 * nc)
+    H.patmol_nuclear_repulsion_gradient(g)
+    return g.reshape(nc, 3)

@@ -1,0 +1,292 @@
+// grad.hip -- the two-electron part of the RHF nuclear gradient on the GPU.
+//
+// dE_2e/dR = sum_{abcd} Gamma_abcd d(ab|cd)/dR with, for the half density P
+// of the reference (E = 2 sum P H + sum P G(P), rhfscf.c:49, scfaux.c:97-118),
+//   Gamma_abcd = 2 P_ab P_cd - (P_ac P_bd + P_ad P_bc) / 2,
+// which has the 8-fold symmetry of the integrals, so the sum runs over the
+// canonical shell quartets of the screened work list (the same tile
+// quartets as the J/K build) with the factor 8 w, w = 2^-([I=J]+[K=L]+[IJ=KL]).
+//
+// Derivative integrals: d/dA_x of a primitive Cartesian Gaussian is
+// 2 alpha G(a + 1_x) - a_x G(a - 1_x), so the derivatives with respect to
+// the centres A, B, C come from the Rys 2-D recursion (rys.c:2269-2292) one
+// order higher on each side and the transfer (rys.c:2295-2335) with the
+// shifted exponents; the centre D follows from translational invariance,
+// d/dD = -(d/dA + d/dB + d/dC).  Roots are digested one at a time (the
+// contraction with Gamma is linear), the per-root 1-D integrals of all
+// shifted exponents held in a per-lane table.
+//
+// One workgroup per tile quartet; its threads take the item's shell quartets
+// in turn.  Per-shell gradient sums go through LDS and global memory as
+// fixed-point integers (FxAcc), so the gradient is reproducible bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "eri_kernels.h"
+#include "grad.h"
+
+namespace pm {
+
+namespace {
+
+__device__ __forceinline__ void fx_lds_add(FxAcc *p, double v)
+{
+    if (v == 0.0) return;
+    const FxAcc s = fx_split(v);
+    __hip_atomic_fetch_add(&p->hi, s.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&p->lo, s.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void fx_global_add(FxAcc *p, FxAcc v)
+{
+    // the low word brought into [-2^39, 2^39) first, as the J/K flush does
+    const long long carry = (v.lo + (1LL << 39)) >> 40;
+    const long long lo = v.lo - carry * (1LL << 40), hi = v.hi + carry;
+    if (hi != 0) __hip_atomic_fetch_add(&p->hi, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lo != 0) __hip_atomic_fetch_add(&p->lo, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Cartesian exponents of component i of shell l (reference order,
+// basis.c:294-308), packed (lx | ly << 2 | lz << 4)
+template <int L>
+__device__ __forceinline__ int cart_exps(int i)
+{
+    int v = 0;
+    static_for<ncart(L)>([&](auto k) {
+        constexpr int K = decltype(k)::value;
+        if (i == K) v = cart_x(L, K) | (cart_y(L, K) << 2) | (cart_z(L, K) << 4);
+    });
+    return v;
+}
+
+}  // namespace
+
+// 1-D integrals of one root and one axis with every exponent shift the
+// derivatives need: t[ax][bx][cx][dx], ax <= LA+1, bx <= LB+1, cx <= LC+1,
+// dx <= LD (entries with ax + bx > LA + LB + 1 or cx + dx > LC + LD + 1 are
+// never read)
+template <typename R, int LA, int LB, int LC, int LD>
+struct GradTab {
+    static constexpr int SA = (LB + 2) * (LC + 2) * (LD + 1), SB = (LC + 2) * (LD + 1), SC = LD + 1;
+    static constexpr int NX = (LA + 2) * SA;
+    R t[3][NX];
+    __device__ __forceinline__ R at(int axis, int a, int b, int c, int d) const
+    {
+        return t[axis][a * SA + b * SB + c * SC + d];
+    }
+};
+
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void grad2e_kernel(GradArgs a)
+{
+    constexpr int TI = tile_of(LA), TJ = tile_of(LB), TK = tile_of(LC), TL = tile_of(LD);
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int L12 = LA + LB + 1, L34 = LC + LD + 1;     // recursion orders with the shift
+    constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;       // roots of the derivative integrals
+    using Tab = GradTab<R, LA, LB, LC, LD>;
+    __shared__ FxAcc sg[(TI + TJ + TK + TL) * 3];
+
+    const int4 it = a.items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int fI = a.tile_first[it.x], nI = a.tile_size[it.x];
+    const int fJ = a.tile_first[it.y], nJ = a.tile_size[it.y];
+    const int fK = a.tile_first[it.z], nK = a.tile_size[it.z];
+    const int fL = a.tile_first[it.w], nL = a.tile_size[it.w];
+    const bool sameBra = it.x == it.y, sameKet = it.z == it.w, samePair = it.x == it.z && it.y == it.w;
+    const int N = a.N;
+    for (int e = tid; e < (TI + TJ + TK + TL) * 3; e += 256) sg[e] = FxAcc{0, 0};
+    __syncthreads();
+
+    for (int q = tid; q < TI * TJ * TK * TL; q += 256) {
+        const int lq = q % TL, kq = (q / TL) % TK, jq = (q / (TL * TK)) % TJ, iq = q / (TL * TK * TJ);
+        if (iq >= nI || jq >= nJ || kq >= nK || lq >= nL) continue;
+        const int I = fI + iq, J = fJ + jq, K = fK + kq, L = fL + lq;
+        if (sameBra && J > I) continue;
+        if (sameKet && L > K) continue;
+        if (samePair && !(I > K || (I == K && J >= L))) continue;
+        const int pb = I * (I + 1) / 2 + J, pk = K * (K + 1) / 2 + L;
+        if (a.pair_q[pb] * a.pair_q[pk] < a.tau) continue;
+        double w = 8.0;
+        if (I == J) w *= 0.5;
+        if (K == L) w *= 0.5;
+        if (I == K && J == L) w *= 0.5;
+        const PairRef bref = a.pairs[pb], kref = a.pairs[pk];
+        double xyz[4][3];
+        for (int x = 0; x < 3; ++x) {
+            xyz[0][x] = a.shell_xyz[3 * I + x];
+            xyz[1][x] = a.shell_xyz[3 * J + x];
+            xyz[2][x] = a.shell_xyz[3 * K + x];
+            xyz[3][x] = a.shell_xyz[3 * L + x];
+        }
+        const int oI = a.shell_off[I], oJ = a.shell_off[J], oK = a.shell_off[K], oL = a.shell_off[L];
+        double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
+        for (int ib = 0; ib < bref.npp; ++ib) {
+            const PrimPair bp = a.pp[bref.pp0 + ib];
+            const double2 bab = a.ppab[bref.pp0 + ib];
+            for (int ik = 0; ik < kref.npp; ++ik) {
+                const PrimPair kp = a.pp[kref.pp0 + ik];
+                const double2 kab = a.ppab[kref.pp0 + ik];
+                const double p = bp.p, qq = kp.p, pq = p + qq;
+                const double rpq = rsqrt_pos(pq), ipq = rpq * rpq;
+                const double sr0 = SRTERM * (bp.s * bp.ip) * (kp.s * kp.ip) * rpq;
+                if (sr0 < PRIM_SCREEN) continue;   // twoints.c:110-111
+                const double sr = sr0 * bp.c * kp.c * w;
+                double PQ[3], PA[3], QC[3], AB[3], CD[3];
+                for (int x = 0; x < 3; ++x) {
+                    PQ[x] = bp.P[x] - kp.P[x];
+                    PA[x] = bp.P[x] - xyz[0][x];
+                    QC[x] = kp.P[x] - xyz[2][x];
+                    AB[x] = xyz[0][x] - xyz[1][x];
+                    CD[x] = xyz[2][x] - xyz[3][x];
+                }
+                const double T = (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]) * p * qq * ipq;
+                const double al2 = 2.0 * bab.x, be2 = 2.0 * bab.y, ga2 = 2.0 * kab.x;
+#pragma unroll 1
+                for (int r = 0; r < NR; ++r) {
+                    double xr, wr;
+                    rys_eval_one<NR>(T, r, xr, wr, a.rys);
+                    const double fff = xr * ipq, B00 = 0.5 * fff;
+                    const double B1 = (0.5 - B00 * qq) * bp.ip, B1p = (0.5 - B00 * p) * kp.ip;
+                    Tab tb;
+                    for (int x = 0; x < 3; ++x) {
+                        double G[L12 + 1][L34 + 1];
+                        rys_vrr<double, L12, L34>(G, B00, B1, B1p, PA[x] - qq * PQ[x] * fff, QC[x] + p * PQ[x] * fff,
+                                                  x == 2 ? wr * sr : 1.0);
+                        double abp[LB + 2], cdp[LD + 1];
+                        abp[0] = 1.0;
+#pragma unroll
+                        for (int i = 1; i <= LB + 1; ++i) abp[i] = abp[i - 1] * AB[x];
+                        cdp[0] = 1.0;
+#pragma unroll
+                        for (int i = 1; i <= LD; ++i) cdp[i] = cdp[i - 1] * CD[x];
+                        static_for<LA + 2>([&](auto ia) {
+                            static_for<LB + 2>([&](auto ib_) {
+                                static_for<LC + 2>([&](auto ic) {
+                                    static_for<LD + 1>([&](auto id) {
+                                        constexpr int A_ = decltype(ia)::value, B_ = decltype(ib_)::value;
+                                        constexpr int C_ = decltype(ic)::value, D_ = decltype(id)::value;
+                                        if constexpr (A_ + B_ <= L12 && C_ + D_ <= L34) {
+                                            double s = 0.0;
+#pragma unroll
+                                            for (int i = 0; i <= B_; ++i) {
+                                                double t = 0.0;
+#pragma unroll
+                                                for (int j = 0; j <= D_; ++j)
+                                                    t += binom(D_, j) * cdp[j] * G[A_ + B_ - i][C_ + D_ - j];
+                                                s += binom(B_, i) * abp[i] * t;
+                                            }
+                                            tb.t[x][A_ * Tab::SA + B_ * Tab::SB + C_ * Tab::SC + D_] = s;
+                                        }
+                                    });
+                                });
+                            });
+                        });
+                    }
+                    // contraction of this root's derivative integrals with Gamma
+                    for (int ia = 0; ia < NA; ++ia) {
+                        const int ea = cart_exps<LA>(ia);
+                        const int ax = ea & 3, ay = (ea >> 2) & 3, az = ea >> 4;
+                        const double na = cart_norm_rt<LA>(ia);
+                        for (int ib2 = 0; ib2 < NB; ++ib2) {
+                            const int eb = cart_exps<LB>(ib2);
+                            const int bx = eb & 3, by = (eb >> 2) & 3, bz = eb >> 4;
+                            const double nab = na * cart_norm_rt<LB>(ib2);
+                            const double Pab = a.D[(size_t)(oI + ia) * N + oJ + ib2];
+                            for (int ic = 0; ic < NC; ++ic) {
+                                const int ec = cart_exps<LC>(ic);
+                                const int cx = ec & 3, cy = (ec >> 2) & 3, cz = ec >> 4;
+                                const double nabc = nab * cart_norm_rt<LC>(ic);
+                                const double Pac = a.D[(size_t)(oI + ia) * N + oK + ic];
+                                const double Pbc = a.D[(size_t)(oJ + ib2) * N + oK + ic];
+                                for (int id = 0; id < ND; ++id) {
+                                    const int ed = cart_exps<LD>(id);
+                                    const int dx = ed & 3, dy = (ed >> 2) & 3, dz = ed >> 4;
+                                    const double Pcd = a.D[(size_t)(oK + ic) * N + oL + id];
+                                    const double Pad = a.D[(size_t)(oI + ia) * N + oL + id];
+                                    const double Pbd = a.D[(size_t)(oJ + ib2) * N + oL + id];
+                                    const double gam = (2.0 * Pab * Pcd - 0.5 * (Pac * Pbd + Pad * Pbc)) * nabc *
+                                                       cart_norm_rt<LD>(id);
+                                    const int e[3][4] = {{ax, bx, cx, dx}, {ay, by, cy, dy}, {az, bz, cz, dz}};
+                                    double v[3];
+                                    for (int x = 0; x < 3; ++x) v[x] = tb.at(x, e[x][0], e[x][1], e[x][2], e[x][3]);
+                                    for (int x = 0; x < 3; ++x) {
+                                        const int y = x == 0 ? 1 : 0, z = x == 2 ? 1 : 2;
+                                        const double o = gam * v[y] * v[z];
+                                        const int ea_ = e[x][0], eb_ = e[x][1], ec_ = e[x][2], ed_ = e[x][3];
+                                        double da = al2 * tb.at(x, ea_ + 1, eb_, ec_, ed_);
+                                        if (ea_) da -= ea_ * tb.at(x, ea_ - 1, eb_, ec_, ed_);
+                                        double db = be2 * tb.at(x, ea_, eb_ + 1, ec_, ed_);
+                                        if (eb_) db -= eb_ * tb.at(x, ea_, eb_ - 1, ec_, ed_);
+                                        double dc = ga2 * tb.at(x, ea_, eb_, ec_ + 1, ed_);
+                                        if (ec_) dc -= ec_ * tb.at(x, ea_, eb_, ec_ - 1, ed_);
+                                        gA[x] += o * da;
+                                        gB[x] += o * db;
+                                        gC[x] += o * dc;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // per-shell sums: I, J, K in tile order, L by translational invariance
+        for (int x = 0; x < 3; ++x) {
+            fx_lds_add(&sg[3 * iq + x], gA[x]);
+            fx_lds_add(&sg[3 * (TI + jq) + x], gB[x]);
+            fx_lds_add(&sg[3 * (TI + TJ + kq) + x], gC[x]);
+            fx_lds_add(&sg[3 * (TI + TJ + TK + lq) + x], -(gA[x] + gB[x] + gC[x]));
+        }
+    }
+    __syncthreads();
+    // flush: one thread per (tile slot, axis) into the per-shell array
+    for (int e = tid; e < (TI + TJ + TK + TL) * 3; e += 256) {
+        const int slot = e / 3, x = e % 3;
+        int sh;
+        if (slot < TI) sh = slot < nI ? fI + slot : -1;
+        else if (slot < TI + TJ) sh = slot - TI < nJ ? fJ + slot - TI : -1;
+        else if (slot < TI + TJ + TK) sh = slot - TI - TJ < nK ? fK + slot - TI - TJ : -1;
+        else sh = slot - TI - TJ - TK < nL ? fL + slot - TI - TJ - TK : -1;
+        const FxAcc v = sg[e];
+        if (sh >= 0 && (v.hi != 0 || v.lo != 0)) fx_global_add(&a.gshell[3 * sh + x], v);
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+static void launch_grad_t(const GradArgs &args, hipStream_t s)
+{
+    if (args.nitems <= 0) return;
+    hipLaunchKernelGGL((grad2e_kernel<LA, LB, LC, LD, double>), dim3(args.nitems), dim3(256), 0, s, args);
+}
+
+int launch_grad(int bra_pc, int ket_pc, const GradArgs &args, hipStream_t s)
+{
+    switch (bra_pc * PM_NPC + ket_pc) {
+    case 0: launch_grad_t<0, 0, 0, 0>(args, s); return 0;
+    case 1: launch_grad_t<0, 0, 1, 0>(args, s); return 0;
+    case 3: launch_grad_t<0, 0, 2, 0>(args, s); return 0;
+    case 6: launch_grad_t<1, 0, 0, 0>(args, s); return 0;
+    case 7: launch_grad_t<1, 0, 1, 0>(args, s); return 0;
+    case 9: launch_grad_t<1, 0, 2, 0>(args, s); return 0;
+    case 12: launch_grad_t<1, 1, 0, 0>(args, s); return 0;
+    case 13: launch_grad_t<1, 1, 1, 0>(args, s); return 0;
+    case 14: launch_grad_t<1, 1, 1, 1>(args, s); return 0;
+    case 18: launch_grad_t<2, 0, 0, 0>(args, s); return 0;
+    case 19: launch_grad_t<2, 0, 1, 0>(args, s); return 0;
+    case 20: launch_grad_t<2, 0, 1, 1>(args, s); return 0;
+    case 21: launch_grad_t<2, 0, 2, 0>(args, s); return 0;
+    case 24: launch_grad_t<2, 1, 0, 0>(args, s); return 0;
+    case 25: launch_grad_t<2, 1, 1, 0>(args, s); return 0;
+    case 26: launch_grad_t<2, 1, 1, 1>(args, s); return 0;
+    case 27: launch_grad_t<2, 1, 2, 0>(args, s); return 0;
+    case 28: launch_grad_t<2, 1, 2, 1>(args, s); return 0;
+    case 30: launch_grad_t<2, 2, 0, 0>(args, s); return 0;
+    case 31: launch_grad_t<2, 2, 1, 0>(args, s); return 0;
+    case 32: launch_grad_t<2, 2, 1, 1>(args, s); return 0;
+    case 33: launch_grad_t<2, 2, 2, 0>(args, s); return 0;
+    case 34: launch_grad_t<2, 2, 2, 1>(args, s); return 0;
+    case 35: launch_grad_t<2, 2, 2, 2>(args, s); return 0;
+    default: return -1;
+    }
+}
+
+}  // namespace pm

@@ -24,6 +24,7 @@ static constexpr int kTableNC = RYS_NC;
 #undef RYS_NMAX
 
 #include "eri_kernels.h"
+#include "grad.h"
 #include "patmol_hip.h"
 
 extern "C" {
@@ -203,6 +204,7 @@ struct Ctx {
     DevBuf<int> d_off, d_soff, d_fperm, d_finv, d_orig, d_tile_first, d_tile_size;
     DevBuf<PairRef> d_pairs;
     DevBuf<PrimPair> d_pp;
+    DevBuf<double2> d_ppab;    // primitive exponents of each primitive pair (gradient)
     DevBuf<int4> d_items;
     // work buffers
     DevBuf<double> d_P, d_PB, d_D, d_D2, d_DJ, d_G, d_G2, d_dmax;
@@ -325,6 +327,7 @@ static int build_tables(Ctx &c, int start)
     const int64_t np = (int64_t)ns * (ns + 1) / 2;
     c.pairs.assign(np, PairRef{0, 0});
     c.pp.clear();
+    std::vector<double2> ppab;
     for (int I = 0; I < ns; ++I) {
         const Shell &a = sh[c.sorted_orig[I]];
         const double *A = ce[a.cen].r;
@@ -353,6 +356,7 @@ static int build_tables(Ctx &c, int start)
                     q.c = a.co[i] * b.co[j];
                     q.ip = 1.0 / p;
                     c.pp.push_back(q);
+                    ppab.push_back(make_double2(al, be));
                 }
         }
     }
@@ -372,7 +376,7 @@ static int build_tables(Ctx &c, int start)
     for (int i = 0; i < ns; ++i) orig_sorted_arr[i] = c.sorted_orig[i];
     if (c.d_xyz.upload(c.shell_xyz) || c.d_off.upload(c.shell_off) || c.d_soff.upload(c.shell_soff) ||
         c.d_fperm.upload(c.fn_sorted_to_orig) || c.d_finv.upload(c.fn_orig_to_sorted) || c.d_orig.upload(orig_sorted_arr) ||
-        c.d_pairs.upload(c.pairs) || c.d_pp.upload(c.pp) || c.d_tile_first.upload(c.tile_first) ||
+        c.d_pairs.upload(c.pairs) || c.d_pp.upload(c.pp) || c.d_ppab.upload(ppab) || c.d_tile_first.upload(c.tile_first) ||
         c.d_tile_size.upload(c.tile_size))
         return g_err;
     // Schwarz bounds on the device
@@ -869,6 +873,7 @@ void clean_ERIFS(ERIFS *e)
             b->release();
         c->d_pairs.release();
         c->d_pp.release();
+        c->d_ppab.release();
         c->d_items.release();
         for (auto st : c->side) (void)hipStreamDestroy(st);
         for (auto ev : c->side_ev) (void)hipEventDestroy(ev);
@@ -1155,6 +1160,43 @@ static int class_stats_impl(Ctx &c, int64_t *q36, double *f36)
         }
     }
     c.prim_kept = pkept_all;
+    return 0;
+}
+
+// Two-electron part of the RHF nuclear gradient over the handle's screened
+// work list (grad.hip): grad[3 c + x] += sum Gamma d(ab|cd)/dR_cx for
+// centre c, with the per-shell fixed-point sums converted and added per
+// centre in shell order on the host (reproducible).
+int patmol_hip_eri_gradient(const ERIFS *e, const double *Pmat, double *grad)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *cp = ctx_of(e);
+    if (!cp || !Pmat || !grad) return fail(PATMOL_HIP_EBADARG, "patmol_hip_eri_gradient: bad handle or null array");
+    if (!cp->built) return fail(PATMOL_HIP_ESTATE, "two_electron_ints was not run for this handle");
+    Ctx &c = *cp;
+    const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
+    if (alloc_work(c, false)) return g_err;
+    DevBuf<FxAcc> gsh;
+    if (gsh.alloc((size_t)3 * c.nshell)) return g_err;
+    HIPCHK(hipMemcpy(c.d_P.p, Pmat, sizeof(double) * no2, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, 0, c.d_P.p, c.d_D.p, c.d_fperm.p, c.N);
+    hipLaunchKernelGGL(zero_fx_kernel, dim3((unsigned)((3 * c.nshell + 255) / 256)), dim3(256), 0, 0, gsh.p,
+                       (int64_t)3 * c.nshell);
+    for (const ClassRange &r : c.ranges) {
+        GradArgs a{c.d_items.p + r.first, r.count, c.d_tile_first.p, c.d_tile_size.p, c.d_xyz.p, c.d_soff.p,
+                   c.d_pairs.p, c.d_q.p, c.d_pp.p, c.d_ppab.p, c.d_D.p, gsh.p, c.N, c.tau, g_rys};
+        if (launch_grad(r.bra_pc, r.ket_pc, a, 0))
+            return fail(PATMOL_HIP_EUNSUPPORTED, "no gradient kernel for class %d/%d", r.bra_pc, r.ket_pc);
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<FxAcc> h((size_t)3 * c.nshell);
+    HIPCHK(hipMemcpy(h.data(), gsh.p, sizeof(FxAcc) * h.size(), hipMemcpyDeviceToHost));
+    gsh.release();
+    const Shell *sh = shells();
+    for (int sidx = 0; sidx < c.nshell; ++sidx) {
+        const int cen = sh[c.sorted_orig[sidx]].cen;
+        for (int x = 0; x < 3; ++x) grad[3 * cen + x] += fx_value(h[3 * sidx + x].hi, h[3 * sidx + x].lo);
+    }
     return 0;
 }
 

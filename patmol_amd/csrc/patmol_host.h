@@ -66,6 +66,18 @@ int patmol_uhf_moments(const double *PA, const double *PB, double *elec, double 
  * alfa[5] = xx, yy, zz, mean, and the reference's alfa2 */
 int patmol_rhf_finite_field(double efx, double *alfa, PatmolScfResult *gs);
 
+/* RHF nuclear gradient (no counterpart in the reference): grad[3 c + x] =
+ * dE/dR_{c,x} for every centre at the converged half density P (packed),
+ * E = 2 sum P H + sum P G(P) + E_nuc; two-electron part on the device
+ * (patmol_hip_eri_gradient over mints, or over a handle built here when
+ * mints is null), one-electron and nuclear-repulsion parts on the host with
+ * W = P F P.  Returns 0 or an error code. */
+int patmol_rhf_gradient(const ERIFS *mints, const double *P, double *grad);
+/* its host parts: grad += 2 sum P dH/dR - 2 sum W dS/dR (P, W packed), and
+ * grad += dE_nuc/dR */
+int patmol_host_onee_gradient(const double *P, const double *W, double *grad);
+void patmol_nuclear_repulsion_gradient(double *grad);
+
 /* DPM grid loop, rhf_dpm_calculation / uhf_dpm_calculation (rhfscf.c:217-263,
  * uhfscf.c:213-262), on the device SCF.  The DPM centre and shells must have
  * been added (patmol_basis_add_dpm_center, or init_Basis with the dpm flag)

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+# torch (where installed) is imported before any test loads the library:
+# a torch imported after libpatmol_hip.so has loaded the HIP runtime binds
+# its extension libraries to that runtime and crashes
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -173,6 +173,46 @@ int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int nocc, doubl
 int patmol_hip_uhf_gmatrix_device(const ERIFS *tfiles, const double *d_PA, const double *d_PB, double *d_GA,
                                   double *d_GB, void *stream);
 
+/* ---- molecule batches ---- */
+/* No counterpart in the reference (it treats one molecule per run).  Many
+ * independent molecules of uneven size in one G build: each
+ * patmol_hip_batch_add snapshots the basis current at the call (the
+ * basis.h accessors, as two_electron_ints reads them, with the current
+ * Schwarz threshold) and returns its molecule index; patmol_hip_batch_finalize
+ * merges every molecule's screened work list into one queue per
+ * angular-momentum class (work items never mix molecules), splits the items
+ * whose bound x dmax is below `threshold` to the FP32 kernels when
+ * prec_mode is 1, and captures the whole G build (unpack, zeroing, one launch
+ * per class over all molecules, finalisation) as a HIP graph.  Densities and
+ * G of the batch are the molecules' packed triangles concatenated in the
+ * order they were added.  patmol_hip_batch_gmatrix adds 2J[P] - K[P] of
+ * every molecule to host G (use_graph 0 issues the same launches without the
+ * graph); patmol_hip_batch_buffers / patmol_hip_batch_launch give device
+ * callers the batch's own P and G buffers and one graph replay on their
+ * stream.  The J/K sums are fixed point: results are bitwise reproducible
+ * and independent of the other molecules of the batch. */
+typedef struct PatmolHipBatch PatmolHipBatch;
+PatmolHipBatch *patmol_hip_batch_new(void);
+int patmol_hip_batch_add(PatmolHipBatch *b);
+int patmol_hip_batch_finalize(PatmolHipBatch *b, int prec_mode, double threshold, double dmax);
+int patmol_hip_batch_gmatrix(PatmolHipBatch *b, const double *P, double *G, int use_graph);
+int patmol_hip_batch_buffers(PatmolHipBatch *b, double **d_P, double **d_G);
+int patmol_hip_batch_launch(PatmolHipBatch *b, void *stream);
+int patmol_hip_batch_info(PatmolHipBatch *b, int *nmol, int64_t *no2tot, int64_t *items, int64_t *items_fp32);
+void patmol_hip_batch_free(PatmolHipBatch *b);
+/* RHF of every molecule of a finalised batch, all iterating together: per
+ * iteration one replay of the batch's G graph, then per molecule not yet
+ * converged the density step of patmol_hip_rhf_scf (DIIS for scf_accel 3,
+ * else the plain Roothaan step; scf_accel 1 and 2 are not supported here)
+ * and the reference's convergence test (rhfscf.c:52-59, per molecule).
+ * S, H: host packed one-electron matrices of the molecules concatenated
+ * (batch order); nocc, enuc per molecule; outputs per molecule (energy,
+ * iterations, converged) and optionally the densities (concatenated);
+ * total_ms / gbuild_ms: wall time of the loop and of its G builds. */
+int patmol_hip_batch_rhf_scf(PatmolHipBatch *b, const double *S, const double *H, const int *nocc,
+                             const double *enuc, int maxits, double scf_eps, int scf_accel, double *energy,
+                             int *iterations, int *converged, double *Pout, double *total_ms, double *gbuild_ms);
+
 /* ---- nuclear gradient ---- */
 /* No counterpart in the reference (it has no gradients).  The two-electron
  * part of the RHF energy gradient for the packed half density P (host,

@@ -167,6 +167,24 @@ def lib():
         H.patmol_nuclear_repulsion_gradient.argtypes = [_dp]
         G.patmol_hip_eri_gradient.restype = C.c_int
         G.patmol_hip_eri_gradient.argtypes = [C.c_void_p, _dp, _dp]
+        G.patmol_hip_batch_new.restype = C.c_void_p
+        G.patmol_hip_batch_free.argtypes = [C.c_void_p]
+        G.patmol_hip_batch_add.restype = C.c_int
+        G.patmol_hip_batch_add.argtypes = [C.c_void_p]
+        G.patmol_hip_batch_finalize.restype = C.c_int
+        G.patmol_hip_batch_finalize.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        G.patmol_hip_batch_gmatrix.restype = C.c_int
+        G.patmol_hip_batch_gmatrix.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        G.patmol_hip_batch_buffers.restype = C.c_int
+        G.patmol_hip_batch_buffers.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+        G.patmol_hip_batch_launch.restype = C.c_int
+        G.patmol_hip_batch_launch.argtypes = [C.c_void_p, C.c_void_p]
+        G.patmol_hip_batch_info.restype = C.c_int
+        G.patmol_hip_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        G.patmol_hip_batch_rhf_scf.restype = C.c_int
+        G.patmol_hip_batch_rhf_scf.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, C.c_int, C.c_double, C.c_int, _dp, _ip,
+                                               _ip, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         H.patmol_dpm_calculation.restype = C.c_int
         H.patmol_dpm_calculation.argtypes = [C.c_int, C.c_int, _dp, C.c_void_p, C.POINTER(ScfResult)]
         _host, _hip = H, G
@@ -191,7 +209,10 @@ def exported_symbols():
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
             "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
             "patmol_hip_write_erints", "patmol_hip_rhf_gmatrix_erints", "patmol_hip_rhf_gmatrix_erint_file",
-            "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file", "patmol_hip_eri_gradient"]
+            "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file", "patmol_hip_eri_gradient",
+            "patmol_hip_batch_new", "patmol_hip_batch_add", "patmol_hip_batch_finalize", "patmol_hip_batch_gmatrix",
+            "patmol_hip_batch_buffers", "patmol_hip_batch_launch", "patmol_hip_batch_info", "patmol_hip_batch_free",
+            "patmol_hip_batch_rhf_scf"]
 
 
 def load_system(sys_: "systems.System") -> None:
@@ -657,3 +678,90 @@ def nuclear_repulsion_gradient() -> np.ndarray:
     g = np.zeros(3 * nc)
     H.patmol_nuclear_repulsion_gradient(g)
     return g.reshape(nc, 3)
+
+
+class Batch:
+    """Many independent molecules in one G build (patmol_hip_batch_*): add
+    each system (its basis is loaded, its S and H computed on the GPU and its
+    screened tables snapshotted), finalize (one work queue per class, FP32
+    split, the G build captured as a HIP graph), then G builds or the RHF of
+    every molecule iterating together."""
+
+    def __init__(self):
+        _, G = lib()
+        self.h = G.patmol_hip_batch_new()
+        self.systems, self.S, self.H = [], [], []
+
+    def add(self, system) -> int:
+        _, G = lib()
+        load_system(system)
+        S, _, Hm = one_electron_ints()
+        idx = G.patmol_hip_batch_add(self.h)
+        if idx < 0:
+            _check("patmol_hip_batch_add")
+        self.systems.append(system)
+        self.S.append(S)
+        self.H.append(Hm)
+        return idx
+
+    def finalize(self, precision: int = 0, threshold: float = 1e-6, dmax: float = 1.0) -> None:
+        _, G = lib()
+        if G.patmol_hip_batch_finalize(self.h, int(precision), threshold, dmax):
+            _check("patmol_hip_batch_finalize")
+
+    def offsets(self):
+        o = [0]
+        for s in self.systems:
+            o.append(o[-1] + s.norb * (s.norb + 1) // 2)
+        return o
+
+    def gmatrix(self, P, use_graph: bool = True) -> np.ndarray:
+        _, L = lib()
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        G = np.zeros_like(P)
+        if L.patmol_hip_batch_gmatrix(self.h, P, G, int(use_graph)):
+            _check("patmol_hip_batch_gmatrix")
+        return G
+
+    def buffers(self):
+        _, L = lib()
+        p, g = C.c_void_p(), C.c_void_p()
+        if L.patmol_hip_batch_buffers(self.h, C.byref(p), C.byref(g)):
+            _check("patmol_hip_batch_buffers")
+        return p.value, g.value
+
+    def launch(self, stream: int = 0) -> None:
+        _, L = lib()
+        if L.patmol_hip_batch_launch(self.h, C.c_void_p(stream)):
+            _check("patmol_hip_batch_launch")
+
+    def info(self) -> dict:
+        _, L = lib()
+        nm, n2, it, i32 = C.c_int(), C.c_int64(), C.c_int64(), C.c_int64()
+        L.patmol_hip_batch_info(self.h, C.byref(nm), C.byref(n2), C.byref(it), C.byref(i32))
+        return {"molecules": nm.value, "no2_total": n2.value, "work_items": it.value, "work_items_fp32": i32.value}
+
+    def rhf_scf(self, maxits: int = 200, eps: float = 1e-10, accel: int = 3):
+        """(energies, iterations, converged, P concatenated, total ms, G-build ms)."""
+        _, L = lib()
+        nm = len(self.systems)
+        S, Hm = np.concatenate(self.S), np.concatenate(self.H)
+        nocc = np.array([s.nelec // 2 for s in self.systems], dtype=np.int32)
+        enuc = np.array([s.nuclear_repulsion() for s in self.systems])
+        e, it, cv = np.zeros(nm), np.zeros(nm, np.int32), np.zeros(nm, np.int32)
+        P = np.zeros(len(S))
+        tot, gms = C.c_double(), C.c_double()
+        rc = L.patmol_hip_batch_rhf_scf(self.h, S, Hm, nocc, enuc, maxits, eps, accel, e, it, cv, P.ctypes.data,
+                                        C.byref(tot), C.byref(gms))
+        if rc:
+            _check("patmol_hip_batch_rhf_scf")
+            raise PatmolError(f"patmol_hip_batch_rhf_scf failed ({rc})")
+        return e, it, cv, P, tot.value, gms.value
+
+    def close(self):
+        if getattr(self, "h", None) and _hip is not None:
+            _hip.patmol_hip_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -219,7 +219,21 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     const int fK = a.tile_first[it.z], nK = a.tile_size[it.z];
     const int fL = a.tile_first[it.w], nL = a.tile_size[it.w];
     const bool sameBra = it.x == it.y, sameKet = it.z == it.w, samePair = it.x == it.z && it.y == it.w;
-    const int N = a.N;
+    // the item's molecule (batches) or the single basis
+    int N = a.N, sb = 0;
+    long long pbase = 0, dadj = 0;
+    if (a.item_mol) {
+        const MolDesc m = a.mols[a.item_mol[item]];
+        N = m.N;
+        sb = m.sbase;
+        pbase = m.pbase;
+        dadj = m.dadj;
+    }
+    // pair table index of sorted shells hi >= lo
+    auto pidx = [&](int hi, int lo) -> long long {
+        const long long h = hi - sb, l = lo - sb;
+        return pbase + h * (h + 1) / 2 + l;
+    };
 
     // ---- stage A: function offsets, centres, ket pairs, Rys table ----
     if (tid < TI) sOff[tid] = a.shell_off[fI + (tid < nI ? tid : 0)];
@@ -244,7 +258,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         // a structurally valid stand-in pair for lanes whose (K, L) does not exist
         const int Ls = ok ? L : (sameKet ? K : fL);
         const int Kp = K >= Ls ? K : Ls, Lp = K >= Ls ? Ls : K;
-        const int pk = Kp * (Kp + 1) / 2 + Lp;
+        const long long pk = pidx(Kp, Lp);
         PairRef r = {0, 0};
         if (here) {
             r = a.pairs[pk];
@@ -301,7 +315,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 const int rs = r / nrc, cs = c / ncc;
                 const bool ex = c < cn * ncc && rs < rn && cs < cn;
                 double v = 0.0;
-                if (ex) v = src[(size_t)(sOff[rsh + rs] + r % nrc) * N + sOff[csh + cs] + c % ncc];
+                if (ex) v = src[dadj + (long long)(sOff[rsh + rs] + r % nrc) * N + sOff[csh + cs] + c % ncc];
                 sD[o + e] = v;
             }
         };
@@ -322,7 +336,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     const bool bvalid = iexist && jexist && (!sameBra || I >= J);
     const int Is = bvalid ? I : fI, Js = bvalid ? J : fJ;
     const int Ip = Is >= Js ? Is : Js, Jp = Is >= Js ? Js : Is;
-    const int pb = Ip * (Ip + 1) / 2 + Jp;
+    const long long pb = pidx(Ip, Jp);
     const double qbra = a.pair_q[pb];
     const PairRef bref = a.pairs[pb];
     PairGeom<R> bg;
@@ -529,7 +543,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     }
 
     // ---- flush: copies summed in copy order, fixed point, this XCD's J'/K' ----
-    const size_t xoff = (size_t)xcd_slot() * N * N;
+    const long long xoff = (long long)xcd_slot() * a.slot_stride + dadj;
     FxAcc *const Jx = a.J + xoff;
     FxAcc *const Kx = a.K + xoff;
     auto flush = [&](int o, int copies, int size, int rows, int stride, int cols, int rsh, int nrc, int csh, int ncc,
@@ -540,7 +554,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             double v = sO[o + e];
             for (int k = 1; k < copies; ++k) v += sO[o + k * size + e];
             if (v == 0.0) continue;
-            fx_flush(&dst[(size_t)(sOff[rsh + r / nrc] + r % nrc) * N + sOff[csh + c / ncc] + c % ncc], fx_split(v));
+            fx_flush(&dst[(long long)(sOff[rsh + r / nrc] + r % nrc) * N + sOff[csh + c / ncc] + c % ncc], fx_split(v));
         }
     };
     flush(pIJ, 1, ZIJ, FI, SJ, FJ, 0, NA, TI, NB, Jx);

@@ -59,6 +59,17 @@ __host__ __device__ inline double fx_value(long long hi, long long lo)
     return (double)hi * (1.0 / FX_HI_SCALE) + (double)lo * (1.0 / (FX_HI_SCALE * FX_LO_SCALE));
 }
 
+// One molecule of a batch: the batch's work items never mix molecules, and
+// each carries its molecule's index.  Element (row, col) of the molecule's
+// D, J', K' (rows and columns numbered by the batch's global sorted
+// functions) is at dadj + row N + col; its shell pairs start at pbase.
+struct MolDesc {
+    int N;            // functions of the molecule (row stride of its blocks)
+    int sbase;        // first sorted shell of the molecule in the batch
+    long long pbase;  // its first shell pair in the pair tables
+    long long dadj;   // element offset of its blocks (see above)
+};
+
 struct JKArgs {
     const int4 *items;         // (tI, tJ, tK, tL): bra tile pair first
     int nitems;
@@ -77,6 +88,10 @@ struct JKArgs {
     double tau;                // Schwarz threshold on Q_IJ Q_KL
     int start_shell;           // keep quartets whose largest original shell index >= start_shell
     RysTables rys;
+    // batches (null item_mol: one molecule, N x N blocks at offset 0)
+    const int *item_mol;       // molecule of each item
+    const MolDesc *mols;
+    long long slot_stride;     // elements of one per-XCD copy of J' (and of K')
 };
 
 struct EriOutArgs {

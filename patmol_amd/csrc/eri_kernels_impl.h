@@ -157,8 +157,20 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     const int fK = a.tile_first[it.z], nK = a.tile_size[it.z];
     const int fL = a.tile_first[it.w], nL = a.tile_size[it.w];
     const bool sameBra = it.x == it.y, sameKet = it.z == it.w, samePair = it.x == it.z && it.y == it.w;
-    const int N = a.N;
-    const size_t xoff = (size_t)xcd_slot() * N * N;
+    int N = a.N, sb = 0;
+    long long pbase = 0, dadj = 0;
+    if (a.item_mol) {
+        const MolDesc m = a.mols[a.item_mol[item]];
+        N = m.N;
+        sb = m.sbase;
+        pbase = m.pbase;
+        dadj = m.dadj;
+    }
+    auto pidx = [&](int hi, int lo) -> long long {
+        const long long h = hi - sb, l = lo - sb;
+        return pbase + h * (h + 1) / 2 + l;
+    };
+    const long long xoff = (long long)xcd_slot() * a.slot_stride + dadj;
     FxAcc *const Jx = a.J + xoff;
     FxAcc *const Kx = a.K + xoff;
 
@@ -168,7 +180,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     const bool kvalid = kexist && lexist && (!sameKet || K >= L);
     const int Ks = kvalid ? K : fK, Ls = kvalid ? L : fL;  // safe indices
     const int Kp = Ks >= Ls ? Ks : Ls, Lp = Ks >= Ls ? Ls : Ks;
-    const int pk = Kp * (Kp + 1) / 2 + Lp;
+    const long long pk = pidx(Kp, Lp);
     const double qket = a.pair_q[pk];
     const PairRef kref = a.pairs[pk];
     PairGeom<R> kg;
@@ -194,7 +206,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         const bool jvalid = jexist && (!sameBra || I >= J);
         const int Js = jvalid ? J : fJ;
         const int Ip = I >= Js ? I : Js, Jp = I >= Js ? Js : I;
-        const int pb = Ip * (Ip + 1) / 2 + Jp;
+        const long long pb = pidx(Ip, Jp);
         const double qbra = a.pair_q[pb];
         bool valid = kvalid && jvalid;
         if (samePair) valid = valid && (I > K || (I == K && J >= L));
@@ -217,8 +229,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             bg.AB[x] = R(a.shell_xyz[3 * I + x] - a.shell_xyz[3 * Js + x]);
         }
         const int offI = a.shell_off[I], offJ = a.shell_off[Js];
-        const double *__restrict__ DJ = a.DJ;
-        const double *__restrict__ DK = a.DK;
+        const double *__restrict__ DJ = a.DJ + dadj;
+        const double *__restrict__ DK = a.DK + dadj;
 
         // primitive loops: bounds are made wave-uniform so the lane reductions
         // inside the component loop see every lane

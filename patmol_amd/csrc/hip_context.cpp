@@ -23,6 +23,7 @@ static constexpr int kTableNC = RYS_NC;
 #undef RYS_NC
 #undef RYS_NMAX
 
+#include "batch.h"
 #include "eri_kernels.h"
 #include "grad.h"
 #include "patmol_hip.h"
@@ -195,6 +196,7 @@ struct Ctx {
     std::vector<int> tile_first, tile_size, tile_l;
     std::vector<PairRef> pairs;
     std::vector<PrimPair> pp;
+    std::vector<double2> ppab;   // exponents of each primitive pair (I's, J's)
     std::vector<double> pair_q;
     std::vector<int4> items;
     std::vector<double> item_bound;
@@ -327,7 +329,8 @@ static int build_tables(Ctx &c, int start)
     const int64_t np = (int64_t)ns * (ns + 1) / 2;
     c.pairs.assign(np, PairRef{0, 0});
     c.pp.clear();
-    std::vector<double2> ppab;
+    c.ppab.clear();
+    std::vector<double2> &ppab = c.ppab;
     for (int I = 0; I < ns; ++I) {
         const Shell &a = sh[c.sorted_orig[I]];
         const double *A = ce[a.cen].r;
@@ -634,6 +637,9 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         a.tau = c.tau;
         a.start_shell = c.start;
         a.rys = g_rys;
+        a.item_mol = nullptr;
+        a.mols = nullptr;
+        a.slot_stride = (long long)c.N * c.N;
         if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
         if (n64 > 0) {
             a.items = c.d_items.p + r.first;
@@ -797,6 +803,33 @@ static int uhf_device(Ctx &x, const double *dPA, const double *dPB, double *dGA,
     return 0;
 }
 
+// device memory, streams and events of a context
+static void release_ctx(Ctx &c)
+{
+    for (auto *b : {&c.d_xyz, &c.d_q, &c.d_bound, &c.d_P, &c.d_PB, &c.d_D, &c.d_D2, &c.d_DJ, &c.d_G, &c.d_G2, &c.d_dmax})
+        b->release();
+    c.d_J.release();
+    c.d_K.release();
+    for (auto *b : {&c.d_off, &c.d_soff, &c.d_fperm, &c.d_finv, &c.d_orig, &c.d_tile_first, &c.d_tile_size}) b->release();
+    c.d_pairs.release();
+    c.d_pp.release();
+    c.d_ppab.release();
+    c.d_items.release();
+    for (auto st : c.side) (void)hipStreamDestroy(st);
+    for (auto ev : c.side_ev) (void)hipEventDestroy(ev);
+    c.side.clear();
+    c.side_ev.clear();
+    if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
+    c.fork_ev = nullptr;
+    if (c.ev0) {
+        (void)hipEventDestroy(c.ev0);
+        (void)hipEventDestroy(c.ev1);
+        for (auto ev : c.cls_ev) (void)hipEventDestroy(ev);
+        c.ev0 = c.ev1 = nullptr;
+        c.cls_ev.clear();
+    }
+}
+
 int aux_fail(int code, const char *msg) { return fail(code, "%s", msg); }
 
 RysTables aux_tables()
@@ -805,6 +838,192 @@ RysTables aux_tables()
     std::lock_guard<std::mutex> lk(g_mu);
     if (ensure_tables()) return RysTables{nullptr, nullptr, nullptr, nullptr};
     return g_rys;
+}
+
+// ---------------------------------------------------------------------------
+// molecule batches: every molecule's tables (as two_electron_ints builds
+// them) concatenated; work items never mix molecules and carry their
+// molecule's index, so each class is one launch over all molecules; the
+// whole G build (unpack, zero, class launches, finalise) is captured once as
+// a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+struct BatchMol {
+    int N = 0, nshell = 0;
+    int sbase = 0, fbase = 0, tbase = 0;
+    long long pbase = 0, ppbase = 0, dstart = 0, pstart = 0;
+};
+
+// D of every molecule (its block at dstart, rows and columns in its sorted
+// function order) from the packed half densities (at pstart, reference
+// order); fperm: local original function of each global sorted function
+__global__ void batch_unpack_kernel(const double *__restrict__ P, double *__restrict__ D,
+                                    const long long *__restrict__ dstart, const long long *__restrict__ pstart,
+                                    const int *__restrict__ Nm, const int *__restrict__ fbase,
+                                    const int *__restrict__ fperm, int nmol, long long dtot)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= dtot) return;
+    int lo = 0, hi = nmol - 1;   // last molecule with dstart <= idx
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (dstart[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    const int m = lo, N = Nm[m];
+    const long long e = idx - dstart[m];
+    const int i = fperm[fbase[m] + (int)(e / N)], j = fperm[fbase[m] + (int)(e % N)];
+    const int a = i >= j ? i : j, b = i >= j ? j : i;
+    D[idx] = P[pstart[m] + (long long)a * (a + 1) / 2 + b];
+}
+
+// G = cj (J' + J'^T) - (K' + K'^T) per molecule into the packed triangles;
+// finv: local sorted index of each local original function (at fbase)
+__global__ void batch_finalize_kernel(const FxAcc *__restrict__ J, const FxAcc *__restrict__ K, double *__restrict__ G,
+                                      const long long *__restrict__ dstart, const long long *__restrict__ pstart,
+                                      const int *__restrict__ Nm, const int *__restrict__ fbase,
+                                      const int *__restrict__ finv, int nmol, long long no2tot, long long dtot,
+                                      double cj)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= no2tot) return;
+    int lo = 0, hi = nmol - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pstart[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    const int m = lo, N = Nm[m];
+    const long long ij = idx - pstart[m];
+    int i = (int)((sqrt(8.0 * (double)ij + 1.0) - 1.0) * 0.5);
+    while ((long long)(i + 1) * (i + 2) / 2 <= ij) ++i;
+    while ((long long)i * (i + 1) / 2 > ij) --i;
+    const int j = (int)(ij - (long long)i * (i + 1) / 2);
+    const int si = finv[fbase[m] + i], sj = finv[fbase[m] + j];
+    const long long a = dstart[m] + (long long)si * N + sj, b = dstart[m] + (long long)sj * N + si;
+    long long jh = 0, jl = 0, kh = 0, kl = 0;
+    for (int x = 0; x < PM_XCD_SLOTS; ++x) {
+        const FxAcc ja = J[x * dtot + a], jb = J[x * dtot + b], ka = K[x * dtot + a], kb = K[x * dtot + b];
+        jh += ja.hi + jb.hi;
+        jl += ja.lo + jb.lo;
+        kh += ka.hi + kb.hi;
+        kl += ka.lo + kb.lo;
+    }
+    G[idx] = cj * fx_value(jh, jl) - fx_value(kh, kl);
+}
+
+__global__ void zero_kernel(double *__restrict__ p, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+}  // namespace pm
+
+struct PatmolHipBatch {
+    std::vector<pm::BatchMol> mol;
+    std::vector<double> xyz, pair_q;
+    std::vector<int> soff, orig, tile_first, tile_size, fperm, finv;
+    std::vector<pm::PairRef> pairs;
+    std::vector<pm::PrimPair> pp;
+    struct It {
+        int4 t;
+        int mol, bpc, kpc;
+        double bound;
+    };
+    std::vector<It> its;
+    long long dtot = 0, no2tot = 0;
+    int nshell = 0, norb = 0, ntile = 0;
+    // after finalize
+    bool finalized = false;
+    std::vector<pm::ClassRange> ranges;
+    std::vector<int> n64;   // FP64 items at the front of each range
+    std::vector<int> Nvec;
+    std::vector<int64_t> pstart_v;
+    pm::DevBuf<double> d_xyz, d_q, d_P, d_G, d_D;
+    pm::DevBuf<int> d_soff, d_orig, d_tile_first, d_tile_size, d_fperm, d_finv, d_item_mol, d_N, d_fbase;
+    pm::DevBuf<long long> d_dstart, d_pstart;
+    pm::DevBuf<pm::PairRef> d_pairs;
+    pm::DevBuf<pm::PrimPair> d_pp;
+    pm::DevBuf<int4> d_items;
+    pm::DevBuf<pm::MolDesc> d_mols;
+    pm::DevBuf<pm::FxAcc> d_J, d_K;
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t items_fp32 = 0;
+    double tau = 1e-12;
+};
+
+namespace pm {
+
+// the G build of a finalised batch: P (d_P) in, G (d_G) out, on stream s
+static int batch_build(PatmolHipBatch &b, hipStream_t s)
+{
+    const long long dtot = b.dtot, n2 = b.no2tot, nfx = dtot * PM_XCD_SLOTS;
+    const int nm = (int)b.mol.size();
+    hipLaunchKernelGGL(batch_unpack_kernel, dim3((unsigned)((dtot + 255) / 256)), dim3(256), 0, s, b.d_P.p, b.d_D.p,
+                       b.d_dstart.p, b.d_pstart.p, b.d_N.p, b.d_fbase.p, b.d_fperm.p, nm, dtot);
+    hipLaunchKernelGGL(zero_fx_kernel, dim3((unsigned)((nfx + 255) / 256)), dim3(256), 0, s, b.d_J.p, (int64_t)nfx);
+    hipLaunchKernelGGL(zero_fx_kernel, dim3((unsigned)((nfx + 255) / 256)), dim3(256), 0, s, b.d_K.p, (int64_t)nfx);
+    for (size_t ri = 0; ri < b.ranges.size(); ++ri) {
+        const ClassRange &r = b.ranges[ri];
+        JKArgs a;
+        a.tile_first = b.d_tile_first.p;
+        a.tile_size = b.d_tile_size.p;
+        a.shell_xyz = b.d_xyz.p;
+        a.shell_off = b.d_soff.p;
+        a.shell_orig = b.d_orig.p;   // read by the kernels, used only for start_shell > 0
+        a.pairs = b.d_pairs.p;
+        a.pair_q = b.d_q.p;
+        a.pp = b.d_pp.p;
+        a.DJ = b.d_D.p;
+        a.DK = b.d_D.p;
+        a.J = b.d_J.p;
+        a.K = b.d_K.p;
+        a.N = 0;
+        a.tau = b.tau;
+        a.start_shell = 0;
+        a.rys = g_rys;
+        a.item_mol = b.d_item_mol.p + r.first;
+        a.mols = b.d_mols.p;
+        a.slot_stride = dtot;
+        const int n64 = b.n64[ri];
+        if (n64 > 0) {
+            a.items = b.d_items.p + r.first;
+            a.nitems = n64;
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 0, s)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel");
+        }
+        if (r.count > n64) {
+            a.items = b.d_items.p + r.first + n64;
+            a.item_mol = b.d_item_mol.p + r.first + n64;
+            a.nitems = r.count - n64;
+            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, s)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel");
+        }
+    }
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, b.d_G.p, n2);
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, b.d_J.p, b.d_K.p,
+                       b.d_G.p, b.d_dstart.p, b.d_pstart.p, b.d_N.p, b.d_fbase.p, b.d_finv.p, nm, n2, dtot, 2.0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+bool batch_view(PatmolHipBatch *b, BatchView &v)
+{
+    if (!b || !b->finalized) return false;
+    v.nmol = (int)b->mol.size();
+    v.N = b->Nvec.data();
+    v.pstart = b->pstart_v.data();
+    v.no2tot = b->no2tot;
+    v.d_P = b->d_P.p;
+    v.d_G = b->d_G.p;
+    return true;
+}
+
+int batch_launch(PatmolHipBatch *b, hipStream_t s)
+{
+    if (!b || !b->finalized || !b->exec) return fail(PATMOL_HIP_ESTATE, "batch not finalised");
+    HIPCHK(hipGraphLaunch(b->exec, s));
+    return 0;
 }
 
 }  // namespace pm
@@ -864,25 +1083,7 @@ void clean_ERIFS(ERIFS *e)
         }
     }
     if (c) {
-        for (auto *b : {&c->d_xyz, &c->d_q, &c->d_bound, &c->d_P, &c->d_PB, &c->d_D, &c->d_D2, &c->d_DJ, &c->d_G,
-                        &c->d_G2, &c->d_dmax})
-            b->release();
-        c->d_J.release();
-        c->d_K.release();
-        for (auto *b : {&c->d_off, &c->d_soff, &c->d_fperm, &c->d_finv, &c->d_orig, &c->d_tile_first, &c->d_tile_size})
-            b->release();
-        c->d_pairs.release();
-        c->d_pp.release();
-        c->d_ppab.release();
-        c->d_items.release();
-        for (auto st : c->side) (void)hipStreamDestroy(st);
-        for (auto ev : c->side_ev) (void)hipEventDestroy(ev);
-        if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
-        if (c->ev0) {
-            (void)hipEventDestroy(c->ev0);
-            (void)hipEventDestroy(c->ev1);
-            for (auto ev : c->cls_ev) (void)hipEventDestroy(ev);
-        }
+        release_ctx(*c);
         delete c;
     }
     free(e->numints);
@@ -1160,6 +1361,207 @@ static int class_stats_impl(Ctx &c, int64_t *q36, double *f36)
         }
     }
     c.prim_kept = pkept_all;
+    return 0;
+}
+
+// ---- molecule batches ----
+PatmolHipBatch *patmol_hip_batch_new(void)
+{
+    g_err = PATMOL_HIP_OK;
+    return new PatmolHipBatch();
+}
+
+void patmol_hip_batch_free(PatmolHipBatch *b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    for (auto *x : {&b->d_xyz, &b->d_q, &b->d_P, &b->d_G, &b->d_D}) x->release();
+    for (auto *x : {&b->d_soff, &b->d_orig, &b->d_tile_first, &b->d_tile_size, &b->d_fperm, &b->d_finv, &b->d_item_mol, &b->d_N,
+                    &b->d_fbase})
+        x->release();
+    b->d_dstart.release();
+    b->d_pstart.release();
+    b->d_pairs.release();
+    b->d_pp.release();
+    b->d_items.release();
+    b->d_mols.release();
+    b->d_J.release();
+    b->d_K.release();
+    delete b;
+}
+
+int patmol_hip_batch_add(PatmolHipBatch *b)
+{
+    g_err = PATMOL_HIP_OK;
+    if (!b) return fail(PATMOL_HIP_EBADARG, "patmol_hip_batch_add: null batch");
+    if (b->finalized) return fail(PATMOL_HIP_ESTATE, "patmol_hip_batch_add: batch already finalised");
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (ensure_tables()) return g_err;
+    }
+    Ctx c;
+    if (build_tables(c, 0)) {
+        release_ctx(c);
+        return g_err;
+    }
+    if (b->mol.empty()) b->tau = c.tau;
+    BatchMol m;
+    m.N = c.N;
+    m.nshell = c.nshell;
+    m.sbase = b->nshell;
+    m.fbase = b->norb;
+    m.tbase = b->ntile;
+    m.pbase = (long long)b->pairs.size();
+    m.ppbase = (long long)b->pp.size();
+    m.dstart = b->dtot;
+    m.pstart = b->no2tot;
+    const int mi = (int)b->mol.size();
+    b->xyz.insert(b->xyz.end(), c.shell_xyz.begin(), c.shell_xyz.end());
+    for (int v : c.shell_soff) b->soff.push_back(v + m.fbase);
+    for (int v : c.sorted_orig) b->orig.push_back(v + m.sbase);
+    for (int v : c.tile_first) b->tile_first.push_back(v + m.sbase);
+    b->tile_size.insert(b->tile_size.end(), c.tile_size.begin(), c.tile_size.end());
+    for (const PairRef &r : c.pairs) b->pairs.push_back(PairRef{(int)(r.pp0 + m.ppbase), r.npp});
+    b->pair_q.insert(b->pair_q.end(), c.pair_q.begin(), c.pair_q.end());
+    b->pp.insert(b->pp.end(), c.pp.begin(), c.pp.end());
+    b->fperm.insert(b->fperm.end(), c.fn_sorted_to_orig.begin(), c.fn_sorted_to_orig.end());
+    b->finv.insert(b->finv.end(), c.fn_orig_to_sorted.begin(), c.fn_orig_to_sorted.end());
+    for (const ClassRange &r : c.ranges)
+        for (int i = r.first; i < r.first + r.count; ++i) {
+            const int4 t = c.items[i];
+            b->its.push_back({make_int4(t.x + m.tbase, t.y + m.tbase, t.z + m.tbase, t.w + m.tbase), mi, r.bra_pc,
+                              r.ket_pc, c.item_bound[i]});
+        }
+    b->dtot += (long long)c.N * c.N;
+    b->no2tot += (long long)c.N * (c.N + 1) / 2;
+    b->nshell += c.nshell;
+    b->norb += c.N;
+    b->ntile += (int)c.tile_first.size();
+    b->mol.push_back(m);
+    release_ctx(c);
+    if (b->pp.size() > (size_t)INT32_MAX) return fail(PATMOL_HIP_EUNSUPPORTED, "batch too large");
+    return mi;
+}
+
+int patmol_hip_batch_finalize(PatmolHipBatch *b, int prec_mode, double threshold, double dmax)
+{
+    g_err = PATMOL_HIP_OK;
+    if (!b || b->mol.empty()) return fail(PATMOL_HIP_EBADARG, "patmol_hip_batch_finalize: empty batch");
+    if (b->finalized) return fail(PATMOL_HIP_ESTATE, "patmol_hip_batch_finalize: already finalised");
+    // per class: FP64 items first, each part by bound descending (the work
+    // list of every molecule merged into one queue per class)
+    auto fp32 = [&](const PatmolHipBatch::It &x) { return prec_mode == 1 && x.bound * dmax < threshold; };
+    std::stable_sort(b->its.begin(), b->its.end(), [&](const PatmolHipBatch::It &x, const PatmolHipBatch::It &y) {
+        if (x.bpc != y.bpc) return x.bpc > y.bpc;
+        if (x.kpc != y.kpc) return x.kpc > y.kpc;
+        const bool fx = fp32(x), fy = fp32(y);
+        if (fx != fy) return !fx;
+        return x.bound > y.bound;
+    });
+    std::vector<int4> items(b->its.size());
+    std::vector<int> imol(b->its.size());
+    b->ranges.clear();
+    b->n64.clear();
+    b->items_fp32 = 0;
+    for (size_t i = 0; i < b->its.size(); ++i) {
+        items[i] = b->its[i].t;
+        imol[i] = b->its[i].mol;
+        if (b->ranges.empty() || b->ranges.back().bra_pc != b->its[i].bpc || b->ranges.back().ket_pc != b->its[i].kpc) {
+            b->ranges.push_back({b->its[i].bpc, b->its[i].kpc, (int)i, 0});
+            b->n64.push_back(0);
+        }
+        b->ranges.back().count++;
+        if (fp32(b->its[i])) ++b->items_fp32;
+        else ++b->n64.back();
+    }
+    const int nm = (int)b->mol.size();
+    std::vector<MolDesc> md(nm);
+    std::vector<int> Nv(nm), fb(nm);
+    std::vector<long long> ds(nm), ps(nm);
+    b->Nvec.resize(nm);
+    b->pstart_v.resize(nm);
+    for (int i = 0; i < nm; ++i) {
+        const BatchMol &m = b->mol[i];
+        md[i] = MolDesc{m.N, m.sbase, m.pbase, m.dstart - (long long)m.fbase * m.N - m.fbase};
+        Nv[i] = m.N;
+        fb[i] = m.fbase;
+        ds[i] = m.dstart;
+        ps[i] = m.pstart;
+        b->Nvec[i] = m.N;
+        b->pstart_v[i] = m.pstart;
+    }
+    if (b->d_xyz.upload(b->xyz) || b->d_q.upload(b->pair_q) || b->d_soff.upload(b->soff) || b->d_orig.upload(b->orig) ||
+        b->d_tile_first.upload(b->tile_first) || b->d_tile_size.upload(b->tile_size) || b->d_fperm.upload(b->fperm) ||
+        b->d_finv.upload(b->finv) || b->d_item_mol.upload(imol) || b->d_N.upload(Nv) || b->d_fbase.upload(fb) ||
+        b->d_dstart.upload(ds) || b->d_pstart.upload(ps) || b->d_pairs.upload(b->pairs) || b->d_pp.upload(b->pp) ||
+        b->d_items.upload(items) || b->d_mols.upload(md) || b->d_P.alloc(b->no2tot) || b->d_G.alloc(b->no2tot) ||
+        b->d_D.alloc(b->dtot) || b->d_J.alloc((size_t)b->dtot * PM_XCD_SLOTS) ||
+        b->d_K.alloc((size_t)b->dtot * PM_XCD_SLOTS))
+        return g_err;
+    HIPCHK(hipMemset(b->d_P.p, 0, sizeof(double) * b->no2tot));
+    HIPCHK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+    // capture the whole G build once
+    HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_build(*b, b->s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return g_err;
+    }
+    if (e != hipSuccess) return fail(PATMOL_HIP_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->finalized = true;
+    return 0;
+}
+
+// G += 2J[P] - K[P] for every molecule from host arrays (the molecules'
+// packed triangles concatenated in the order they were added); by the
+// captured graph, or (use_graph 0) by the same launches issued directly
+int patmol_hip_batch_gmatrix(PatmolHipBatch *b, const double *P, double *G, int use_graph)
+{
+    g_err = PATMOL_HIP_OK;
+    if (!b || !P || !G) return fail(PATMOL_HIP_EBADARG, "patmol_hip_batch_gmatrix: bad argument");
+    if (!b->finalized) return fail(PATMOL_HIP_ESTATE, "patmol_hip_batch_gmatrix: batch not finalised");
+    HIPCHK(hipMemcpyAsync(b->d_P.p, P, sizeof(double) * b->no2tot, hipMemcpyHostToDevice, b->s));
+    if (use_graph) {
+        HIPCHK(hipGraphLaunch(b->exec, b->s));
+    } else if (batch_build(*b, b->s)) {
+        return g_err;
+    }
+    std::vector<double> g(b->no2tot);
+    HIPCHK(hipMemcpyAsync(g.data(), b->d_G.p, sizeof(double) * b->no2tot, hipMemcpyDeviceToHost, b->s));
+    HIPCHK(hipStreamSynchronize(b->s));
+    for (long long i = 0; i < b->no2tot; ++i) G[i] += g[i];
+    return 0;
+}
+
+// device-resident use: the batch's own P (input) and G (output) buffers, and
+// one replay of the captured G build on `stream`
+int patmol_hip_batch_buffers(PatmolHipBatch *b, double **d_P, double **d_G)
+{
+    if (!b || !b->finalized) return fail(PATMOL_HIP_ESTATE, "batch not finalised");
+    if (d_P) *d_P = b->d_P.p;
+    if (d_G) *d_G = b->d_G.p;
+    return 0;
+}
+
+int patmol_hip_batch_launch(PatmolHipBatch *b, void *stream)
+{
+    g_err = PATMOL_HIP_OK;
+    return batch_launch(b, (hipStream_t)stream);
+}
+
+int patmol_hip_batch_info(PatmolHipBatch *b, int *nmol, int64_t *no2tot, int64_t *items, int64_t *items_fp32)
+{
+    if (!b) return fail(PATMOL_HIP_EBADARG, "null batch");
+    if (nmol) *nmol = (int)b->mol.size();
+    if (no2tot) *no2tot = b->no2tot;
+    if (items) *items = (int64_t)b->its.size();
+    if (items_fp32) *items_fp32 = b->items_fp32;
     return 0;
 }
 

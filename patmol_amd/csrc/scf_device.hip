@@ -35,6 +35,7 @@
 #include <utility>
 #include <vector>
 
+#include "batch.h"
 #include "patmol_hip.h"
 
 extern "C" {
@@ -624,7 +625,8 @@ __global__ void diis_combine_kernel(const double *__restrict__ fbuf, const doubl
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nn) return;
     double v = 0.0;
-    for (int j = 0; j < m; ++j) v += c[j] * fbuf[(size_t)j * nn + idx];
+    for (int j = 0; j < m; ++j)
+        if (c[j] != 0.0) v += c[j] * fbuf[(size_t)j * nn + idx];   // slots not in use are skipped
     F[idx] = v;
 }
 
@@ -667,23 +669,40 @@ struct Scf {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const char *err = nullptr;
 
+    bool owns = true;   // the stream and the rocBLAS handle are this run's own
+
     ~Scf()
     {
-        if (h) rocblas_destroy_handle(h);
+        if (owns && h) rocblas_destroy_handle(h);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
-        if (s) (void)hipStreamDestroy(s);
+        if (owns && s) (void)hipStreamDestroy(s);
     }
 
     bool init(int n_)
     {
+        if (hipStreamCreate(&s) != hipSuccess) return fail("stream creation failed");
+        if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, s) != rocblas_status_success)
+            return fail("rocblas handle creation failed");
+        return init_buffers(n_);
+    }
+
+    // on a stream and rocBLAS handle shared with other runs (a batch)
+    bool init_shared(int n_, hipStream_t st, rocblas_handle hd)
+    {
+        owns = false;
+        s = st;
+        h = hd;
+        return init_buffers(n_);
+    }
+
+    bool init_buffers(int n_)
+    {
         n = n_;
         no2 = (size_t)n * (n + 1) / 2;
         const size_t nn = (size_t)n * n;
-        if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-            return fail("stream/event creation failed");
-        if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, s) != rocblas_status_success)
-            return fail("rocblas handle creation failed");
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            return fail("event creation failed");
         X = mem.get<double>(nn);
         F = mem.get<double>(nn);
         T = mem.get<double>(nn);
@@ -928,6 +947,10 @@ struct Scf {
     // (Pulay, Chem. Phys. Lett. 73, 393 (1980); J. Comput. Chem. 3, 556
     // (1982)).  The error dot products are fixed-order device sums, the small
     // linear system is solved on the host, so the sequence is reproducible.
+    // The first iteration's Fock matrix (of the core-Hamiltonian guess) stays
+    // out of the subspace: with it in, extrapolation from the guess can lock
+    // onto an excited stationary state (seen on a C/N/O test molecule of 50
+    // functions, 0.28 Eh above the ground state that plain iteration reaches).
     static constexpr int DIIS_M = 8;
     double *Sfull = nullptr, *Dfull = nullptr, *Fao = nullptr, *W = nullptr;
     double *dfbuf = nullptr, *debuf = nullptr, *dpart = nullptr, *dout = nullptr, *dcoef = nullptr;
@@ -950,6 +973,10 @@ struct Scf {
         if (!Sfull || !Dfull || !Fao || !W || !dfbuf || !debuf || !dpart || !dout || !dcoef)
             return fail("device allocation failed");
         hipLaunchKernelGGL(sym_from_packed_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, dS, nullptr, Sfull, n);
+        // the ring starts empty: no slot may hold stale values
+        if (hipMemsetAsync(dfbuf, 0, sizeof(double) * DIIS_M * nn, s) != hipSuccess ||
+            hipMemsetAsync(debuf, 0, sizeof(double) * DIIS_M * nn, s) != hipSuccess)
+            return fail("memset failed");
         diis_cnt = diis_next = 0;
         return hipGetLastError() == hipSuccess || fail("launch failure");
     }
@@ -1124,14 +1151,14 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         hipMemsetAsync(P1, 0, sizeof(double) * no2, c.s) != hipSuccess ||
         hipMemsetAsync(P2, 0, sizeof(double) * no2, c.s) != hipSuccess)
         return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: upload failed");
-    if (!c.form_x(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+    if (!c.form_x(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf (orthogonaliser)");
     const bool diis = scf_accel == 3;
-    if (diis && !c.diis_init(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+    if (diis && !c.diis_init(dS)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf (DIIS set-up)");
     if (Pguess) {
         if (hipMemcpyAsync(P, Pguess, sizeof(double) * no2, hipMemcpyHostToDevice, c.s) != hipSuccess)
             return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: upload failed");
     } else if (!c.diag_density(dH, nullptr, nocc, P, 0)) {
-        return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf (guess)");
     }
     double eps = scf_eps;
     if (eps < EPS10 * n * n) eps = EPS10 * n * n;
@@ -1157,8 +1184,8 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         P1 = P;
         P = fresh;
         // P1 is the density this iteration's G was built from
-        const bool dens_ok = diis ? c.diis_density(dH, dG, P1, nocc, P, 0) : c.density(dH, dG, nocc, P, 0);
-        if (!dens_ok) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
+        const bool dens_ok = diis && iteration > 1 ? c.diis_density(dH, dG, P1, nocc, P, 0) : c.density(dH, dG, nocc, P, 0);
+        if (!dens_ok) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf (density step)");
         double sm[3];
         if (!c.sums3(P, P1, dH, dG, sm)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
         const float g_ms = c.elapsed_ms();
@@ -1377,3 +1404,139 @@ extern "C" int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int 
 }
 
 extern "C" void patmol_hip_set_scf_density(int mode) { pm::g_scf_density = (mode == 0 || mode == 2) ? mode : 1; }
+
+// RHF of every molecule of a batch, all iterating together (see patmol_hip.h)
+extern "C" int patmol_hip_batch_rhf_scf(PatmolHipBatch *b, const double *S, const double *H, const int *nocc,
+                                        const double *enuc, int maxits, double scf_eps, int scf_accel, double *energy,
+                                        int *iterations, int *converged, double *Pout, double *total_ms,
+                                        double *gbuild_ms)
+{
+    BatchView v;
+    if (!batch_view(b, v)) return aux_fail(PATMOL_HIP_ESTATE, "patmol_hip_batch_rhf_scf: batch not finalised");
+    if (!S || !H || !nocc || !enuc || !energy || !iterations || !converged)
+        return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_batch_rhf_scf: null argument");
+    if (scf_accel == 1 || scf_accel == 2)
+        return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_batch_rhf_scf: scf_accel 0 (none) or 3 (DIIS)");
+    const int nm = v.nmol;
+    const bool diis = scf_accel == 3;
+    const bool log = scf_log();
+    const double EPS10 = 10 * DBL_EPSILON;
+    hipStream_t st = nullptr;
+    rocblas_handle hd = nullptr;
+    hipEvent_t g0 = nullptr, g1 = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&g0) != hipSuccess || hipEventCreate(&g1) != hipSuccess ||
+        rocblas_create_handle(&hd) != rocblas_status_success || rocblas_set_stream(hd, st) != rocblas_status_success)
+        return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_batch_rhf_scf: stream/handle creation failed");
+    struct Mol {
+        Scf c;
+        double *dS = nullptr, *dH = nullptr, *Pn = nullptr;
+        double eps = 0.0, eold = 0.0, enrg = 0.0;
+        int iteration = 0, done = 0;
+        bool first = true;
+    };
+    std::vector<Mol> mols(nm);
+    int rc = 0;
+    auto cleanup = [&]() {
+        mols.clear();
+        rocblas_destroy_handle(hd);
+        (void)hipEventDestroy(g0);
+        (void)hipEventDestroy(g1);
+        (void)hipStreamDestroy(st);
+    };
+    for (int m = 0; m < nm && !rc; ++m) {
+        Mol &x = mols[m];
+        const int n = v.N[m];
+        if (nocc[m] < 0 || nocc[m] > n) rc = aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_batch_rhf_scf: bad nocc");
+        else if (!x.c.init_shared(n, st, hd)) rc = scf_fail(x.c, PATMOL_HIP_EDEVICE, "patmol_hip_batch_rhf_scf");
+        if (rc) break;
+        const size_t no2 = x.c.no2;
+        x.dS = x.c.mem.get<double>(no2);
+        x.dH = x.c.mem.get<double>(no2);
+        x.Pn = x.c.mem.get<double>(no2);
+        double *Pin = v.d_P + v.pstart[m];
+        if (!x.dS || !x.dH || !x.Pn ||
+            hipMemcpyAsync(x.dS, S + v.pstart[m], sizeof(double) * no2, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(x.dH, H + v.pstart[m], sizeof(double) * no2, hipMemcpyHostToDevice, st) != hipSuccess) {
+            rc = aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_batch_rhf_scf: upload failed");
+            break;
+        }
+        if (!x.c.form_x(x.dS) || (diis && !x.c.diis_init(x.dS)) || !x.c.diag_density(x.dH, nullptr, nocc[m], Pin, 0)) {
+            char where[96];
+            snprintf(where, sizeof where, "patmol_hip_batch_rhf_scf: molecule %d, set-up", m);
+            rc = scf_fail(x.c, PATMOL_HIP_EDEVICE, where);
+            break;
+        }
+        x.c.purify = g_scf_density >= 1 && n >= 128;
+        x.eps = scf_eps < EPS10 * n * n ? EPS10 * n * n : scf_eps;
+    }
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    double gms = 0.0;
+    for (int sweep = 0;; ++sweep) {
+        bool any = false;
+        for (int m = 0; m < nm; ++m) any = any || (!mols[m].done && mols[m].iteration < maxits);
+        if (!any) break;
+        (void)hipEventRecord(g0, st);
+        if (batch_launch(b, st)) {
+            cleanup();
+            return patmol_hip_last_error();
+        }
+        (void)hipEventRecord(g1, st);
+        for (int m = 0; m < nm; ++m) {
+            Mol &x = mols[m];
+            if (x.done || x.iteration >= maxits) continue;
+            ++x.iteration;
+            double *Pin = v.d_P + v.pstart[m], *dG = v.d_G + v.pstart[m];
+            // the rhfscf.c:21-71 update of this molecule: new density from
+            // H + G(Pin), energy of Pin, density change, stop test
+            const bool ok = diis && x.iteration > 1 ? x.c.diis_density(x.dH, dG, Pin, nocc[m], x.Pn, 0) : x.c.density(x.dH, dG, nocc[m], x.Pn, 0);
+            double sm[3];
+            if (!ok || !x.c.sums3(x.Pn, Pin, x.dH, dG, sm)) {
+                char where[96];
+                snprintf(where, sizeof where, "patmol_hip_batch_rhf_scf: molecule %d, sweep %d", m, sweep);
+                rc = scf_fail(x.c, PATMOL_HIP_EDEVICE, where);
+                break;
+            }
+            const double pdiff = sqrt(2.0 * sm[2]) / x.c.n;
+            x.enrg = 2.0 * (sm[0] + sm[0] + sm[1]);
+            const double ediff = x.enrg - x.eold;
+            x.eold = x.enrg;
+            if (log)
+                fprintf(stderr, "batch sweep %d molecule %d: ITERATION = %d EDIFF= %lg PDIFF= %lg energy = %.12f\n", sweep,
+                        m, x.iteration, ediff, pdiff, x.enrg + enuc[m]);
+            int ret = 0;
+            if (x.iteration > 1 && (fabs(ediff) < EPS10 || pdiff < x.eps)) ret = 1;
+            if (x.first) {
+                x.first = false;
+                x.iteration = 0;
+            } else {
+                x.done = ret;
+            }
+            if (hipMemcpyAsync(Pin, x.Pn, sizeof(double) * x.c.no2, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+                rc = aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_batch_rhf_scf: copy failed");
+                break;
+            }
+        }
+        if (rc) break;
+        float ms = 0.0f;
+        if (hipEventSynchronize(g1) == hipSuccess && hipEventElapsedTime(&ms, g0, g1) == hipSuccess) gms += ms;
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_batch_rhf_scf: device failure");
+    if (!rc) {
+        for (int m = 0; m < nm; ++m) {
+            energy[m] = mols[m].enrg + enuc[m];
+            iterations[m] = mols[m].iteration;
+            converged[m] = mols[m].done;
+        }
+        if (Pout && hipMemcpy(Pout, v.d_P, sizeof(double) * v.no2tot, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_batch_rhf_scf: download failed");
+        if (total_ms)
+            *total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (gbuild_ms) *gbuild_ms = gms;
+    }
+    cleanup();
+    return rc;
+}

@@ -190,6 +190,107 @@ def measure(torch, pm, system, key, steps, warmup, scf):
     return out, P
 
 
+def time_batch_launches(torch, batch, steps, warmup):
+    """Mean ms per batched G build (one graph launch: unpack, zero, every
+    class over every molecule, finalize) on the current stream."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        batch.launch(stream.cuda_stream)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        batch.launch(stream.cuda_stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def measure_batch(torch, pm, systems, steps, warmup, nmol=64, seed=1002, scf=True):
+    """The seeded batch of `nmol` H/C/N/O molecules (2-20 atoms): the plain
+    FP64 path (one drop-in G build per molecule, molecule after molecule)
+    against the batched G build (all molecules in one graph) in FP64 and in
+    mixed precision (Schwarz * max|P| < 1e-6 in FP32), each molecule's
+    batched G checked bitwise against its own FP64 build; then the batched
+    RHF of every molecule (DIIS) in both modes with per-molecule |dE|."""
+    mols = systems.molecule_batch(nmol, seed)
+    pm.set_options(tau=TAU, precision=0, timing=False, strict=False)
+    stream = torch.cuda.current_stream()
+    guesses, kept, single_ms, singles = [], 0, 0.0, []
+    for m in mols:
+        pm.load_system(m)
+        S, _, H = pm.one_electron_ints()
+        P = core_guess_density(S, H, m.norb, m.nelec // 2)
+        guesses.append(P)
+        st = pm.EriStore()
+        kept += st.stats()["shell_quartets_kept"]
+        dP = torch.tensor(P, dtype=torch.float64, device="cuda")
+        dG = torch.zeros_like(dP)
+        single_ms += time_builds(torch, st, dP, dG, max(1, steps // 2), 1, stream)
+        singles.append(dG.cpu().numpy().copy())
+        st.close()
+    Pcat = np.concatenate(guesses)
+    dmax = float(np.abs(Pcat).max())
+    log(f"batch of {nmol}: {sum(m.norb for m in mols)} functions, {kept} kept shell quartets; plain FP64 path "
+        f"(one build per molecule) {single_ms:.2f} ms")
+    out = {"workload": f"batch of {nmol} seeded H/C/N/O molecules (seed {seed}), "
+                       f"{min(m.norb for m in mols)}-{max(m.norb for m in mols)} functions each",
+           "molecules": nmol, "functions_total": int(sum(m.norb for m in mols)),
+           "shell_quartets_kept": int(kept), "ms_plain_fp64_one_build_per_molecule": single_ms}
+    for mode in (0, 1):
+        b = pm.Batch()
+        for m in mols:
+            b.add(m)
+        b.finalize(precision=mode, threshold=MIXED_THR, dmax=dmax)
+        G = b.gmatrix(Pcat, use_graph=False)
+        off = b.offsets()
+        if mode == 0:
+            same = all(np.array_equal(G[off[i]:off[i + 1]], singles[i]) for i in range(nmol))
+            out["batched_fp64_bitwise_equal_to_single_builds"] = bool(same)
+        else:
+            out["max_abs_dG_mixed"] = float(max(np.abs(G[off[i]:off[i + 1]] - singles[i]).max()
+                                                for i in range(nmol)))
+        ms = time_batch_launches(torch, b, steps, warmup)
+        info = b.info()
+        key = "mixed" if mode else "fp64"
+        out[f"ms_batched_{key}"] = ms
+        out[f"work_items_{key}"] = info["work_items"]
+        if mode:
+            out["work_items_fp32"] = info["work_items_fp32"]
+        log(f"  batched {key}: {ms:.2f} ms per G build of all molecules ({info})")
+        b.close()
+    out["batching_speedup_fp64"] = single_ms / out["ms_batched_fp64"]
+    out["mixed_speedup_batched"] = out["ms_batched_fp64"] / out["ms_batched_mixed"]
+    out["speedup_batched_mixed_vs_plain_fp64"] = single_ms / out["ms_batched_mixed"]
+    out["quartets_per_s_batched_mixed"] = kept / (out["ms_batched_mixed"] * 1e-3)
+    out["quartets_per_s_plain_fp64"] = kept / (single_ms * 1e-3)
+    if scf:
+        runs = {}
+        for mode in (0, 1):
+            b = pm.Batch()
+            for m in mols:
+                b.add(m)
+            b.finalize(precision=mode, threshold=MIXED_THR, dmax=dmax)
+            e, it, cv, _, tot, gms = b.rhf_scf(maxits=100, accel=3)
+            b.close()
+            sweeps = int(it.max()) + 1
+            runs[mode] = (e, it, cv)
+            key = "mixed" if mode else "fp64"
+            out[f"scf_{key}"] = {"converged": int(cv.sum()), "max_iterations": int(it.max()),
+                                 "ms_total": tot, "ms_g_graphs": gms, "ms_per_sweep": tot / sweeps,
+                                 "g_ms_per_sweep": gms / sweeps}
+            log(f"  batched SCF {key}: {int(cv.sum())}/{nmol} converged, iterations {int(it.min())}-{int(it.max())}, "
+                f"{tot:.0f} ms ({gms:.0f} ms in G graphs)")
+        e0, _, c0 = runs[0]
+        e1, _, c1 = runs[1]
+        both = (c0 > 0) & (c1 > 0)
+        dE = np.abs(e1 - e0)
+        out["dE_scf_mixed_vs_fp64_max"] = float(dE[both].max()) if both.any() else None
+        out["dE_scf_mixed_vs_fp64_per_molecule"] = [float(x) for x in dE]
+    return out
+
+
 def cpu_threads():
     """Threads of the host's share: OMP_NUM_THREADS where the environment
     sets it (16 per GPU on the GPU boxes), else every core."""
@@ -307,6 +408,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--profile-build", default="", metavar="SYSTEM")
+    ap.add_argument("--batch-only", action="store_true", help="only the 64-molecule batch block")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the G build runs on a single MI355X "
@@ -324,9 +426,13 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.batch_only:
+        print(json.dumps({"batch": measure_batch(torch, pm, systems, args.steps, args.warmup)}))
+        return 0
     head_sys = systems.water16()
     head, P16 = measure(torch, pm, head_sys, "water16", args.steps, args.warmup, scf=True)
     sec, _ = measure(torch, pm, systems.water8(), "water8", args.steps, args.warmup, scf=False)
+    batch = measure_batch(torch, pm, systems, args.steps, args.warmup)
 
     dE = head["dE_scf_mixed_vs_fp64"]
     use_mixed = dE <= 1e-6 and head["fp32_items"] > 0 and head["ms_per_build_mixed"] < head["ms_per_build_fp64"]
@@ -378,6 +484,7 @@ def main():
         },
         "headline": head,
         "secondary_water8": sec,
+        "batch_64": batch,
     }
     if not args.no_cpu:
         log("cpu baseline (oracle direct build, bounded sample) ...")

@@ -145,6 +145,11 @@ typedef struct {
     double total_ms;
     double energy0;     /* energy after the first update (rhfscf.c:86, 191) */
     int purify_fallbacks; /* iterations whose purification did not converge */
+    int full_builds;    /* G builds of the whole density */
+    int delta_builds;   /* incremental G builds (patmol_hip_set_scf_incremental) */
+    double items;       /* work items over all G builds of the run */
+    double items_fp32;  /* of those, launched in FP32 */
+    double items_skipped; /* of those, skipped by density-weighted screening */
 } PatmolHipScfResult;
 int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Smat, const double *Hmat,
                        int nocc, int maxits, double scf_eps, int scf_accel, double enuc,
@@ -163,6 +168,16 @@ int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const double *Sma
  * takes by itself above 1024 tiles of 32 x 32, N > 1024).  The guess and the
  * returned orbitals always come from the eigensolver. */
 void patmol_hip_set_scf_density(int mode);
+/* Incremental G in patmol_hip_rhf_scf (no counterpart in the reference;
+ * period 0, the default, builds G of the whole density every iteration as
+ * the reference does).  With period k > 0 the whole density is built every
+ * k-th iteration (the first included) and in between only G(P - Pref) is
+ * built and added to G(Pref), Pref the previous iteration's density: G is
+ * linear in P.  The difference build skips quartets whose Schwarz bound
+ * times max|P - Pref| is below the Schwarz threshold, and in mixed precision
+ * (patmol_hip_set_precision) splits FP32 from FP64 items by that product.
+ * The result's items / items_fp32 / items_skipped count the work. */
+void patmol_hip_set_scf_incremental(int period);
 /* One diagonalisation (spack_trans + spack_diag + form_cmatrix +
  * form_pmatrix) of host packed F in the basis orthogonalised by S: ascending
  * eigenvalues, C (row-major, phase-fixed) and P for nocc occupied orbitals. */

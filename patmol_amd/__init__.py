@@ -64,7 +64,9 @@ class HipScfResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("eelec", C.c_double), ("enuc", C.c_double), ("ediff", C.c_double),
                 ("pdiff", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
                 ("gbuild_ms", C.c_double), ("other_ms", C.c_double), ("total_ms", C.c_double),
-                ("energy0", C.c_double), ("purify_fallbacks", C.c_int)]
+                ("energy0", C.c_double), ("purify_fallbacks", C.c_int), ("full_builds", C.c_int),
+                ("delta_builds", C.c_int), ("items", C.c_double), ("items_fp32", C.c_double),
+                ("items_skipped", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -138,6 +140,7 @@ def lib():
                                          C.c_int, C.c_double, _odp, _odp, _odp, _odp, _odp, _odp,
                                          C.POINTER(HipScfResult)]
         G.patmol_hip_set_scf_density.argtypes = [C.c_int]
+        G.patmol_hip_set_scf_incremental.argtypes = [C.c_int]
         G.patmol_hip_diag_fock.restype = C.c_int
         G.patmol_hip_diag_fock.argtypes = [_dp, _dp, C.c_int, _odp, _odp, _odp]
         G.patmol_hip_uhf_gmatrix_device.restype = C.c_int
@@ -207,7 +210,7 @@ def exported_symbols():
             "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
-            "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
+            "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "patmol_hip_set_scf_incremental", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
             "patmol_hip_write_erints", "patmol_hip_rhf_gmatrix_erints", "patmol_hip_rhf_gmatrix_erint_file",
             "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file", "patmol_hip_eri_gradient",
             "patmol_hip_batch_new", "patmol_hip_batch_add", "patmol_hip_batch_finalize", "patmol_hip_batch_gmatrix",
@@ -227,10 +230,12 @@ def load_system(sys_: "systems.System") -> None:
 
 def set_options(tau: Optional[float] = None, precision: Optional[int] = None,
                 mixed_threshold: float = 1e-6, strict: Optional[bool] = None, timing: Optional[bool] = None,
-                scf_density: Optional[int] = None):
+                scf_density: Optional[int] = None, scf_incremental: Optional[int] = None):
     _, G = lib()
     if scf_density is not None:
         G.patmol_hip_set_scf_density(int(scf_density))
+    if scf_incremental is not None:
+        G.patmol_hip_set_scf_incremental(int(scf_incremental))
     if tau is not None:
         G.patmol_hip_set_schwarz_threshold(tau)
     if precision is not None:

@@ -1,7 +1,10 @@
-// batch.h -- what the device SCF needs from a molecule batch (hip_context.cpp).
+// batch.h -- what the device SCF needs from the G-build side (hip_context.cpp):
+// molecule batches and the incremental (density-difference) build.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "patmol_types.h"
 
 struct PatmolHipBatch;
 
@@ -21,5 +24,21 @@ bool batch_view(PatmolHipBatch *b, BatchView &v);
 // G = 2J[P] - K[P] for every molecule, P and G the batch's own buffers:
 // one replay of the captured graph on `s`
 int batch_launch(PatmolHipBatch *b, hipStream_t s);
+
+// Work-item counts of one G build
+struct BuildCounts {
+    int64_t items = 0;     // work items of the Schwarz-screened list
+    int64_t fp32 = 0;      // launched in FP32
+    int64_t skipped = 0;   // not launched (density-weighted screening of a delta build)
+};
+
+// G += 2J[dP] - K[dP] for a density difference dP (device, packed): the G
+// build of patmol_hip_rhf_gmatrix_device, plus screening by the density:
+// quartets whose Schwarz bound times max|dP| is below the Schwarz threshold
+// are skipped (their contributions are below the threshold).  The precision
+// split, when on, uses max|dP|.
+int rhf_gmatrix_delta(const ERIFS *e, const double *dP, double *dG, hipStream_t s, BuildCounts *counts);
+// the counts of the last build of e (0 if e has no tables)
+int last_build_counts(const ERIFS *e, BuildCounts *counts);
 
 }  // namespace pm

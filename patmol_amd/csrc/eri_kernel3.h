@@ -175,7 +175,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     // different bra shells fall into different LDS banks
     constexpr int SJ = FJ | 1, SK = FK | 1, SL = FL | 1;
     constexpr int ZIJ = FI * SJ, ZKL = FK * SL, ZIK = FI * SK, ZIL = FI * SL, ZJK = FJ * SK, ZJL = FJ * SL;
-    __shared__ double sD[ZIJ + ZKL + ZIK + ZIL + ZJK + ZJL];
+    // Contraction type: the FP32 instantiation contracts its FP32 integrals
+    // with an FP32 copy of the density blocks and sums a step's products
+    // across lanes in FP32; the sums land in the FP64 output blocks, and the
+    // per-item J'(IJ) sum is FP64 in both (the FP32 part of it is one K
+    // shell's products).
+    using A = R;
+    __shared__ A sD[ZIJ + ZKL + ZIK + ZIL + ZJK + ZJL];
     // Output blocks.  An element that several waves can write gets one copy
     // per writing wave: J'(KL) one per wave of a K group (SPLIT), K'(IK) per
     // b block (SB), K'(JK) per a block (SA), K'(IL) per (K group, b block),
@@ -316,7 +322,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 const bool ex = c < cn * ncc && rs < rn && cs < cn;
                 double v = 0.0;
                 if (ex) v = src[dadj + (long long)(sOff[rsh + rs] + r % nrc) * N + sOff[csh + cs] + c % ncc];
-                sD[o + e] = v;
+                sD[o + e] = A(v);
             }
         };
         for (int e = tid; e < pEND; e += 256) sO[e] = 0.0;
@@ -366,18 +372,24 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     double jab[NAH * NBH];
 #pragma unroll
     for (int i = 0; i < NAH * NBH; ++i) jab[i] = 0.0;
-    const double *__restrict__ DIJ = sD + dIJ + (ii * NA) * SJ + jj * NB;
+    const A *__restrict__ DIJ = sD + dIJ + (ii * NA) * SJ + jj * NB;
 
     constexpr int KPER = (TK + KS - 1) / KS;
     const int kq_end = min(TK, (ksub + 1) * KPER);
     for (int kq = ksub * KPER + wave / SPLIT; kq < kq_end; kq += KSTRIDE) {
         if (kq >= nK) break;  // uniform
         const int K = fK + kq;
-        double kac[NAH * NC], kbc[NBH * NC];
+        // FP32: this K shell's J'(IJ) products, added to the FP64 jab after
+        // the K shell (FP64 adds straight into jab)
+        constexpr bool A64 = std::is_same<A, double>::value;
+        constexpr int NJK = A64 ? 1 : NAH * NBH;
+        A kac[NAH * NC], kbc[NBH * NC], jabk[NJK];
 #pragma unroll
-        for (int i = 0; i < NAH * NC; ++i) kac[i] = 0.0;
+        for (int i = 0; i < NAH * NC; ++i) kac[i] = A(0);
 #pragma unroll
-        for (int i = 0; i < NBH * NC; ++i) kbc[i] = 0.0;
+        for (int i = 0; i < NBH * NC; ++i) kbc[i] = A(0);
+#pragma unroll
+        for (int i = 0; i < NJK; ++i) jabk[i] = A(0);
         for (int lc = 0; lc < NLSTEP; ++lc) {
             const int lq0 = lc * KSUB + ks;
             const bool lexist = lq0 < nL;
@@ -407,18 +419,18 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 kg.A[x] = R(sXK[kq][x]);
                 kg.AB[x] = R(sXK[kq][x] - sXL[lqs][x]);
             }
-            double jcd[NC * ND], kad[NAH * ND], kbd[NBH * ND];
+            A jcd[NC * ND], kad[NAH * ND], kbd[NBH * ND];
 #pragma unroll
-            for (int i = 0; i < NC * ND; ++i) jcd[i] = 0.0;
+            for (int i = 0; i < NC * ND; ++i) jcd[i] = A(0);
 #pragma unroll
-            for (int i = 0; i < NAH * ND; ++i) kad[i] = 0.0;
+            for (int i = 0; i < NAH * ND; ++i) kad[i] = A(0);
 #pragma unroll
-            for (int i = 0; i < NBH * ND; ++i) kbd[i] = 0.0;
-            const double *__restrict__ DKL = sD + dKL + (kq * NC) * SL + lq * ND;
-            const double *__restrict__ DJL = sD + dJL + (jj * NB) * SL + lq * ND;
-            const double *__restrict__ DIL = sD + dIL + (ii * NA) * SL + lq * ND;
-            const double *__restrict__ DJK = sD + dJK + (jj * NB) * SK + kq * NC;
-            const double *__restrict__ DIK = sD + dIK + (ii * NA) * SK + kq * NC;
+            for (int i = 0; i < NBH * ND; ++i) kbd[i] = A(0);
+            const A *__restrict__ DKL = sD + dKL + (kq * NC) * SL + lq * ND;
+            const A *__restrict__ DJL = sD + dJL + (jj * NB) * SL + lq * ND;
+            const A *__restrict__ DIL = sD + dIL + (ii * NA) * SL + lq * ND;
+            const A *__restrict__ DJK = sD + dJK + (jj * NB) * SK + kq * NC;
+            const A *__restrict__ DIK = sD + dIK + (ii * NA) * SK + kq * NC;
             const int nkp = sKmax;  // uniform bound of the ket primitive loop
             const PrimPair *kpp = staged ? (sPP + sPPoff[t]) : (a.pp + kref.pp0);
             // the primitive loops for one (a, b) component block
@@ -428,8 +440,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                     constexpr int A_ = decltype(ia)::value, B_ = decltype(ib_)::value;
                     constexpr int C_ = decltype(ic)::value, D_ = decltype(id)::value;
                     constexpr int Al = A_ - A0, Bl = B_ - B0;
-                    const double v = double(vr);
-                    jab[Al * NBH + Bl] += v * DKL[C_ * SL + D_];
+                    const A v = A(vr);
+                    if constexpr (A64) jab[Al * NBH + Bl] += v * DKL[C_ * SL + D_];
+                    else jabk[Al * NBH + Bl] += v * DKL[C_ * SL + D_];
                     jcd[C_ * ND + D_] += v * DIJ[A_ * SJ + B_];
                     kac[Al * NC + C_] += v * DJL[B_ * SL + D_];
                     kad[Al * ND + D_] += v * DJK[B_ * SK + C_];
@@ -490,7 +503,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 for (int c = 0; c < NC; ++c)
 #pragma unroll
                     for (int d = 0; d < ND; ++d) {
-                        lds_acc<red_hi(MASK_I | MASK_J)>(&oKL[(kq * NC + c) * SL + lq * ND + d], 2.0 * jcd[c * ND + d]);
+                        lds_acc<red_hi(MASK_I | MASK_J)>(&oKL[(kq * NC + c) * SL + lq * ND + d],
+                                                          2.0 * double(jcd[c * ND + d]));
                     }
             }
             if ((lane & red_lo(MASK_J)) == 0 && iexist && lexist) {
@@ -498,14 +512,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                 for (int x = 0; x < NAH; ++x)
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
-                        lds_acc<red_hi(MASK_J)>(&oIL[(ii * NA + a0 + x) * SL + lq * ND + d], kad[x * ND + d]);
+                        lds_acc<red_hi(MASK_J)>(&oIL[(ii * NA + a0 + x) * SL + lq * ND + d], double(kad[x * ND + d]));
             }
             if ((lane & red_lo(MASK_I)) == 0 && jexist && lexist) {
 #pragma unroll
                 for (int b = 0; b < NBH; ++b)
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
-                        lds_acc<red_hi(MASK_I)>(&oJL[(jj * NB + b0 + b) * SL + lq * ND + d], kbd[b * ND + d]);
+                        lds_acc<red_hi(MASK_I)>(&oJL[(jj * NB + b0 + b) * SL + lq * ND + d], double(kbd[b * ND + d]));
             }
         }
         // per-K results: K'(IK), K'(JK) (owned by the waves of this K)
@@ -516,7 +530,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             for (int x = 0; x < NAH; ++x)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    lds_acc<red_hi(MASK_J | MASK_S)>(&oIK[(ii * NA + a0 + x) * SK + kq * NC + c], kac[x * NC + c]);
+                    lds_acc<red_hi(MASK_J | MASK_S)>(&oIK[(ii * NA + a0 + x) * SK + kq * NC + c], double(kac[x * NC + c]));
                 }
         }
         if ((lane & red_lo(MASK_I | MASK_S)) == 0 && jexist) {
@@ -524,8 +538,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             for (int b = 0; b < NBH; ++b)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    lds_acc<red_hi(MASK_I | MASK_S)>(&oJK[(jj * NB + b0 + b) * SK + kq * NC + c], kbc[b * NC + c]);
+                    lds_acc<red_hi(MASK_I | MASK_S)>(&oJK[(jj * NB + b0 + b) * SK + kq * NC + c], double(kbc[b * NC + c]));
                 }
+        }
+        if constexpr (!A64) {
+#pragma unroll
+            for (int i = 0; i < NAH * NBH; ++i) jab[i] += double(jabk[i]);
         }
     }
     // per-item result: J'(IJ), added by one K group after the other (the

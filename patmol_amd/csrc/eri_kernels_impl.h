@@ -28,8 +28,24 @@ __device__ __forceinline__ double dpp_row_shl(double v)
     return __hiloint2double(hi, lo);
 }
 
+template <int K>
+__device__ __forceinline__ float dpp_row_shl(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x100 | (1 << K), 0xf, 0xf, false));
+}
+
 // Lane i + 16 for lanes in even rows (v_permlane16_swap), lane i + 32 for
 // lanes 0..31 (v_permlane32_swap).
+__device__ __forceinline__ float lane_plus16(float v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(v), __float_as_int(v), false, false);
+    return __int_as_float(r[1]);
+}
+__device__ __forceinline__ float lane_plus32(float v)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+    return __int_as_float(r[1]);
+}
 __device__ __forceinline__ double lane_plus16(double v)
 {
     const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
@@ -49,8 +65,8 @@ __device__ __forceinline__ double lane_plus32(double v)
 // is itself a leader of the lower stages, so the partial sums combine.
 // The same for N values at once, stage by stage, so that the cross-lane
 // moves of different values overlap.
-template <int MASK, int N>
-__device__ __forceinline__ void lane_reduce_n(double (&v)[N])
+template <int MASK, typename T, int N>
+__device__ __forceinline__ void lane_reduce_n(T (&v)[N])
 {
     if constexpr ((MASK & 1) != 0) {
 #pragma unroll

@@ -220,6 +220,18 @@ struct Ctx {
     double last_ms = 0.0;
     double class_ms[36] = {0};
     int64_t items_fp32 = 0;
+    int64_t items_skipped = 0;
+    double last_dmax = 0.0;
+    // per-item precision / screening by density-weighted bounds: tile
+    // function ranges, tile-pair max|D|, each item's class range, the
+    // classified lists (per class range: FP64 items from the front, FP32
+    // items from the back) and their counts
+    DevBuf<int> d_tfun0, d_tfunn, d_rfirst, d_rcount, d_cnt;
+    DevBuf<unsigned char> d_item_rng;
+    DevBuf<double> d_tmax;
+    DevBuf<int4> d_cls_items;
+    std::vector<int> cnt_host;
+    bool cls_ready = false;   // tables above match the current items (reset by every table build)
     int64_t prim_kept = 0;
 };
 
@@ -501,6 +513,7 @@ static int build_tables(Ctx &c, int start)
         c.ranges.back().count++;
     }
     if (c.d_items.upload(c.items) || c.d_bound.upload(c.item_bound)) return g_err;
+    c.cls_ready = false;
     c.built = true;
     return 0;
 }
@@ -571,11 +584,137 @@ __global__ void absmax_kernel(const double *__restrict__ p, int64_t n, double *o
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// max|D| over each tile-pair block of the full N x N density D (sorted
+// function order); one workgroup of 64 per tile pair (grid nt x nt)
+__global__ __launch_bounds__(64) void tile_absmax_kernel(const double *__restrict__ D, int N,
+                                                         const int *__restrict__ f0, const int *__restrict__ fn, int nt,
+                                                         double *__restrict__ T)
+{
+    const int t1 = blockIdx.y, t2 = blockIdx.x;
+    const int r0 = f0[t1], nr = fn[t1], c0 = f0[t2], nc = fn[t2];
+    double m = 0.0;
+    for (int e = threadIdx.x; e < nr * nc; e += 64) {
+        const int i = e / nc, j = e % nc;
+        m = fmax(m, fabs(D[(int64_t)(r0 + i) * N + c0 + j]));
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+    if (threadIdx.x == 0) T[(int64_t)t1 * nt + t2] = m;
+}
+
+__global__ void zero_int_kernel(int *__restrict__ p, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+// Each work item's density-weighted bound: its Schwarz bound times the
+// largest max|D| of the six tile-pair blocks its quartets contract with.
+// Below tau the item is dropped; below thr it goes to the FP32 list of its
+// class range (filled from the back), else to the FP64 list (from the
+// front).  Slots are taken per wavefront (one atomic per class range and
+// list, lanes in lane order); the order of the wavefronts depends on the
+// atomics, which does not change the fixed-point J'/K' sums.
+__global__ __launch_bounds__(256) void classify_items_kernel(const int4 *__restrict__ items,
+                                                             const double *__restrict__ bound,
+                                                             const unsigned char *__restrict__ rng, int nitems,
+                                                             const double *__restrict__ T, int nt, double tau,
+                                                             double thr, const int *__restrict__ rfirst,
+                                                             const int *__restrict__ rcount, int *__restrict__ cnt,
+                                                             int4 *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool here = i < nitems;
+    int4 t = make_int4(0, 0, 0, 0);
+    int r = -1, cat = 0;   // cat 0 dropped, 1 FP64, 2 FP32
+    if (here) {
+        t = items[i];
+        r = rng[i];
+        auto tm = [&](int a, int b) { return T[(int64_t)a * nt + b]; };
+        const double m = fmax(fmax(fmax(tm(t.x, t.y), tm(t.z, t.w)), fmax(tm(t.x, t.z), tm(t.x, t.w))),
+                              fmax(tm(t.y, t.z), tm(t.y, t.w)));
+        const double b = bound[i] * m;
+        cat = b < tau ? 0 : (b < thr ? 2 : 1);
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(cat != 0);
+    while (todo) {   // wave-uniform: one pass per class range present in the wave
+        const int leader = __builtin_ctzll(todo);
+        const int rr = __shfl(r, leader, 64);
+        const bool mine = cat != 0 && r == rr;
+        const unsigned long long m64 = __ballot(mine && cat == 1), m32 = __ballot(mine && cat == 2);
+        int b64 = 0, b32 = 0;
+        if (lane == leader) {
+            if (m64) b64 = atomicAdd(&cnt[2 * rr], __popcll(m64));
+            if (m32) b32 = atomicAdd(&cnt[2 * rr + 1], __popcll(m32));
+        }
+        b64 = __shfl(b64, leader, 64);
+        b32 = __shfl(b32, leader, 64);
+        if (mine) {
+            if (cat == 1) out[rfirst[rr] + b64 + __popcll(m64 & below)] = t;
+            else out[rfirst[rr] + rcount[rr] - 1 - (b32 + __popcll(m32 & below))] = t;
+        }
+        todo &= ~__ballot(mine);
+    }
+}
+
+// tables of the item classification (once per built context)
+static int prepare_classify(Ctx &c)
+{
+    if (c.cls_ready) return 0;
+    const int nt = (int)c.tile_first.size(), nr = (int)c.ranges.size();
+    if (nr > 255) return fail(PATMOL_HIP_EUNSUPPORTED, "too many class ranges");
+    std::vector<int> f0(nt), fn(nt), rf(nr), rc(nr);
+    for (int t = 0; t < nt; ++t) {
+        f0[t] = c.shell_soff[c.tile_first[t]];
+        fn[t] = c.tile_size[t] * c.shell_nst[c.tile_first[t]];
+    }
+    std::vector<unsigned char> rng(c.items.size());
+    for (int r = 0; r < nr; ++r) {
+        rf[r] = c.ranges[r].first;
+        rc[r] = c.ranges[r].count;
+        for (int i = 0; i < c.ranges[r].count; ++i) rng[c.ranges[r].first + i] = (unsigned char)r;
+    }
+    if (c.d_tfun0.upload(f0) || c.d_tfunn.upload(fn) || c.d_rfirst.upload(rf) || c.d_rcount.upload(rc) ||
+        c.d_item_rng.upload(rng) || c.d_tmax.alloc((size_t)nt * nt) || c.d_cls_items.alloc(c.items.size()) ||
+        c.d_cnt.alloc(2 * (size_t)nr))
+        return g_err;
+    c.cnt_host.assign(2 * (size_t)nr, 0);
+    c.cls_ready = true;
+    return 0;
+}
+
+// classify every item by the density D (full, sorted order) into the lists
+// of c.d_cls_items; counts into c.cnt_host (the stream is synchronised)
+static int classify_items(Ctx &c, const double *D, double tau, double thr, hipStream_t s)
+{
+    if (prepare_classify(c)) return g_err;
+    const int nt = (int)c.tile_first.size(), nr = (int)c.ranges.size(), ni = (int)c.items.size();
+    hipLaunchKernelGGL(tile_absmax_kernel, dim3(nt, nt), dim3(64), 0, s, D, c.N, c.d_tfun0.p, c.d_tfunn.p, nt,
+                       c.d_tmax.p);
+    hipLaunchKernelGGL(zero_int_kernel, dim3((2 * nr + 255) / 256), dim3(256), 0, s, c.d_cnt.p, 2 * nr);
+    if (ni > 0)
+        hipLaunchKernelGGL(classify_items_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, c.d_items.p, c.d_bound.p,
+                           c.d_item_rng.p, ni, c.d_tmax.p, nt, tau, thr, c.d_rfirst.p, c.d_rcount.p, c.d_cnt.p,
+                           c.d_cls_items.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c.cnt_host.data(), c.d_cnt.p, sizeof(int) * 2 * nr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
 // Launch every class: J' from density DJ, K' from density DK (both full
 // N x N in sorted function order), into the fixed-point copies J, K (zeroed
-// here).  nside side streams (0: everything on s).
+// here).  nside side streams (0: everything on s).  dmax is max|density|
+// (used by the precision split); with delta (a density difference of an
+// incremental build) quartets whose Schwarz bound times dmax is below the
+// Schwarz threshold are skipped as well: whole items by their bound, single
+// quartets in the kernel.  With per_item (DJ == DK, RHF) the precision
+// split and the delta screening use each item's own density-weighted bound
+// (classify_items) instead of max|density| over the whole matrix.
 static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K, double dmax, int nside,
-                  bool timed, hipStream_t s)
+                  bool timed, hipStream_t s, bool delta = false, bool per_item = false)
 {
     const int64_t NN = (int64_t)c.N * c.N * PM_XCD_SLOTS;
     hipLaunchKernelGGL(zero_fx_kernel, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, s, J, NN);
@@ -591,6 +730,8 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         }
         HIPCHK(hipEventRecord(c.ev0, s));
     }
+    const bool cls = per_item && (g_prec_mode == 1 || delta) && dmax > 0.0;
+    if (cls && classify_items(c, DJ, delta ? c.tau : 0.0, g_prec_mode == 1 ? g_prec_thr : -1.0, s)) return g_err;
     c.items_fp32 = 0;
     // Classes are independent: spread them over side streams so small grids
     // overlap.  With timing on, launches stay on the caller's stream so each
@@ -609,17 +750,31 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         HIPCHK(hipEventRecord(c.fork_ev, s));
         for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(c.side[i], c.fork_ev, 0));
     }
+    c.items_skipped = 0;
+    // items are sorted by bound within a class (descending): the FP32 items
+    // and, in a delta build, the skipped items form suffixes
+    auto count_at_least = [&](const ClassRange &r, double t) {
+        return (int)(std::lower_bound(c.item_bound.begin() + r.first, c.item_bound.begin() + r.first + r.count, t,
+                                      [](double b, double v) { return b >= v; }) -
+                     (c.item_bound.begin() + r.first));
+    };
+    const double ktau = delta && dmax > 0.0 ? std::max(c.tau, c.tau / dmax) : c.tau;
     int ci = 0;
     for (const ClassRange &r : c.ranges) {
         hipStream_t ls = nside ? c.side[ci % nside] : s;
-        // items are sorted by bound within a class: FP32 items form a suffix
-        int n64 = r.count;
-        if (g_prec_mode == 1 && dmax > 0.0) {
-            double thr = g_prec_thr / dmax;
-            n64 = (int)(std::lower_bound(c.item_bound.begin() + r.first, c.item_bound.begin() + r.first + r.count, thr,
-                                         [](double b, double t) { return b >= t; }) -
-                        (c.item_bound.begin() + r.first));
+        int nkeep = r.count;
+        if (delta) nkeep = dmax > 0.0 ? count_at_least(r, c.tau / dmax) : 0;
+        int n64 = nkeep;
+        if (g_prec_mode == 1 && dmax > 0.0) n64 = std::min(nkeep, count_at_least(r, g_prec_thr / dmax));
+        // classified lists: FP64 items at the front of the range, FP32 at the back
+        const int4 *list64 = c.d_items.p + r.first, *list32 = c.d_items.p + r.first + n64;
+        if (cls) {
+            n64 = c.cnt_host[2 * ci];
+            nkeep = n64 + c.cnt_host[2 * ci + 1];
+            list64 = c.d_cls_items.p + r.first;
+            list32 = c.d_cls_items.p + r.first + r.count - (nkeep - n64);
         }
+        c.items_skipped += r.count - nkeep;
         JKArgs a;
         a.tile_first = c.d_tile_first.p;
         a.tile_size = c.d_tile_size.p;
@@ -634,7 +789,7 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         a.J = J;
         a.K = K;
         a.N = c.N;
-        a.tau = c.tau;
+        a.tau = ktau;
         a.start_shell = c.start;
         a.rys = g_rys;
         a.item_mol = nullptr;
@@ -642,14 +797,14 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         a.slot_stride = (long long)c.N * c.N;
         if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
         if (n64 > 0) {
-            a.items = c.d_items.p + r.first;
+            a.items = list64;
             a.nitems = n64;
             if (launch_jk(r.bra_pc, r.ket_pc, a, 0, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
-        if (r.count > n64) {
-            a.items = c.d_items.p + r.first + n64;
-            a.nitems = r.count - n64;
-            c.items_fp32 += r.count - n64;
+        if (nkeep > n64) {
+            a.items = list32;
+            a.nitems = nkeep - n64;
+            c.items_fp32 += nkeep - n64;
             if (launch_jk(r.bra_pc, r.ket_pc, a, 1, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
         if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci + 1], s));
@@ -707,23 +862,27 @@ static double device_absmax(Ctx &c, const double *p, int64_t n, hipStream_t s)
 // G += 2J[P] - K[P] on stream s with the work buffers D (N x N doubles) and
 // J, K (PM_XCD_SLOTS x N x N fixed-point each) given by the caller.
 static int rhf_build(Ctx &c, const double *dP, double *dG, double *D, FxAcc *J, FxAcc *K, double dmax, int nside,
-                     bool timed, hipStream_t s)
+                     bool timed, hipStream_t s, bool delta = false)
 {
     const int64_t NN = (int64_t)c.N * c.N, no2 = (int64_t)c.N * (c.N + 1) / 2;
     hipLaunchKernelGGL(unpack_kernel, dim3((NN + 255) / 256), dim3(256), 0, s, dP, D, c.d_fperm.p, c.N);
-    if (run_jk(c, D, D, J, K, dmax, nside, timed, s)) return g_err;
+    if (run_jk(c, D, D, J, K, dmax, nside, timed, s, delta, true)) return g_err;
     hipLaunchKernelGGL(finalize_kernel, dim3((no2 + 255) / 256), dim3(256), 0, s, J, K, dG, c.d_finv.p, c.N, 2.0);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s)
+static int rhf_device(Ctx &c, const double *dP, double *dG, hipStream_t s, bool delta = false)
 {
     if (alloc_work(c, false)) return g_err;
     const int64_t no2 = (int64_t)c.N * (c.N + 1) / 2;
     double dmax = 0.0;
-    if (g_prec_mode == 1) dmax = device_absmax(c, dP, no2, s);
-    return rhf_build(c, dP, dG, c.d_D.p, c.d_J.p, c.d_K.p, dmax, 4, true, s);
+    if (g_prec_mode == 1 || delta) {
+        dmax = device_absmax(c, dP, no2, s);
+        if (dmax < 0.0) return fail(PATMOL_HIP_EDEVICE, "device failure in max|P|");
+    }
+    c.last_dmax = dmax;
+    return rhf_build(c, dP, dG, c.d_D.p, c.d_J.p, c.d_K.p, dmax, 4, true, s, delta);
 }
 
 // ---------------------------------------------------------------------------
@@ -806,8 +965,12 @@ static int uhf_device(Ctx &x, const double *dPA, const double *dPB, double *dGA,
 // device memory, streams and events of a context
 static void release_ctx(Ctx &c)
 {
-    for (auto *b : {&c.d_xyz, &c.d_q, &c.d_bound, &c.d_P, &c.d_PB, &c.d_D, &c.d_D2, &c.d_DJ, &c.d_G, &c.d_G2, &c.d_dmax})
+    for (auto *b : {&c.d_xyz, &c.d_q, &c.d_bound, &c.d_P, &c.d_PB, &c.d_D, &c.d_D2, &c.d_DJ, &c.d_G, &c.d_G2, &c.d_dmax,
+                    &c.d_tmax})
         b->release();
+    for (auto *b : {&c.d_tfun0, &c.d_tfunn, &c.d_rfirst, &c.d_rcount, &c.d_cnt}) b->release();
+    c.d_item_rng.release();
+    c.d_cls_items.release();
     c.d_J.release();
     c.d_K.release();
     for (auto *b : {&c.d_off, &c.d_soff, &c.d_fperm, &c.d_finv, &c.d_orig, &c.d_tile_first, &c.d_tile_size}) b->release();
@@ -1690,3 +1853,27 @@ int patmol_hip_eri_blocks(const ERIFS *e, int nq, const int *quartets, double *o
 }
 
 }  // extern "C"
+
+namespace pm {
+int rhf_gmatrix_delta(const ERIFS *e, const double *dP, double *dG, hipStream_t s, BuildCounts *counts)
+{
+    g_err = PATMOL_HIP_OK;
+    Ctx *c = ctx_of(e);
+    if (!c || !dP || !dG) return fail(PATMOL_HIP_EBADARG, "bad handle or null pointer");
+    if (!c->built) return fail(PATMOL_HIP_ESTATE, "two_electron_ints was not run for this handle");
+    if (rhf_device(*c, dP, dG, s, true)) return g_err;
+    return last_build_counts(e, counts);
+}
+
+int last_build_counts(const ERIFS *e, BuildCounts *counts)
+{
+    Ctx *c = ctx_of(e);
+    if (!counts) return 0;
+    *counts = BuildCounts();
+    if (!c) return 0;
+    counts->items = (int64_t)c->items.size();
+    counts->fp32 = c->items_fp32;
+    counts->skipped = c->items_skipped;
+    return 0;
+}
+}  // namespace pm

@@ -49,6 +49,7 @@ int aux_fail(int code, const char *msg);
 namespace {
 
 int g_scf_density = 1;   // 1: purification inside the loop, 2: the same with separate trace totals, 0: eigensolver
+int g_scf_incremental = 0;   // RHF: 0 full G every iteration, k > 0 full G every k iterations, delta builds between
 
 constexpr int RED_THREADS = 1024;
 
@@ -578,6 +579,27 @@ __global__ __launch_bounds__(256) void tc2_step_kernel(const double *__restrict_
 // ---- Pulay DIIS (scf_accel 3) on the orthonormal-basis Fock matrices ----
 // The commutator error e = F D S - S D F = A - A^T with A = F D S (F, D, S
 // symmetric), column-major n x n.
+// incremental G: dD = P - Pref, Pref = P (packed)
+__global__ void delta_density_kernel(const double *__restrict__ P, double *__restrict__ Pref, double *__restrict__ dD,
+                                     int64_t no2)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= no2) return;
+    const double p = P[k];
+    dD[k] = p - Pref[k];
+    Pref[k] = p;
+}
+
+// incremental G: Gacc += dG, dG = Gacc (packed)
+__global__ void accumulate_g_kernel(double *__restrict__ Gacc, double *__restrict__ dG, int64_t no2)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= no2) return;
+    const double g = Gacc[k] + dG[k];
+    Gacc[k] = g;
+    dG[k] = g;
+}
+
 __global__ void antisym_kernel(const double *__restrict__ A, double *__restrict__ E, int n)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1165,6 +1187,29 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     // below ~128 functions the eigensolver is faster than the ~40 GEMMs of
     // the purification (their launches and trace checks dominate)
     c.purify = g_scf_density >= 1 && n >= 128;
+    // Incremental G (no counterpart in the reference): G is linear in P, so
+    // G(P) = G(Pref) + G(P - Pref); between full builds only the difference
+    // is built, with density-weighted screening and, in mixed precision, a
+    // precision split by max|P - Pref|, which falls as the SCF converges.
+    const int incr = g_scf_incremental;
+    double *Gacc = nullptr, *Pref = nullptr, *dD = nullptr;
+    if (incr > 0) {
+        Gacc = c.mem.get<double>(no2);
+        Pref = c.mem.get<double>(no2);
+        dD = c.mem.get<double>(no2);
+        if (!Gacc || !Pref || !dD) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: device allocation failed");
+    }
+    const dim3 gp((unsigned)((no2 + 255) / 256));
+    int since_full = 0;
+    BuildCounts tally;
+    int full_builds = 0, delta_builds = 0;
+    auto count = [&](const ERIFS *e) {
+        BuildCounts b;
+        last_build_counts(e, &b);
+        tally.items += b.items;
+        tally.fp32 += b.fp32;
+        tally.skipped += b.skipped;
+    };
     const bool log = scf_log();
     double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0, dms = 0.0;
     int iteration = 0, done = 0;
@@ -1175,8 +1220,31 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         (void)hipEventRecord(c.e0, c.s);
         if (hipMemsetAsync(dG, 0, sizeof(double) * no2, c.s) != hipSuccess)
             return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: memset failed");
-        if (patmol_hip_rhf_gmatrix_device(mints, P, dG, c.s)) return patmol_hip_last_error();
-        if (xints && patmol_hip_rhf_gmatrix_device(xints, P, dG, c.s)) return patmol_hip_last_error();
+        if (incr <= 0 || since_full == 0) {
+            if (patmol_hip_rhf_gmatrix_device(mints, P, dG, c.s)) return patmol_hip_last_error();
+            count(mints);
+            if (xints) {
+                if (patmol_hip_rhf_gmatrix_device(xints, P, dG, c.s)) return patmol_hip_last_error();
+                count(xints);
+            }
+            if (incr > 0 && (hipMemcpyAsync(Gacc, dG, sizeof(double) * no2, hipMemcpyDeviceToDevice, c.s) != hipSuccess ||
+                             hipMemcpyAsync(Pref, P, sizeof(double) * no2, hipMemcpyDeviceToDevice, c.s) != hipSuccess))
+                return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: copy failed");
+            ++full_builds;
+        } else {
+            hipLaunchKernelGGL(delta_density_kernel, gp, dim3(256), 0, c.s, P, Pref, dD, (int64_t)no2);
+            BuildCounts b;
+            if (rhf_gmatrix_delta(mints, dD, dG, c.s, &b)) return patmol_hip_last_error();
+            count(mints);
+            if (xints) {
+                if (rhf_gmatrix_delta(xints, dD, dG, c.s, &b)) return patmol_hip_last_error();
+                count(xints);
+            }
+            hipLaunchKernelGGL(accumulate_g_kernel, gp, dim3(256), 0, c.s, Gacc, dG, (int64_t)no2);
+            if (hipGetLastError() != hipSuccess) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: launch failed");
+            ++delta_builds;
+        }
+        if (incr > 0) since_full = (since_full + 1) % incr;
         (void)hipEventRecord(c.e1, c.s);
         // P2 <- P1 <- P, new P into the freed buffer (rhfscf.c:43-46)
         double *fresh = P2;
@@ -1230,6 +1298,11 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     res->other_ms = dms;
     res->total_ms = tot;
     res->purify_fallbacks = c.purify_fallbacks;
+    res->full_builds = full_builds;
+    res->delta_builds = delta_builds;
+    res->items = (double)tally.items;
+    res->items_fp32 = (double)tally.fp32;
+    res->items_skipped = (double)tally.skipped;
     // orbitals of the last Fock matrix (the loop's densities came from
     // purification)
     if (c.purify && (evals || Cmat) && !c.orbitals(dH, dG)) return scf_fail(c, PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf");
@@ -1404,6 +1477,7 @@ extern "C" int patmol_hip_diag_fock(const double *Smat, const double *Fmat, int 
 }
 
 extern "C" void patmol_hip_set_scf_density(int mode) { pm::g_scf_density = (mode == 0 || mode == 2) ? mode : 1; }
+extern "C" void patmol_hip_set_scf_incremental(int period) { pm::g_scf_incremental = period > 0 ? period : 0; }
 
 // RHF of every molecule of a batch, all iterating together (see patmol_hip.h)
 extern "C" int patmol_hip_batch_rhf_scf(PatmolHipBatch *b, const double *S, const double *H, const int *nocc,

@@ -65,8 +65,15 @@ void patmol_hip_set_strict(int strict);
 /* Schwarz threshold on Q_IJ*Q_KL (default 1e-12; 0 keeps every quartet). */
 void patmol_hip_set_schwarz_threshold(double tau);
 /* mode 0: all quartets in FP64.  mode 1: work items whose bound
- * Q_IJ*Q_KL*max|P| is below `threshold` run in FP32 (FP64 accumulation). */
+ * Q_IJ*Q_KL*max|P| is below `threshold` run in FP32 (FP32 integrals and
+ * products, FP64 block sums, fixed-point accumulation), max|P| taken over
+ * the six density blocks each item contracts with (RHF builds) or over the
+ * whole density (UHF, batches); only in the classes whose FP32 kernel the
+ * autotuner measured faster than their FP64 kernel (jk3_tuned.inc). */
 void patmol_hip_set_precision(int mode, double threshold);
+/* Autotuning aid (no counterpart in the reference): 1 sends the items below
+ * the threshold to FP32 in every class, 0 (default) follows jk3_tuned.inc. */
+void patmol_hip_set_fp32_policy(int every_class);
 
 /* ---- device-resident G build (inputs already in HBM) ---- */
 /* d_P: packed half-density (no2 doubles, device); d_G: packed G (device),

@@ -25,8 +25,11 @@ import numpy as np
 from . import systems  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_HIP = os.path.join(HERE, "libpatmol_hip.so")
-LIB_HOST = os.path.join(HERE, "libpatmol_host.so")
+# PATMOL_AMD_LIB_DIR points at another build of the two libraries (the
+# autotuner's candidate builds); by default the in-tree ones
+_LIB_DIR = os.environ.get("PATMOL_AMD_LIB_DIR") or HERE
+LIB_HIP = os.path.join(_LIB_DIR, "libpatmol_hip.so")
+LIB_HOST = os.path.join(_LIB_DIR, "libpatmol_host.so")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -114,6 +117,7 @@ def lib():
         G.patmol_hip_set_strict.argtypes = [C.c_int]
         G.patmol_hip_set_schwarz_threshold.argtypes = [C.c_double]
         G.patmol_hip_set_precision.argtypes = [C.c_int, C.c_double]
+        G.patmol_hip_set_fp32_policy.argtypes = [C.c_int]
         G.patmol_hip_rhf_gmatrix_device.restype = C.c_int
         G.patmol_hip_rhf_gmatrix_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         G.patmol_hip_rhf_gmatrix_device_noalloc.restype = C.c_int
@@ -205,7 +209,7 @@ def exported_symbols():
     """Names the C ABI declares (include/patmol_hip.h)."""
     return ["init_ERIFS", "clean_ERIFS", "two_electron_ints", "rhf_form_gmatrix", "uhf_form_gmatrix",
             "dpm_ints", "one_electron_ints", "patmol_hip_last_error", "patmol_hip_error_string", "patmol_hip_set_strict",
-            "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision",
+            "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision", "patmol_hip_set_fp32_policy",
             "patmol_hip_rhf_gmatrix_device", "patmol_hip_rhf_gmatrix_device_noalloc", "patmol_hip_stats", "patmol_hip_class_stats",
             "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",

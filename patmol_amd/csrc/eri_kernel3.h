@@ -19,7 +19,7 @@
 // Classes whose 2-D tables for all roots would not fit the register file
 // digest one Rys root at a time (primitive_quartet_rootwise); that choice, the
 // density-read fence and the register budget per class are measured
-// (jk3_shape).
+// (jk3_code).
 #pragma once
 #include <type_traits>
 #include "eri_kernels.h"
@@ -36,96 +36,74 @@ struct K3 {
     static constexpr int PPCAP = 128;  // staged ket primitive pairs
 };
 
-// Per-class launch shape, from timing every class on (H2O)8 with both
-// digestion orders, with and without the density-read fence, at 1 and 2
-// waves per SIMD, with and without splitting the bra components over the
-// waves: code = rys_global * 256 + rys_lds * 128 + ksplit * 64 +
-// (split_b - 1) * 32 + (split_a - 1) * 16 + fence * 8 + waves * 2 + rootwise
-// (rys_lds: the Rys coefficients of the class's root count are staged in LDS
+// Per-class launch shape, one code per class and precision:
+// code = rys_global * 256 + rys_lds * 128 + ksplit * 64 + (split_b - 1) * 32
+// + (split_a - 1) * 16 + fence * 8 + waves * 2 + rootwise
+// (rootwise: digest one Rys root at a time; fence: the density-read fence of
+// the rootwise digestion; waves: waves per SIMD the register budget is set
+// for; split_a / split_b: the bra components split over the waves; ksplit:
+// every item is two workgroups, each taking half of the ket tile's K shells;
+// rys_lds: the Rys coefficients of the class's root count are staged in LDS
 // also above three roots; rys_global: they are read from global memory even
 // at three roots or fewer, which keeps two workgroups' LDS within a CU's
-// 160 KB; ksplit: every item is two workgroups, each taking half of the ket
-// tile's K shells).
-constexpr int jk3_shape(int la, int lb, int lc, int ld)
+// 160 KB).  The codes are the ones patmol_amd/autotune.py measured fastest
+// per class, written to jk3_tuned.inc (found through the include path, so a
+// tuning build can put its candidate table first).
+template <int LA, int LB, int LC, int LD, typename R>
+constexpr int jk3_code()
 {
-    const int c = ((la * 3 + lb) * 3 + lc) * 3 + ld;
-    switch (c) {
-    case 0: return 1 * 2 + 1;    // (ss|ss)
-    case 3: return 2 * 2 + 1;    // (ss|ps), reversed (ps|ss)
-    case 6: return 2 * 2 + 0;    // (ss|ds), reversed (ds|ss)
-    case 33: return 2 * 2 + 1;   // (ps|ds), reversed (ds|ps)
-    case 27: return 2 * 2 + 1;   // (ps|ss)
-    case 30: return 2 * 2 + 0;   // (ps|ps)
-    case 36: return 2 * 2 + 0;   // (pp|ss)
-    case 39: return 2 * 2 + 1;   // (pp|ps)
-    case 40: return 1 * 2 + 1;   // (pp|pp)
-    case 54: return 2 * 2 + 0;   // (ds|ss)
-    case 57: return 2 * 2 + 1;   // (ds|ps)
-    case 58: return 256 + 16 + 2 * 2 + 1;   // (ds|pp)
-    case 60: return 1 * 2 + 0;   // (ds|ds)
-    case 63: return 2 * 2 + 1;   // (dp|ss)
-    case 66: return 1 * 2 + 0;   // (dp|ps)
-    case 67: return 1 * 2 + 1;   // (dp|pp)
-    case 69: return 1 * 2 + 1;   // (dp|ds)
-    case 70: return 64 + 16 + 8 + 1 * 2 + 1;   // (dp|dp)
-    case 72: return 1 * 2 + 0;   // (dd|ss)
-    case 75: return 1 * 2 + 1;   // (dd|ps)
-    case 76: return 128 + 64 + 32 + 16 + 1 * 2 + 1;   // (dd|pp)
-    case 78: return 128 + 32 + 1 * 2 + 1;   // (dd|ds)
-    case 79: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dp)
-    case 80: return 32 + 16 + 8 + 1 * 2 + 1;   // (dd|dd) (runs jk_kernel)
-    default: return 1 * 2 + 1;
-    }
+    const int t = jk3_tuned_shape(jk3_class_index(LA, LB, LC, LD), std::is_same<R, float>::value);
+    return t >= 0 ? t : 1 * 2 + 1;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr bool jk3_rootwise()
 {
-    return jk3_shape(LA, LB, LC, LD) & 1;
+    return jk3_code<LA, LB, LC, LD, R>() & 1;
 }
 
 // The Rys coefficient section of the class's root count is kept in LDS for
 // the classes with up to three roots (7 / 17 / 30 KB) unless their shape
 // says otherwise, and for the classes whose shape asks for it (4 roots:
 // 47 KB)
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr bool jk3_rys_lds()
 {
-    constexpr int code = jk3_shape(LA, LB, LC, LD);
+    constexpr int code = jk3_code<LA, LB, LC, LD, R>();
     return ((LA + LB + LC + LD) / 2 + 1 <= 3 && !((code >> 8) & 1)) || ((code >> 7) & 1);
 }
 
-// Workgroups per work item (jk3_shape's ksplit): the items of the largest
-// classes take hundreds of microseconds each and there are only a few
-// hundred of them, so halving them evens out the end of the launch.
-template <int LA, int LB, int LC, int LD>
+// Workgroups per work item (ksplit): the items of the largest classes take
+// hundreds of microseconds each and there are only a few hundred of them,
+// so halving them evens out the end of the launch.
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr int jk3_wg_per_item()
 {
-    return ((jk3_shape(LA, LB, LC, LD) >> 6) & 1) ? 2 : 1;
+    return ((jk3_code<LA, LB, LC, LD, R>() >> 6) & 1) ? 2 : 1;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr bool jk3_block_fence()
 {
-    return (jk3_shape(LA, LB, LC, LD) >> 3) & 1;
+    return (jk3_code<LA, LB, LC, LD, R>() >> 3) & 1;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr int jk3_split_a()
 {
-    return ((jk3_shape(LA, LB, LC, LD) >> 4) & 1) + 1;
+    return ((jk3_code<LA, LB, LC, LD, R>() >> 4) & 1) + 1;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr int jk3_split_b()
 {
-    return ((jk3_shape(LA, LB, LC, LD) >> 5) & 1) + 1;
+    return ((jk3_code<LA, LB, LC, LD, R>() >> 5) & 1) + 1;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 constexpr int jk3_waves_per_eu()
 {
-    return (jk3_shape(LA, LB, LC, LD) >> 1) & 3;
+    return (jk3_code<LA, LB, LC, LD, R>() >> 1) & 3;
 }
 
 // Cross-lane sums: lane bits 0-3 by DPP inside rows of 16; bits 4-5 (other
@@ -148,7 +126,7 @@ __device__ __forceinline__ void lds_acc(double *o, double v)
 }
 
 template <int LA, int LB, int LC, int LD, typename R>
-__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(jk3_waves_per_eu<LA, LB, LC, LD>()))) void jk_kernel3(JKArgs a)
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(jk3_waves_per_eu<LA, LB, LC, LD, R>()))) void jk_kernel3(JKArgs a)
 {
     using S = K3<LA, LB, LC, LD>;
     constexpr int TI = S::TI, TJ = S::TJ, TK = S::TK, TL = S::TL;
@@ -166,7 +144,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     // of NAH bra-a components x NBH bra-b components of every quartet (and
     // only that block's accumulators); without a split the four waves own
     // different K.  Waves w with the same w / SPLIT (the K group) share K.
-    constexpr int SA = jk3_split_a<LA, LB, LC, LD>(), SB = jk3_split_b<LA, LB, LC, LD>();
+    constexpr int SA = jk3_split_a<LA, LB, LC, LD, R>(), SB = jk3_split_b<LA, LB, LC, LD, R>();
     constexpr int SPLIT = SA * SB, KSTRIDE = 4 / SPLIT;
     constexpr int NAH = NA / SA, NBH = NB / SB;
     static_assert(NA % SA == 0 && NB % SB == 0 && 4 % SPLIT == 0, "bra component split");
@@ -204,7 +182,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     // The Rys coefficient section of this class's root count kept in LDS
     // (the lookups are otherwise global-memory latency on the critical path
     // of every step)
-    constexpr bool RYS_LDS = std::is_same<R, double>::value && jk3_rys_lds<LA, LB, LC, LD>();
+    constexpr bool RYS_LDS = std::is_same<R, double>::value && jk3_rys_lds<LA, LB, LC, LD, R>();
     // only the NR-root section of the table (rys_eval<NR, true>)
     constexpr int RYS_LDS_N = RYS_LDS ? rys_tmax(S::NR) * rys_stride(S::NR) : 2;
     __shared__ __attribute__((aligned(16))) double sRys[RYS_LDS_N];
@@ -213,7 +191,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
 
     // with KS workgroups per item, workgroups KS i .. KS i + KS - 1 share
     // item i and each takes one range of the K shells of the ket tile
-    constexpr int KS = jk3_wg_per_item<LA, LB, LC, LD>();
+    constexpr int KS = jk3_wg_per_item<LA, LB, LC, LD, R>();
     const int item = (int)(blockIdx.x / (unsigned)KS), ksub = (int)(blockIdx.x % (unsigned)KS);
     if (item >= a.nitems) return;
     const int4 it = a.items[item];
@@ -450,8 +428,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
                     kbd[Bl * ND + D_] += v * DIK[A_ * SK + C_];
                 };
                 auto quartet = [&](const PrimPair &bp, const PrimPair &kp, R sc) {
-                    if constexpr (jk3_rootwise<LA, LB, LC, LD>())
-                        primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD>(), A0, A0 + NAH,
+                    if constexpr (jk3_rootwise<LA, LB, LC, LD, R>())
+                        primitive_quartet_rootwise<R, LA, LB, LC, LD, jk3_block_fence<LA, LB, LC, LD, R>(), A0, A0 + NAH,
                                                    B0, B0 + NBH, RYS_LDS>(bp, kp, bg, kg, rt, sc, digest);
                     else
                         primitive_quartet<R, LA, LB, LC, LD, true, A0, A0 + NAH, B0, B0 + NBH, RYS_LDS>(bp, kp, bg, kg,

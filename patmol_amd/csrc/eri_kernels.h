@@ -13,6 +13,15 @@ constexpr int PM_XCD_SLOTS = 8;
 // Shells per tile for angular momentum l (a power of two; the ket tile pair
 // TK*TL must divide 64).
 __host__ __device__ constexpr int tile_of(int l) { return l <= 1 ? 8 : 4; }
+// index of the class (la lb | lc ld) in the tuned tables
+__host__ __device__ constexpr int jk3_class_index(int la, int lb, int lc, int ld) { return ((la * 3 + lb) * 3 + lc) * 3 + ld; }
+// (l, l') of pair class pc = l (l + 1) / 2 + l'
+__host__ __device__ constexpr int pair_class_la(int pc) { return pc >= 3 ? 2 : (pc >= 1 ? 1 : 0); }
+__host__ __device__ constexpr int pair_class_lb(int pc) { return pc - pair_class_la(pc) * (pair_class_la(pc) + 1) / 2; }
+}  // namespace pm
+// the measured launch shapes and FP32 policy (jk3_tuned_shape, jk3_fp32_pays)
+#include <jk3_tuned.inc>
+namespace pm {
 
 // A J'/K' accumulator element in fixed point: the value is
 // hi * 2^-FX_HI_BITS + lo * 2^-FX_LO_BITS.  Every contribution is rounded

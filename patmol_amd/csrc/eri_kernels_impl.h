@@ -461,11 +461,14 @@ inline void launch_jk_t(const JKArgs &args, int prec, hipStream_t s)
     } else {
         // every other class: the LDS-staged kernel, possibly several
         // workgroups per item (jk3_wg_per_item)
-        const dim3 grid((unsigned)args.nitems * (unsigned)jk3_wg_per_item<LA, LB, LC, LD>());
         if (prec == 0)
-            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, double>), grid, block, 0, s, args);
+            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, double>),
+                               dim3((unsigned)args.nitems * (unsigned)jk3_wg_per_item<LA, LB, LC, LD, double>()), block,
+                               0, s, args);
         else
-            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, float>), grid, block, 0, s, args);
+            hipLaunchKernelGGL((jk_kernel3<LA, LB, LC, LD, float>),
+                               dim3((unsigned)args.nitems * (unsigned)jk3_wg_per_item<LA, LB, LC, LD, float>()), block,
+                               0, s, args);
     }
 }
 

@@ -82,6 +82,17 @@ static int fail(int code, const char *fmt, ...)
 static double g_tau = 1e-12;
 static int g_prec_mode = 0;
 static double g_prec_thr = 1e-6;
+// 0: items below the threshold run in FP32 only in the classes whose FP32
+// kernel was measured faster (jk3_fp32_pays); 1: in every class (tuning)
+static int g_fp32_policy = 0;
+
+// precision of the items a class would run in FP32: 1 where FP32 pays
+static int fp32_prec(int bra_pc, int ket_pc)
+{
+    const int k = jk3_class_index(pair_class_la(bra_pc), pair_class_lb(bra_pc), pair_class_la(ket_pc),
+                                  pair_class_lb(ket_pc));
+    return (g_fp32_policy == 1 || jk3_fp32_pays(k)) ? 1 : 0;
+}
 static bool g_timing = false;
 
 // ---------------------------------------------------------------------------
@@ -802,10 +813,11 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
             if (launch_jk(r.bra_pc, r.ket_pc, a, 0, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
         if (nkeep > n64) {
+            const int p32 = fp32_prec(r.bra_pc, r.ket_pc);
             a.items = list32;
             a.nitems = nkeep - n64;
-            c.items_fp32 += nkeep - n64;
-            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
+            if (p32) c.items_fp32 += nkeep - n64;
+            if (launch_jk(r.bra_pc, r.ket_pc, a, p32, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
         if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci + 1], s));
         HIPCHK(hipGetLastError());
@@ -1160,7 +1172,8 @@ static int batch_build(PatmolHipBatch &b, hipStream_t s)
             a.items = b.d_items.p + r.first + n64;
             a.item_mol = b.d_item_mol.p + r.first + n64;
             a.nitems = r.count - n64;
-            if (launch_jk(r.bra_pc, r.ket_pc, a, 1, s)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel");
+            if (launch_jk(r.bra_pc, r.ket_pc, a, fp32_prec(r.bra_pc, r.ket_pc), s))
+                return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel");
         }
     }
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, b.d_G.p, n2);
@@ -1207,6 +1220,7 @@ void patmol_hip_set_precision(int mode, double thr)
     g_prec_mode = mode ? 1 : 0;
     if (thr > 0) g_prec_thr = thr;
 }
+void patmol_hip_set_fp32_policy(int every_class) { g_fp32_policy = every_class ? 1 : 0; }
 void patmol_hip_set_timing(int on) { g_timing = on != 0; }
 
 // erifs.c:32-60: the handle keeps the reference's public fields; nfiles
@@ -1615,7 +1629,9 @@ int patmol_hip_batch_finalize(PatmolHipBatch *b, int prec_mode, double threshold
     if (b->finalized) return fail(PATMOL_HIP_ESTATE, "patmol_hip_batch_finalize: already finalised");
     // per class: FP64 items first, each part by bound descending (the work
     // list of every molecule merged into one queue per class)
-    auto fp32 = [&](const PatmolHipBatch::It &x) { return prec_mode == 1 && x.bound * dmax < threshold; };
+    auto fp32 = [&](const PatmolHipBatch::It &x) {
+        return prec_mode == 1 && x.bound * dmax < threshold && fp32_prec(x.bpc, x.kpc) == 1;
+    };
     std::stable_sort(b->its.begin(), b->its.end(), [&](const PatmolHipBatch::It &x, const PatmolHipBatch::It &y) {
         if (x.bpc != y.bpc) return x.bpc > y.bpc;
         if (x.kpc != y.kpc) return x.kpc > y.kpc;

@@ -37,6 +37,8 @@ METRIC = "ERI+Fock s/SCF-iter, mixed vs FP64 on 1 GPU at |dE|<=1e-6 Eh; quartets
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix)
 TAU = 1e-12
 MIXED_THR = 1e-6
+# incremental SCF: a full G build every INCR_PERIOD iterations
+INCR_PERIOD = 8
 # the oracle's direct build keeps one canonical tuple in CPU_SAMPLE_MOD
 # ((I + 3J + 7K + 13L) mod CPU_SAMPLE_MOD == 0): about 10-20 s of CPU work
 CPU_SAMPLE_MOD = 7
@@ -172,20 +174,32 @@ def measure(torch, pm, system, key, steps, warmup, scf):
         system.scf_a = 3
         pm.load_system(system)
         runs = {}
-        for mode in (0, 1):
-            pm.set_options(precision=mode, mixed_threshold=MIXED_THR)
+        # (precision, incremental period): the reference's algorithm (a full
+        # G every iteration) in FP64 and mixed, then incremental G builds
+        # (a full build every INCR_PERIOD iterations, density differences
+        # in between) in FP64 and mixed
+        for key, mode, incr in (("fp64", 0, 0), ("mixed", 1, 0), ("fp64_incremental", 0, INCR_PERIOD),
+                                ("mixed_incremental", 1, INCR_PERIOD)):
+            pm.set_options(precision=mode, mixed_threshold=MIXED_THR, scf_incremental=incr)
             r, _, _ = pm.rhf_scf_device(store=store)
             nupd = r["iterations"] + 1
-            runs[mode] = {"energy": r["energy"], "iterations": r["iterations"], "converged": r["converged"],
-                          "ms_per_iteration": r["total_ms"] / nupd, "gbuild_ms_per_iteration": r["gbuild_ms"] / nupd,
-                          "other_ms_per_iteration": r["other_ms"] / nupd, "purify_fallbacks": r["purify_fallbacks"]}
-            log(f"  device SCF, DIIS ({'mixed' if mode else 'FP64'}): E {r['energy']:.10f} in {r['iterations']} "
-                f"iterations, {runs[mode]['ms_per_iteration']:.2f} ms/iteration (G "
-                f"{runs[mode]['gbuild_ms_per_iteration']:.2f})")
-        pm.set_options(precision=0)
-        out["scf_fp64"] = runs[0]
-        out["scf_mixed"] = runs[1]
-        out["dE_scf_mixed_vs_fp64"] = abs(runs[1]["energy"] - runs[0]["energy"])
+            runs[key] = {"energy": r["energy"], "iterations": r["iterations"], "converged": r["converged"],
+                         "ms_per_iteration": r["total_ms"] / nupd, "gbuild_ms_per_iteration": r["gbuild_ms"] / nupd,
+                         "other_ms_per_iteration": r["other_ms"] / nupd, "purify_fallbacks": r["purify_fallbacks"],
+                         "full_builds": r["full_builds"], "delta_builds": r["delta_builds"],
+                         "items_fp32_frac": r["items_fp32"] / max(r["items"], 1.0),
+                         "items_skipped_frac": r["items_skipped"] / max(r["items"], 1.0)}
+            log(f"  device SCF, DIIS ({key}): E {r['energy']:.10f} in {r['iterations']} iterations, "
+                f"{runs[key]['ms_per_iteration']:.2f} ms/iteration (G {runs[key]['gbuild_ms_per_iteration']:.2f}); "
+                f"FP32 {runs[key]['items_fp32_frac']:.2f}, skipped {runs[key]['items_skipped_frac']:.2f} of the items")
+        pm.set_options(precision=0, scf_incremental=0)
+        e0 = runs["fp64"]["energy"]
+        for key in runs:
+            out[f"scf_{key}"] = runs[key]
+            runs[key]["dE_vs_fp64"] = abs(runs[key]["energy"] - e0)
+        out["dE_scf_mixed_vs_fp64"] = runs["mixed"]["dE_vs_fp64"]
+        g0 = runs["fp64"]["gbuild_ms_per_iteration"]
+        out["scf_g_speedup_vs_fp64"] = {k: g0 / runs[k]["gbuild_ms_per_iteration"] for k in runs}
     store.close()
     return out, P
 
@@ -207,14 +221,18 @@ def time_batch_launches(torch, batch, steps, warmup):
     return ev0.elapsed_time(ev1) / steps
 
 
-def measure_batch(torch, pm, systems, steps, warmup, nmol=64, seed=1002, scf=True):
-    """The seeded batch of `nmol` H/C/N/O molecules (2-20 atoms): the plain
-    FP64 path (one drop-in G build per molecule, molecule after molecule)
-    against the batched G build (all molecules in one graph) in FP64 and in
-    mixed precision (Schwarz * max|P| < 1e-6 in FP32), each molecule's
-    batched G checked bitwise against its own FP64 build; then the batched
-    RHF of every molecule (DIIS) in both modes with per-molecule |dE|."""
+def measure_batch(torch, pm, systems, steps, warmup, nmol=64, seed=1002, scf=True, smallest=0):
+    """The seeded batch of `nmol` H/C/N/O molecules (2-20 atoms; with
+    smallest > 0 only that many of the smallest): the plain FP64 path (one
+    drop-in G build per molecule, molecule after molecule) against the
+    batched G build (all molecules in one graph) in FP64 and in mixed
+    precision (Schwarz * max|P| < 1e-6 in FP32), each molecule's batched G
+    checked bitwise against its own FP64 build; optionally the batched RHF
+    of every molecule (DIIS) in both modes with per-molecule |dE|."""
     mols = systems.molecule_batch(nmol, seed)
+    if smallest:
+        mols = sorted(mols, key=lambda m: m.norb)[:smallest]
+        nmol = len(mols)
     pm.set_options(tau=TAU, precision=0, timing=False, strict=False)
     stream = torch.cuda.current_stream()
     guesses, kept, single_ms, singles = [], 0, 0.0, []
@@ -234,7 +252,8 @@ def measure_batch(torch, pm, systems, steps, warmup, nmol=64, seed=1002, scf=Tru
     dmax = float(np.abs(Pcat).max())
     log(f"batch of {nmol}: {sum(m.norb for m in mols)} functions, {kept} kept shell quartets; plain FP64 path "
         f"(one build per molecule) {single_ms:.2f} ms")
-    out = {"workload": f"batch of {nmol} seeded H/C/N/O molecules (seed {seed}), "
+    out = {"workload": f"batch of {nmol} seeded H/C/N/O molecules (seed {seed}"
+                       f"{f', the {nmol} smallest of 64' if smallest else ''}), "
                        f"{min(m.norb for m in mols)}-{max(m.norb for m in mols)} functions each",
            "molecules": nmol, "functions_total": int(sum(m.norb for m in mols)),
            "shell_quartets_kept": int(kept), "ms_plain_fp64_one_build_per_molecule": single_ms}
@@ -409,6 +428,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--profile-build", default="", metavar="SYSTEM")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-molecule batch block")
+    ap.add_argument("--batch-scf", action="store_true",
+                    help="also converge every molecule of the batch (FP64 and mixed, about two minutes each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the G build runs on a single MI355X "
@@ -427,12 +448,15 @@ def main():
         print("no GPU visible", file=sys.stderr)
         return 1
     if args.batch_only:
-        print(json.dumps({"batch": measure_batch(torch, pm, systems, args.steps, args.warmup)}))
+        print(json.dumps({"batch": measure_batch(torch, pm, systems, args.steps, args.warmup, scf=args.batch_scf),
+                          "batch_16_smallest": measure_batch(torch, pm, systems, args.steps, args.warmup, scf=False,
+                                                             smallest=16)}))
         return 0
     head_sys = systems.water16()
     head, P16 = measure(torch, pm, head_sys, "water16", args.steps, args.warmup, scf=True)
     sec, _ = measure(torch, pm, systems.water8(), "water8", args.steps, args.warmup, scf=False)
-    batch = measure_batch(torch, pm, systems, args.steps, args.warmup)
+    batch = measure_batch(torch, pm, systems, args.steps, args.warmup, scf=args.batch_scf)
+    batch_small = measure_batch(torch, pm, systems, args.steps, args.warmup, scf=False, smallest=16)
 
     dE = head["dE_scf_mixed_vs_fp64"]
     use_mixed = dE <= 1e-6 and head["fp32_items"] > 0 and head["ms_per_build_mixed"] < head["ms_per_build_fp64"]
@@ -485,6 +509,7 @@ def main():
         "headline": head,
         "secondary_water8": sec,
         "batch_64": batch,
+        "batch_16_smallest": batch_small,
     }
     if not args.no_cpu:
         log("cpu baseline (oracle direct build, bounded sample) ...")

@@ -396,10 +396,11 @@ def calibrate_copy(nbytes):
     return 0
 
 
-def profile_build(name):
-    """One FP64 G build of `name` with the class kernels serialised on one
-    stream (after one warm-up build), for rocprofv3 kernel-trace and counter
-    runs."""
+def profile_build(name, fp32=False):
+    """One G build of `name` with the class kernels serialised on one stream
+    (after one warm-up build), for rocprofv3 kernel-trace and counter runs:
+    FP64, or with fp32 every item in FP32 in the classes whose FP32 kernel
+    pays (the tuned policy)."""
     import torch
     import patmol_amd as pm
     from patmol_amd import systems
@@ -410,12 +411,16 @@ def profile_build(name):
     S, T, H = pm.one_electron_ints()
     P = core_guess_density(S, H, system.norb, system.nelec // 2)
     store = pm.EriStore()
+    if fp32:
+        pm.set_options(precision=1, mixed_threshold=1e300)
     stream = torch.cuda.current_stream()
     dP = torch.tensor(P, dtype=torch.float64, device="cuda")
     dG = torch.zeros_like(dP)
     ms = time_builds(torch, store, dP, dG, 1, 1, stream)
+    st = store.stats()
     store.close()
-    print(json.dumps({"profile_build": name, "ms": ms}))
+    pm.set_options(precision=0)
+    print(json.dumps({"profile_build": name, "fp32": fp32, "ms": ms, "items_fp32": st["work_items_fp32"]}))
     return 0
 
 
@@ -427,6 +432,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--profile-build", default="", metavar="SYSTEM")
+    ap.add_argument("--profile-fp32", action="store_true", help="with --profile-build: every item in FP32")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-molecule batch block")
     ap.add_argument("--batch-scf", action="store_true",
                     help="also converge every molecule of the batch (FP64 and mixed, about two minutes each)")
@@ -438,7 +444,7 @@ def main():
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.profile_build:
-        return profile_build(args.profile_build)
+        return profile_build(args.profile_build, args.profile_fp32)
 
     import torch
     import patmol_amd as pm

@@ -276,6 +276,10 @@ int64_t patmol_hip_work_items(const ERIFS *tfiles, int *tiles4, int *classes2, d
  * indices (4 per quartet); out receives, per quartet, the nst(I)*nst(J)*
  * nst(K)*nst(L) values in (i,j,k,l) row-major order. */
 int patmol_hip_eri_blocks(const ERIFS *tfiles, int nq, const int *quartets, double *out);
+/* The same with prec 1: the integrals computed in the FP32 arithmetic of the
+ * mixed-precision J/K kernels (FP32 roots, recursions and products; the sum
+ * over primitive quartets in FP64), for the per-class FP32 error. */
+int patmol_hip_eri_blocks_prec(const ERIFS *tfiles, int nq, const int *quartets, double *out, int prec);
 
 /* Wall time of the last G build measured with HIP events around the kernel
  * launches (milliseconds), and of each class kernel (array of 36, indexed

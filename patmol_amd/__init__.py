@@ -132,6 +132,8 @@ def lib():
         G.patmol_hip_work_items.argtypes = [C.c_void_p] + [C.c_void_p] * 7
         G.patmol_hip_eri_blocks.restype = C.c_int
         G.patmol_hip_eri_blocks.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        G.patmol_hip_eri_blocks_prec.restype = C.c_int
+        G.patmol_hip_eri_blocks_prec.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int]
         G.patmol_hip_last_timing.restype = C.c_int
         G.patmol_hip_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), _dp]
         G.patmol_hip_set_timing.argtypes = [C.c_int]
@@ -212,7 +214,7 @@ def exported_symbols():
             "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision", "patmol_hip_set_fp32_policy",
             "patmol_hip_rhf_gmatrix_device", "patmol_hip_rhf_gmatrix_device_noalloc", "patmol_hip_stats", "patmol_hip_class_stats",
             "patmol_hip_pair_schwarz",
-            "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_last_timing",
+            "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_eri_blocks_prec", "patmol_hip_last_timing",
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
             "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "patmol_hip_set_scf_incremental", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
             "patmol_hip_write_erints", "patmol_hip_rhf_gmatrix_erints", "patmol_hip_rhf_gmatrix_erint_file",
@@ -382,14 +384,16 @@ class EriStore:
         return dict(tiles=t4[:4 * n].reshape(n, 4), classes=c2[:2 * n].reshape(n, 2), bounds=b[:n],
                     sorted_orig=so, tile_first=tf[:k], tile_size=ts[:k])
 
-    def eri_blocks(self, quartets: Sequence[Sequence[int]], nst: Sequence[int]):
+    def eri_blocks(self, quartets: Sequence[Sequence[int]], nst: Sequence[int], fp32: bool = False):
+        """Contracted ERI blocks of the listed shell quartets (original shell
+        indices); fp32: in the FP32 arithmetic of the mixed J/K kernels."""
         _, L = lib()
         q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1))
         nq = len(q) // 4
         sizes = [int(nst[a] * nst[b] * nst[c] * nst[d]) for a, b, c, d in np.asarray(quartets).reshape(-1, 4)]
         out = np.zeros(max(sum(sizes), 1))
-        if L.patmol_hip_eri_blocks(self.h, nq, q, out):
-            _check("patmol_hip_eri_blocks")
+        if L.patmol_hip_eri_blocks_prec(self.h, nq, q, out, int(fp32)):
+            _check("patmol_hip_eri_blocks_prec")
         res, o = [], 0
         for (a, b, c, d), s in zip(np.asarray(quartets).reshape(-1, 4), sizes):
             res.append(out[o:o + s].reshape(nst[a], nst[b], nst[c], nst[d]))

@@ -111,6 +111,7 @@ struct EriOutArgs {
     const PrimPair *pp;
     double *out;
     RysTables rys;
+    int prec;                  // 0 FP64, 1 the FP32 arithmetic of the mixed J/K kernels
 };
 
 struct SchwarzArgs {

@@ -338,7 +338,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
 // ---------------------------------------------------------------------------
 // ERI export: all Cartesian components of listed shell quartets (tests).
 // ---------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD>
+// R = float: the FP32 arithmetic of the mixed-precision J/K kernels (for the
+// per-class error of FP32 integrals), summed over primitives in FP64
+template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(64) void eri_out_kernel(EriOutArgs a)
 {
     constexpr int NCOMP = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
@@ -349,28 +351,28 @@ __global__ __launch_bounds__(64) void eri_out_kernel(EriOutArgs a)
     for (int i = 0; i < NCOMP; ++i) out[i] = 0.0;
     const int Ip = I >= J ? I : J, Jp = I >= J ? J : I, Kp = K >= L ? K : L, Lp = K >= L ? L : K;
     const PairRef bref = a.pairs[Ip * (Ip + 1) / 2 + Jp], kref = a.pairs[Kp * (Kp + 1) / 2 + Lp];
-    PairGeom<double> bg, kg;
+    PairGeom<R> bg, kg;
     for (int x = 0; x < 3; ++x) {
-        bg.A[x] = a.shell_xyz[3 * I + x];
-        bg.AB[x] = a.shell_xyz[3 * I + x] - a.shell_xyz[3 * J + x];
-        kg.A[x] = a.shell_xyz[3 * K + x];
-        kg.AB[x] = a.shell_xyz[3 * K + x] - a.shell_xyz[3 * L + x];
+        bg.A[x] = R(a.shell_xyz[3 * I + x]);
+        bg.AB[x] = R(a.shell_xyz[3 * I + x] - a.shell_xyz[3 * J + x]);
+        kg.A[x] = R(a.shell_xyz[3 * K + x]);
+        kg.AB[x] = R(a.shell_xyz[3 * K + x] - a.shell_xyz[3 * L + x]);
     }
     for (int ib = 0; ib < bref.npp; ++ib) {
         PrimPair bp = a.pp[bref.pp0 + ib];
         for (int ik = 0; ik < kref.npp; ++ik) {
             PrimPair kp = a.pp[kref.pp0 + ik];
             if constexpr (!HeavyClass<LA, LB, LC, LD>::value) {
-                primitive_quartet<double, LA, LB, LC, LD>(bp, kp, bg, kg, a.rys, 1.0,
-                                                          [&](auto ia, auto ib_, auto ic, auto id, double v) {
-                                                              constexpr int idx =
-                                                                  ((decltype(ia)::value * ncart(LB) + decltype(ib_)::value) * ncart(LC) +
-                                                                   decltype(ic)::value) * ncart(LD) + decltype(id)::value;
-                                                              out[idx] += v;
-                                                          });
+                primitive_quartet<R, LA, LB, LC, LD>(bp, kp, bg, kg, a.rys, R(1),
+                                                     [&](auto ia, auto ib_, auto ic, auto id, R v) {
+                                                         constexpr int idx =
+                                                             ((decltype(ia)::value * ncart(LB) + decltype(ib_)::value) * ncart(LC) +
+                                                              decltype(ic)::value) * ncart(LD) + decltype(id)::value;
+                                                         out[idx] += double(v);
+                                                     });
             } else {
-                OneDTable<double, LA, LB, LC, LD> tbl;
-                tbl.build(bp, kp, bg, kg, a.rys, 1.0);
+                OneDTable<R, LA, LB, LC, LD> tbl;
+                tbl.build(bp, kp, bg, kg, a.rys, R(1));
                 for (int ia = 0; ia < ncart(LA); ++ia)
                     for (int ib2 = 0; ib2 < ncart(LB); ++ib2) {
                         const double nab = cart_norm_rt<LA>(ia) * cart_norm_rt<LB>(ib2);
@@ -379,7 +381,7 @@ __global__ __launch_bounds__(64) void eri_out_kernel(EriOutArgs a)
                                 constexpr int IC = decltype(ic)::value, ID = decltype(id)::value;
                                 constexpr double ncd = 1.0 / csqrt(cart_dfprod(LC, IC) * cart_dfprod(LD, ID));
                                 out[((ia * ncart(LB) + ib2) * ncart(LC) + IC) * ncart(LD) + ID] +=
-                                    tbl.template value<IC, ID>(ia, ib2) * nab * ncd;
+                                    double(tbl.template value<IC, ID>(ia, ib2)) * nab * ncd;
                             });
                         });
                     }
@@ -476,7 +478,10 @@ template <int LA, int LB, int LC, int LD>
 inline void launch_out_t(const EriOutArgs &args, hipStream_t s)
 {
     if (args.nq <= 0) return;
-    hipLaunchKernelGGL((eri_out_kernel<LA, LB, LC, LD>), dim3((args.nq + 63) / 64), dim3(64), 0, s, args);
+    if (args.prec == 0)
+        hipLaunchKernelGGL((eri_out_kernel<LA, LB, LC, LD, double>), dim3((args.nq + 63) / 64), dim3(64), 0, s, args);
+    else
+        hipLaunchKernelGGL((eri_out_kernel<LA, LB, LC, LD, float>), dim3((args.nq + 63) / 64), dim3(64), 0, s, args);
 }
 
 template <int LA, int LB>

@@ -1783,6 +1783,11 @@ int patmol_hip_eri_gradient(const ERIFS *e, const double *Pmat, double *grad)
 
 int patmol_hip_eri_blocks(const ERIFS *e, int nq, const int *quartets, double *out)
 {
+    return patmol_hip_eri_blocks_prec(e, nq, quartets, out, 0);
+}
+
+int patmol_hip_eri_blocks_prec(const ERIFS *e, int nq, const int *quartets, double *out, int prec)
+{
     g_err = PATMOL_HIP_OK;
     Ctx *cp = ctx_of(e);
     if (!cp || !cp->built) return fail(PATMOL_HIP_ESTATE, "no tables");
@@ -1839,7 +1844,7 @@ int patmol_hip_eri_blocks(const ERIFS *e, int nq, const int *quartets, double *o
             DevBuf<int> dq;
             DevBuf<double> dout;
             if (dq.upload(qv) || dout.alloc(ncomp * idx.size())) return g_err;
-            EriOutArgs a{dq.p, (int)idx.size(), c.d_xyz.p, c.d_pairs.p, c.d_pp.p, dout.p, g_rys};
+            EriOutArgs a{dq.p, (int)idx.size(), c.d_xyz.p, c.d_pairs.p, c.d_pp.p, dout.p, g_rys, prec ? 1 : 0};
             if (launch_eri_out(b, k, a, 0)) return fail(PATMOL_HIP_EUNSUPPORTED, "no ERI kernel");
             HIPCHK(hipGetLastError());
             std::vector<double> h(ncomp * idx.size());

@@ -8,6 +8,16 @@
         per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of every kernel, scaled by
         the factor that makes a plain device copy of B bytes read B and write
         B bytes (markdown table on stdout, per-kernel bytes in OUT.json)
+    python -m patmol_amd.profiling counters DIR [DIR ...] --json OUT.json [--command C]
+        per-dispatch means of every counter of --pmc runs (one or more
+        directories, e.g. one per pass), per kernel (markdown, OUT.json)
+    python -m patmol_amd.profiling valu --fp64 DIR [DIR ...] --fp32 DIR [DIR ...]
+                                   --bench BENCH.json [--command C]
+        per J/K class: VALU instructions and executed FP64 / FP32 FLOPs of
+        the FP64 and the FP32 kernel (SQ_INSTS_VALU, SQ_INSTS_VALU_FLOPS_*,
+        64 lanes per wave instruction), their durations in the counter runs,
+        executed TFLOP/s, and the algorithmic FLOPs of the class from a
+        bench.py JSON line (headline per_class_fp64)
 
 Only reads the CSV files rocprofv3 writes with --output-format csv.
 """
@@ -92,6 +102,119 @@ def traffic(args):
         json.dump(res, fh, indent=1)
 
 
+def counters(args):
+    vals = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(lambda: collections.defaultdict(set))
+    names = []
+    for d in args.dirs:
+        for r in csv.DictReader(open(_one(d, "counter_collection.csv"))):
+            k, c = r["Kernel_Name"], r["Counter_Name"]
+            if c not in names:
+                names.append(c)
+            vals[k][c] += float(r["Counter_Value"])
+            disp[k][c].add(r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+    res = {}
+    out = []
+    if args.command:
+        out.append(f"Command: `{args.command}`\n")
+    out.append("| kernel | " + " | ".join(names) + " |")
+    out.append("|---|" + "---|" * len(names))
+    order = sorted(vals, key=lambda k: -max(vals[k].values()))
+    for k in order:
+        row = {c: vals[k][c] / max(1, len(disp[k][c])) for c in names if c in vals[k]}
+        res[k] = row
+        out.append(f"| `{_short(k)}` | " + " | ".join(f"{row[c]:.4g}" if c in row else "-" for c in names) + " |")
+    print("\n".join(out))
+    with open(args.json, "w") as fh:
+        json.dump(res, fh, indent=1)
+
+
+_L = "spd"
+
+
+def _class_of(kernel):
+    """'(ab|cd)' and precision of a J/K kernel symbol, or None."""
+    import re
+
+    m = re.search(r"jk_kernel3?<(\d), (\d), (\d), (\d), (double|float)>", kernel)
+    if not m:
+        return None
+    la, lb, lc, ld = (int(m.group(i)) for i in range(1, 5))
+    return f"({_L[la]}{_L[lb]}|{_L[lc]}{_L[ld]})", m.group(5)
+
+
+def _collect(dirs):
+    vals = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(lambda: collections.defaultdict(set))
+    dur = collections.defaultdict(list)
+    for d in dirs:
+        for r in csv.DictReader(open(_one(d, "counter_collection.csv"))):
+            vals[r["Kernel_Name"]][r["Counter_Name"]] += float(r["Counter_Value"])
+            disp[r["Kernel_Name"]][r["Counter_Name"]].add(r.get("Dispatch_Id", ""))
+        for r in csv.DictReader(open(_one(d, "kernel_trace.csv"))):
+            dur[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+    out = {}
+    for k in vals:
+        row = {c: vals[k][c] / max(1, len(disp[k][c])) for c in vals[k]}
+        row["seconds"] = sum(dur[k]) / max(1, len(dur[k]))
+        out[k] = row
+    return out
+
+
+def valu(args):
+    f64, f32 = _collect(args.fp64), _collect(args.fp32)
+    bench = json.load(open(args.bench))
+    per = bench["headline"]["per_class_fp64"]
+    rows = {}
+    for data, prec in ((f64, "double"), (f32, "float")):
+        for k, row in data.items():
+            c = _class_of(k)
+            if c and c[1] == prec:
+                rows.setdefault(c[0], {})[prec] = row
+    out = []
+    if args.command:
+        out.append(f"Command: `{args.command}`\n")
+    out.append("Executed FLOPs = 64 x SQ_INSTS_VALU_FLOPS_FP64 (FP32) (2 per FMA, 1 per add / multiply / "
+               "transcendental, per wave instruction); durations from the counter runs' kernel trace; "
+               "algorithmic FLOPs of the class from the bench line.\n")
+    out.append("| class | alg. GFLOP | FP64 kernel: VALU instr (M) | exec. FP64 GFLOP | ms | exec. TFLOP/s | exec./alg. "
+               "| FP32 kernel: VALU instr (M) | exec. FP32 GFLOP | exec. FP64 GFLOP | ms | FP64/FP32 time |")
+    out.append("|---|---|---|---|---|---|---|---|---|---|---|---|")
+    res = {}
+    t64 = t32 = 0.0
+    for cls in sorted(rows, key=lambda c: -rows[c].get("double", {}).get("seconds", 0)):
+        d, f = rows[cls].get("double"), rows[cls].get("float")
+        alg = per.get(cls, {}).get("tflops", 0) * per.get(cls, {}).get("ms", 0) * 1e9 / 1e9   # GFLOP
+        line = f"| {cls} | {alg:.1f} |"
+        r = {"alg_gflop": alg}
+        if d:
+            ex = 64 * d.get("SQ_INSTS_VALU_FLOPS_FP64", 0) / 1e9
+            t64 += d["seconds"]
+            r.update(fp64_valu=d.get("SQ_INSTS_VALU", 0), fp64_exec_gflop=ex, fp64_s=d["seconds"])
+            line += (f" {d.get('SQ_INSTS_VALU', 0) / 1e6:.0f} | {ex:.1f} | {d['seconds'] * 1e3:.3f} | "
+                     f"{ex / max(d['seconds'], 1e-12) / 1e3:.2f} | {ex / max(alg, 1e-12):.2f} |")
+        else:
+            line += " - | - | - | - | - |"
+        if f:
+            ex32 = 64 * f.get("SQ_INSTS_VALU_FLOPS_FP32", 0) / 1e9
+            ex64 = 64 * f.get("SQ_INSTS_VALU_FLOPS_FP64", 0) / 1e9
+            t32 += f["seconds"]
+            r.update(fp32_valu=f.get("SQ_INSTS_VALU", 0), fp32_exec_gflop=ex32, fp32_s=f["seconds"])
+            ratio = d["seconds"] / f["seconds"] if d and f["seconds"] > 0 else 0
+            line += (f" {f.get('SQ_INSTS_VALU', 0) / 1e6:.0f} | {ex32:.1f} | {ex64:.1f} | {f['seconds'] * 1e3:.3f} |"
+                     f" {ratio:.2f} |")
+        else:
+            line += " - | - | - | - | - |"
+        res[cls] = r
+        out.append(line)
+    out.append(f"\nSum of the class kernels: FP64 {t64 * 1e3:.2f} ms, FP32 (where a FP32 kernel ran) "
+               f"{t32 * 1e3:.2f} ms.")
+    print("\n".join(out))
+    if args.json:
+        with open(args.json, "w") as fh:
+            json.dump(res, fh, indent=1)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m patmol_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -105,8 +228,18 @@ def main(argv=None):
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("--json", required=True)
     t.add_argument("--command", default="")
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--json", required=True)
+    c.add_argument("--command", default="")
+    v = sub.add_parser("valu")
+    v.add_argument("--fp64", nargs="+", required=True)
+    v.add_argument("--fp32", nargs="+", required=True)
+    v.add_argument("--bench", required=True)
+    v.add_argument("--json", default="")
+    v.add_argument("--command", default="")
     args = ap.parse_args(argv)
-    stats(args) if args.cmd == "stats" else traffic(args)
+    {"stats": stats, "traffic": traffic, "counters": counters, "valu": valu}[args.cmd](args)
     return 0
 
 

@@ -107,6 +107,29 @@ def test_dpm_grid_loop_vs_oracle():
         assert abs(g["iterations"] - r["iterations"]) <= 3
 
 
+@pytest.mark.parametrize("uhf", [False, True])
+def test_dpm_grid_loop_mixed_vs_oracle(uhf):
+    """The grid loop in mixed precision (items with Schwarz bound x max|P|
+    < 1e-6 in FP32, RHF and UHF SCF): energies and vpol within 1e-6 Eh of
+    the oracle's FP64 loop."""
+    s = systems.water(d_shells=True)
+    pm.load_system(s)
+    pm.set_options(tau=0.0, precision=1, mixed_threshold=1e-6, strict=False, scf_density=1)
+    pm.add_dpm_center()
+    try:
+        got, gs = pm.dpm_calculation(POINTS, uhf=uhf)
+    finally:
+        pm.set_options(precision=0)
+    ogs, ref = oracle.rhf_dpm_calculation(s, POINTS)
+    note(f"mixed ({'UHF' if uhf else 'RHF'}): ground state |dE| {abs(gs['energy'] - ogs.energy):.2e}")
+    assert abs(gs["energy"] - ogs.energy) <= 1e-6
+    for p, g, r in zip(POINTS, got, ref):
+        note(f"  r={p}: vpol mixed {g['vpol']:.10f} oracle {r['vpol']:.10f} (|d| {abs(g['vpol'] - r['vpol']):.2e}); "
+             f"E |d| {abs(g['e_conv'] - r['e_conv']):.2e}")
+        assert g["converged"] == 1
+        assert abs(g["e_conv"] - r["e_conv"]) <= 1e-6 and abs(g["vpol"] - r["vpol"]) <= 1e-6
+
+
 def test_dpm_closed_shell_uhf_equals_rhf():
     s = systems.water(d_shells=True)
     pm.set_options(tau=1e-12, precision=0, strict=False)

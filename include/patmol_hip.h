@@ -244,6 +244,13 @@ int patmol_hip_batch_rhf_scf(PatmolHipBatch *b, const double *S, const double *H
  * the basis (grad has 3 * number_of_centers() doubles, accumulated).  FP64,
  * reproducible bit for bit. */
 int patmol_hip_eri_gradient(const ERIFS *tfiles, const double *Pmat, double *grad);
+/* The DPM potential's part of the gradient (no counterpart in the
+ * reference): for the packed half density P of the current basis,
+ * grad[3 c + x] += d/dR_{c,x} of -2 sum_ab P_ab V_ab, V_ab the dpm_ints
+ * integrals (dpmints.c:120-199) of the positron at skip_center(), for every
+ * centre, the positron centre included (-(d/dA + d/dB) per pair).  One GPU
+ * thread per shell pair; per-pair sums added on the host in pair order. */
+int patmol_hip_dpm_gradient(const double *Pmat, double *grad);
 
 /* ---- introspection ---- */
 typedef struct {

@@ -171,11 +171,15 @@ def lib():
         H.patmol_rhf_finite_field.argtypes = [C.c_double, _dp, C.POINTER(ScfResult)]
         H.patmol_rhf_gradient.restype = C.c_int
         H.patmol_rhf_gradient.argtypes = [C.c_void_p, _dp, _dp]
+        H.patmol_rhf_dpm_gradient.restype = C.c_int
+        H.patmol_rhf_dpm_gradient.argtypes = [C.c_void_p, _dp, _dp]
         H.patmol_host_onee_gradient.restype = C.c_int
         H.patmol_host_onee_gradient.argtypes = [_dp, _dp, _dp]
         H.patmol_nuclear_repulsion_gradient.argtypes = [_dp]
         G.patmol_hip_eri_gradient.restype = C.c_int
         G.patmol_hip_eri_gradient.argtypes = [C.c_void_p, _dp, _dp]
+        G.patmol_hip_dpm_gradient.restype = C.c_int
+        G.patmol_hip_dpm_gradient.argtypes = [_dp, _dp]
         G.patmol_hip_batch_new.restype = C.c_void_p
         G.patmol_hip_batch_free.argtypes = [C.c_void_p]
         G.patmol_hip_batch_add.restype = C.c_int
@@ -219,6 +223,7 @@ def exported_symbols():
             "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "patmol_hip_set_scf_incremental", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
             "patmol_hip_write_erints", "patmol_hip_rhf_gmatrix_erints", "patmol_hip_rhf_gmatrix_erint_file",
             "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file", "patmol_hip_eri_gradient",
+            "patmol_hip_dpm_gradient",
             "patmol_hip_batch_new", "patmol_hip_batch_add", "patmol_hip_batch_finalize", "patmol_hip_batch_gmatrix",
             "patmol_hip_batch_buffers", "patmol_hip_batch_launch", "patmol_hip_batch_info", "patmol_hip_batch_free",
             "patmol_hip_batch_rhf_scf"]
@@ -455,13 +460,18 @@ def _scf_inputs():
 
 
 def rhf_scf_device(store: Optional["EriStore"] = None, guess: Optional[np.ndarray] = None,
-                   want_c: bool = False):
+                   want_c: bool = False, dpm: bool = False):
     """The whole RHF iteration on the GPU (patmol_hip_rhf_scf) for the
-    current basis; returns (result dict, P, orbital energies[, C])."""
+    current basis; returns (result dict, P, orbital energies[, C]).  With
+    dpm, H carries the DPM potential of the positron at skip_center()
+    (dpm_ints), as one grid point of the DPM loop has it."""
     _, G = lib()
     a = _scf_inputs()
     n = a["n"]
     S, T, Hm = one_electron_ints()
+    if dpm:
+        G.dpm_ints(Hm)
+        _check("dpm_ints")
     own = store is None
     st = EriStore() if own else store
     P = np.zeros(n * (n + 1) // 2)
@@ -661,6 +671,33 @@ def rhf_gradient(P: np.ndarray, store: Optional["EriStore"] = None) -> np.ndarra
     if rc:
         _check("patmol_rhf_gradient")
         raise PatmolError(f"patmol_rhf_gradient failed ({rc})")
+    return g.reshape(nc, 3)
+
+
+def rhf_dpm_gradient(P: np.ndarray, store: Optional["EriStore"] = None) -> np.ndarray:
+    """dE/dR (ncen x 3) of the DPM-augmented RHF energy of one grid point
+    (basis after dpm_augment, positron at skip_center; its row is dE/dQ) at
+    the converged half density P: rhf_gradient's terms with H - V and the
+    DPM potential's derivative (patmol_hip_dpm_gradient).  store must cover
+    the whole augmented basis, or None."""
+    H, _ = lib()
+    nc = H.number_of_centers()
+    g = np.zeros(3 * nc)
+    rc = H.patmol_rhf_dpm_gradient(None if store is None else store.h, np.ascontiguousarray(P, dtype=np.float64), g)
+    if rc:
+        _check("patmol_rhf_dpm_gradient")
+        raise PatmolError(f"patmol_rhf_dpm_gradient failed ({rc})")
+    return g.reshape(nc, 3)
+
+
+def dpm_gradient(P: np.ndarray) -> np.ndarray:
+    """The DPM potential's part alone (patmol_hip_dpm_gradient): d/dR of
+    -2 sum_ab P_ab V_ab, ncen x 3."""
+    H, G = lib()
+    nc = H.number_of_centers()
+    g = np.zeros(3 * nc)
+    if G.patmol_hip_dpm_gradient(np.ascontiguousarray(P, dtype=np.float64), g):
+        _check("patmol_hip_dpm_gradient")
     return g.reshape(nc, 3)
 
 

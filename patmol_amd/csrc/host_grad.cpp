@@ -248,13 +248,15 @@ extern "C" void patmol_nuclear_repulsion_gradient(double *grad)
 // current basis: grad[3 c + x] = dE/dR_{c,x} (assigned), every centre.
 // mints: an ERIFS handle of the current basis (two_electron_ints run), or
 // null to build one here.  W = P F P with F = H + G(P) from the device.
-extern "C" int patmol_rhf_gradient(const ERIFS *mints, const double *P, double *grad)
+// With dpm, H carries -V of dpm_ints and the DPM term's derivative is added.
+static int rhf_gradient(const ERIFS *mints, const double *P, double *grad, bool dpm)
 {
     const int n = number_of_orbitals(), nc = number_of_centers();
     if (!P || !grad || n <= 0) return 1;
     const size_t no2 = (size_t)n * (n + 1) / 2;
     std::vector<double> S(no2), T(no2), H(no2), G(no2, 0.0);
     one_electron_ints(S.data(), T.data(), H.data());
+    if (!patmol_hip_last_error() && dpm) dpm_ints(H.data());
     if (patmol_hip_last_error()) return 2;
     ERIFS *own = nullptr;
     if (!mints) {
@@ -294,6 +296,17 @@ extern "C" int patmol_rhf_gradient(const ERIFS *mints, const double *P, double *
     if (!rc && patmol_hip_eri_gradient(mints, P, grad)) rc = 2;
     if (!rc && patmol_host_onee_gradient(P, W.data(), grad)) rc = 3;
     if (!rc) patmol_nuclear_repulsion_gradient(grad);
+    if (!rc && dpm && patmol_hip_dpm_gradient(P, grad)) rc = 2;
     if (own) clean_ERIFS(own);
     return rc;
+}
+
+extern "C" int patmol_rhf_gradient(const ERIFS *mints, const double *P, double *grad)
+{
+    return rhf_gradient(mints, P, grad, false);
+}
+
+extern "C" int patmol_rhf_dpm_gradient(const ERIFS *mints, const double *P, double *grad)
+{
+    return rhf_gradient(mints, P, grad, true);
 }

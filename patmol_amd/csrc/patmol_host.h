@@ -73,6 +73,12 @@ int patmol_rhf_finite_field(double efx, double *alfa, PatmolScfResult *gs);
  * mints is null), one-electron and nuclear-repulsion parts on the host with
  * W = P F P.  Returns 0 or an error code. */
 int patmol_rhf_gradient(const ERIFS *mints, const double *P, double *grad);
+/* The same for the DPM-augmented energy of one grid point (the basis after
+ * dpm_augment(), the positron at skip_center()): H carries -V of dpm_ints,
+ * so F and W do too, and the DPM term's derivative (patmol_hip_dpm_gradient)
+ * is added; the positron centre's entry is dE/dQ.  mints must cover the
+ * whole augmented basis (start 0), or null. */
+int patmol_rhf_dpm_gradient(const ERIFS *mints, const double *P, double *grad);
 /* its host parts: grad += 2 sum P dH/dR - 2 sum W dS/dR (P, W packed), and
  * grad += dE_nuc/dR */
 int patmol_host_onee_gradient(const double *P, const double *W, double *grad);

@@ -74,6 +74,14 @@ void patmol_hip_set_precision(int mode, double threshold);
 /* Autotuning aid (no counterpart in the reference): 1 sends the items below
  * the threshold to FP32 in every class, 0 (default) follows jk3_tuned.inc. */
 void patmol_hip_set_fp32_policy(int every_class);
+/* No counterpart in the reference (the reference has one integral algorithm,
+ * Rys quadrature).  Which classes run their FP64 work items on the
+ * McMurchie-Davidson kernel (Hermite expansion, the two contractions on the
+ * FP64 matrix cores) instead of their Rys kernels, a mask: bit 0 (dd|dd),
+ * bit 1 (dd|dp), bit 2 (dp|dp), bit 3 (dd|pp); -1 restores the default,
+ * (dd|dd) and (dd|dp).  For comparisons and timing: both engines give the
+ * reference's integrals. */
+void patmol_hip_set_jk_engine(int mask);
 
 /* ---- device-resident G build (inputs already in HBM) ---- */
 /* d_P: packed half-density (no2 doubles, device); d_G: packed G (device),

@@ -118,6 +118,7 @@ def lib():
         G.patmol_hip_set_schwarz_threshold.argtypes = [C.c_double]
         G.patmol_hip_set_precision.argtypes = [C.c_int, C.c_double]
         G.patmol_hip_set_fp32_policy.argtypes = [C.c_int]
+        G.patmol_hip_set_jk_engine.argtypes = [C.c_int]
         G.patmol_hip_rhf_gmatrix_device.restype = C.c_int
         G.patmol_hip_rhf_gmatrix_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         G.patmol_hip_rhf_gmatrix_device_noalloc.restype = C.c_int
@@ -216,6 +217,7 @@ def exported_symbols():
     return ["init_ERIFS", "clean_ERIFS", "two_electron_ints", "rhf_form_gmatrix", "uhf_form_gmatrix",
             "dpm_ints", "one_electron_ints", "patmol_hip_last_error", "patmol_hip_error_string", "patmol_hip_set_strict",
             "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision", "patmol_hip_set_fp32_policy",
+            "patmol_hip_set_jk_engine",
             "patmol_hip_rhf_gmatrix_device", "patmol_hip_rhf_gmatrix_device_noalloc", "patmol_hip_stats", "patmol_hip_class_stats",
             "patmol_hip_pair_schwarz",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_eri_blocks_prec", "patmol_hip_last_timing",
@@ -241,8 +243,14 @@ def load_system(sys_: "systems.System") -> None:
 
 def set_options(tau: Optional[float] = None, precision: Optional[int] = None,
                 mixed_threshold: float = 1e-6, strict: Optional[bool] = None, timing: Optional[bool] = None,
-                scf_density: Optional[int] = None, scf_incremental: Optional[int] = None):
+                scf_density: Optional[int] = None, scf_incremental: Optional[int] = None,
+                md_engine: Optional[int] = None):
+    """md_engine: classes whose FP64 items run on the McMurchie-Davidson
+    kernel, a mask (1 (dd|dd), 2 (dd|dp), 4 (dp|dp), 8 (dd|pp); 0 none, -1 the
+    library's default (dd|dd) and (dd|dp))."""
     _, G = lib()
+    if md_engine is not None:
+        G.patmol_hip_set_jk_engine(int(md_engine))
     if scf_density is not None:
         G.patmol_hip_set_scf_density(int(scf_density))
     if scf_incremental is not None:
@@ -351,17 +359,19 @@ class EriStore:
         return q, f
 
     @staticmethod
-    def class_kernel_name(bra_pc: int, ket_pc: int, fp32: bool = False) -> str:
-        """Symbol prefix (as profilers print it) of the FP64 J/K kernel that
-        runs class (bra_pc | ket_pc); pair class pc = l(l+1)/2 + l'.  Mirrors
-        the dispatch rule of launch_jk_t in eri_kernels_impl.h."""
+    def class_kernel_name(bra_pc: int, ket_pc: int, fp32: bool = False, md_mask: int = 3) -> str:
+        """Symbol prefix (as profilers print it) of the J/K kernel that runs
+        class (bra_pc | ket_pc); pair class pc = l(l+1)/2 + l'.  Mirrors the
+        dispatch rule of launch_jk_t in eri_kernels_impl.h (md_mask: the
+        classes on the McMurchie-Davidson kernel, the library's default)."""
         ls = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
         (la, lb), (lc, ld) = ls[bra_pc], ls[ket_pc]
-        nc = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
-        heavy = (la, lb, lc, ld) == (2, 2, 2, 2)
         r = "float" if fp32 else "double"
-        if heavy:
-            return f"void pm::jk_kernel<{la}, {lb}, {lc}, {ld}, {r}, 1>"
+        bit = {(2, 2, 2, 2): 1, (2, 2, 2, 1): 2, (2, 1, 2, 1): 4, (2, 2, 1, 1): 8}.get((la, lb, lc, ld), 0)
+        if not fp32 and (bit & md_mask):
+            return f"void pm::jk_md_kernel<{la}, {lb}, {lc}, {ld}>"
+        if (la, lb, lc, ld) == (2, 2, 2, 2):
+            return f"void pm::jk_kernel<{la}, {lb}, {lc}, {ld}, {r}>"
         return f"void pm::jk_kernel3<{la}, {lb}, {lc}, {ld}, {r}>"
 
     def pair_schwarz(self) -> np.ndarray:

@@ -101,6 +101,12 @@ struct JKArgs {
     const int *item_mol;       // molecule of each item
     const MolDesc *mols;
     long long slot_stride;     // elements of one per-XCD copy of J' (and of K')
+    // McMurchie-Davidson tables (md_kernel.h): slot of each primitive pair
+    // and the bra / ket Hermite-coefficient tables per pair class; with
+    // md_slot null the classes that could use them run their Rys kernels
+    const int *md_slot = nullptr;
+    const double *md_bra[PM_NPC] = {};
+    const double *md_ket[PM_NPC] = {};
 };
 
 struct EriOutArgs {
@@ -125,10 +131,84 @@ struct SchwarzArgs {
     RysTables rys;
 };
 
+// ---------------------------------------------------------------------------
+// McMurchie-Davidson path (md_kernel.h): Hermite lists and the layouts of
+// the per-pair-class Hermite-coefficient tables
+// ---------------------------------------------------------------------------
+// Hermite functions (t,u,v) with t + u + v <= L, ordered by n = t + u + v,
+// then t descending, then u descending (a list for L is a prefix of the list
+// for L + 1).
+__host__ __device__ constexpr int herm_count(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+
+__host__ __device__ inline void herm_decode(int h, int &t, int &u, int &v)
+{
+    int n = 0, base = 0;
+    while (base + (n + 1) * (n + 2) / 2 <= h) {
+        base += (n + 1) * (n + 2) / 2;
+        ++n;
+    }
+    int k = h - base;
+    t = n;
+    while (k >= n - t + 1) {
+        k -= n - t + 1;
+        --t;
+    }
+    u = n - t - k;
+    v = k;
+}
+
+// Table layouts of one pair class (la >= lb), in the lane order the kernel
+// loads them (64 doubles per register).  Component pairs are numbered with
+// the second index padded to a power of two, slot = a pb + b (pb =
+// md_pad(ncart(lb))), so that a lane's slots split into register and lane
+// bits; padded slots hold zeros.
+//   bra table: E_ab for the second product's A operand, [step][slot block][lane]
+//     with slot = 16 block + lane % 16 and step (bm, q) ->
+//     h = 16 bm + 4 q + lane / 16, over the steps with 16 bm + 4 q < nh;
+//   ket table: E~_cd for the first product's B operand, [kstep][slot block][lane]
+//     with h' = 4 kstep + lane / 16, slot = 16 block + lane % 16.
+__host__ __device__ constexpr int md_pad(int n) { return n <= 1 ? 1 : (n <= 2 ? 2 : (n <= 4 ? 4 : 8)); }
+__host__ __device__ constexpr int md_nab(int la, int lb) { return ncart(la) * md_pad(ncart(lb)); }
+__host__ __device__ constexpr int md_blocks(int n) { return (n + 15) / 16; }
+__host__ __device__ constexpr int md_bra_steps(int nh)
+{
+    int s = 0;
+    for (int bm = 0; 16 * bm < nh; ++bm)
+        for (int q = 0; q < 4; ++q)
+            if (16 * bm + 4 * q < nh) ++s;
+    return s;
+}
+__host__ __device__ constexpr int md_bra_regs(int la, int lb)
+{
+    return md_bra_steps(herm_count(la + lb)) * md_blocks(md_nab(la, lb));
+}
+__host__ __device__ constexpr int md_ket_regs(int la, int lb)
+{
+    return ((herm_count(la + lb) + 3) / 4) * md_blocks(md_nab(la, lb));
+}
+
+// pair classes whose tables the kernel reads: bra dd, dp; ket dd, dp, pp
+__host__ __device__ constexpr bool md_bra_class(int la, int lb) { return la == 2 && lb >= 1; }
+__host__ __device__ constexpr bool md_ket_class(int la, int lb) { return (la == 2 && lb >= 1) || (la == 1 && lb == 1); }
+
+struct MdTabArgs {
+    const int *pair_list;      // pair index I(I+1)/2 + J of each shell pair
+    const int *pair_I, *pair_J;
+    const int *slot0;          // first slot of each shell pair
+    int npairs;
+    const double *shell_xyz;
+    const PairRef *pairs;
+    const PrimPair *pp;
+    double *bra;               // bra table of the class or null
+    double *ket;               // ket table of the class or null
+};
+
 // prec 0: FP64 instantiation, 1: FP32 (FP64 accumulation).  Returns nonzero
 // for a class without a kernel.
 int launch_jk(int bra_pc, int ket_pc, const JKArgs &args, int prec, hipStream_t s);
 int launch_eri_out(int bra_pc, int ket_pc, const EriOutArgs &args, hipStream_t s);
 int launch_schwarz(int pc, const SchwarzArgs &args, hipStream_t s);
+// Hermite-coefficient tables of pair class (la, lb) (md_tables.hip)
+int launch_md_etab(int la, int lb, const MdTabArgs &args, hipStream_t s);
 
 }  // namespace pm

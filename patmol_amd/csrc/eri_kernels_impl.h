@@ -443,6 +443,7 @@ __global__ __launch_bounds__(64) void schwarz_kernel(SchwarzArgs a)
 }  // namespace pm
 
 #include "eri_kernel3.h"
+#include "md_kernel.h"
 
 namespace pm {
 
@@ -453,6 +454,14 @@ inline void launch_jk_t(const JKArgs &args, int prec, hipStream_t s)
 {
     if (args.nitems <= 0) return;
     const dim3 block(256);
+    if constexpr (MdClass<LA, LB, LC, LD>::value) {
+        // FP64 items of the classes with the most components: the
+        // McMurchie-Davidson kernel on the matrix cores, when the tables exist
+        if (prec == 0 && args.md_slot) {
+            hipLaunchKernelGGL((jk_md_kernel<LA, LB, LC, LD>), dim3((unsigned)args.nitems), block, 0, s, args);
+            return;
+        }
+    }
     if constexpr (HeavyClass<LA, LB, LC, LD>::value && LC == 2 && LD == 2) {
         // (dd|dd): the ket-pair-per-lane kernel with run-time component loops
         const dim3 grid(args.nitems);

@@ -85,6 +85,23 @@ static double g_prec_thr = 1e-6;
 // 0: items below the threshold run in FP32 only in the classes whose FP32
 // kernel was measured faster (jk3_fp32_pays); 1: in every class (tuning)
 static int g_fp32_policy = 0;
+// classes whose FP64 items run on the McMurchie-Davidson kernel
+// (md_kernel.h), a mask: bit 0 (dd|dd), bit 1 (dd|dp), bit 2 (dp|dp),
+// bit 3 (dd|pp); the others run their Rys kernels.  The default holds the
+// classes where it measured faster than the Rys kernel on (H2O)16.
+static const int MD_DEFAULT_MASK = 1 | 2;
+static int g_md_mask = MD_DEFAULT_MASK;
+static int md_class_bit(int bra_pc, int ket_pc)
+{
+    if (bra_pc == 5 && ket_pc == 5) return 1;
+    if (bra_pc == 5 && ket_pc == 4) return 2;
+    if (bra_pc == 4 && ket_pc == 4) return 4;
+    if (bra_pc == 5 && ket_pc == 2) return 8;
+    return 0;
+}
+// largest Hermite-coefficient table set built per basis; above it the
+// classes stay on their Rys kernels
+static const size_t MD_TABLE_MAX_BYTES = (size_t)8 << 30;
 
 // precision of the items a class would run in FP32: 1 where FP32 pays
 static int fp32_prec(int bra_pc, int ket_pc)
@@ -194,6 +211,33 @@ struct ClassRange {
     int first, count;
 };
 
+// Hermite-coefficient tables of the McMurchie-Davidson J/K path: slot of
+// each primitive pair in its class's tables, bra / ket tables per pair class
+struct MdTables {
+    DevBuf<int> d_slot;
+    DevBuf<double> d_bra[PM_NPC], d_ket[PM_NPC];
+    bool ready = false;
+    void release()
+    {
+        d_slot.release();
+        for (int pc = 0; pc < PM_NPC; ++pc) {
+            d_bra[pc].release();
+            d_ket[pc].release();
+        }
+        ready = false;
+    }
+    // the J/K arguments of class (bra_pc | ket_pc) under mask `mask`
+    void fill(JKArgs &a, int mask_bit) const
+    {
+        if (!ready || !mask_bit) return;
+        a.md_slot = d_slot.p;
+        for (int pc = 0; pc < PM_NPC; ++pc) {
+            a.md_bra[pc] = d_bra[pc].p;
+            a.md_ket[pc] = d_ket[pc].p;
+        }
+    }
+};
+
 struct Ctx {
     bool built = false;
     int N = 0, nshell = 0, start = 0;
@@ -219,6 +263,9 @@ struct Ctx {
     DevBuf<PrimPair> d_pp;
     DevBuf<double2> d_ppab;    // primitive exponents of each primitive pair (gradient)
     DevBuf<int4> d_items;
+    // McMurchie-Davidson tables: slot of each primitive pair in its class's
+    // tables, bra / ket Hermite-coefficient tables per pair class
+    MdTables md;
     // work buffers
     DevBuf<double> d_P, d_PB, d_D, d_D2, d_DJ, d_G, d_G2, d_dmax;
     DevBuf<FxAcc> d_J, d_K;   // PM_XCD_SLOTS fixed-point copies of J', K' (N x N each)
@@ -257,10 +304,78 @@ static Ctx *ctx_of(const ERIFS *e)
 
 static inline int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
 
+// Hermite-coefficient tables of the McMurchie-Davidson J/K path: for every
+// primitive pair of the pair classes pp, dp and dd, its slot, and the bra
+// (dd, dp) and ket (dd, dp, pp) tables in the lane order md_kernel.h reads.
+// plist: (pair index, sorted shell I, sorted shell J) of every shell pair,
+// shell_l: l of each sorted shell; the device arrays hold the same pairs.
+struct MdPair {
+    int64_t p;
+    int I, J;
+};
+static int build_md_tables(MdTables &t, const std::vector<MdPair> &plist, const std::vector<int> &shell_l,
+                           const std::vector<PairRef> &pairs, size_t npp, const double *d_xyz, const PairRef *d_pairs,
+                           const PrimPair *d_pp)
+{
+    t.release();
+    std::vector<int> slot(npp, -1);
+    size_t bytes = 0;
+    for (int pc : {2, 4, 5}) {
+        const int la = pair_class_la(pc), lb = pair_class_lb(pc);
+        const bool bra = md_bra_class(la, lb), ket = md_ket_class(la, lb);
+        std::vector<int> pl, pi, pj, s0;
+        int nslot = 0;
+        for (const MdPair &x : plist)
+            if (shell_l[x.I] == la && shell_l[x.J] == lb) {
+                const PairRef &r = pairs[x.p];
+                if (x.p > INT32_MAX) return fail(PATMOL_HIP_EUNSUPPORTED, "too many shell pairs");
+                pl.push_back((int)x.p);
+                pi.push_back(x.I);
+                pj.push_back(x.J);
+                s0.push_back(nslot);
+                for (int k = 0; k < r.npp; ++k) slot[r.pp0 + k] = nslot + k;
+                nslot += r.npp;
+            }
+        if (pl.empty()) continue;
+        const size_t nb = bra ? (size_t)nslot * md_bra_regs(la, lb) * 64 : 0;
+        const size_t nk = ket ? (size_t)nslot * md_ket_regs(la, lb) * 64 : 0;
+        bytes += 8 * (nb + nk);
+        if (bytes > MD_TABLE_MAX_BYTES) {
+            t.release();
+            return 0;   // the classes keep their Rys kernels
+        }
+        if ((bra && t.d_bra[pc].alloc(nb)) || (ket && t.d_ket[pc].alloc(nk))) return g_err;
+        DevBuf<int> dl, di, dj, ds;
+        if (dl.upload(pl) || di.upload(pi) || dj.upload(pj) || ds.upload(s0)) return g_err;
+        MdTabArgs ta{dl.p, di.p, dj.p, ds.p, (int)pl.size(), d_xyz, d_pairs, d_pp, bra ? t.d_bra[pc].p : nullptr,
+                     ket ? t.d_ket[pc].p : nullptr};
+        if (launch_md_etab(la, lb, ta, 0)) return fail(PATMOL_HIP_EUNSUPPORTED, "no Hermite table kernel");
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipDeviceSynchronize());
+        dl.release();
+        di.release();
+        dj.release();
+        ds.release();
+    }
+    if (t.d_slot.upload(slot)) return g_err;
+    t.ready = true;
+    return 0;
+}
+
+static int build_md_tables(Ctx &c)
+{
+    std::vector<MdPair> plist;
+    plist.reserve((size_t)c.nshell * (c.nshell + 1) / 2);
+    for (int I = 0; I < c.nshell; ++I)
+        for (int J = 0; J <= I; ++J) plist.push_back({(int64_t)I * (I + 1) / 2 + J, I, J});
+    return build_md_tables(c.md, plist, c.shell_l, c.pairs, c.pp.size(), c.d_xyz.p, c.d_pairs.p, c.d_pp.p);
+}
+
 // ---------------------------------------------------------------------------
 // build: shells, pairs, Schwarz, tiles, work items
 // ---------------------------------------------------------------------------
-static int build_tables(Ctx &c, int start)
+// md: also the McMurchie-Davidson tables (a batch builds its own)
+static int build_tables(Ctx &c, int start, bool md = true)
 {
     const Shell *sh = shells();
     const Center *ce = centers();
@@ -524,6 +639,7 @@ static int build_tables(Ctx &c, int start)
         c.ranges.back().count++;
     }
     if (c.d_items.upload(c.items) || c.d_bound.upload(c.item_bound)) return g_err;
+    if (md && build_md_tables(c)) return g_err;
     c.cls_ready = false;
     c.built = true;
     return 0;
@@ -806,6 +922,7 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         a.item_mol = nullptr;
         a.mols = nullptr;
         a.slot_stride = (long long)c.N * c.N;
+        c.md.fill(a, g_md_mask & md_class_bit(r.bra_pc, r.ket_pc));
         if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
         if (n64 > 0) {
             a.items = list64;
@@ -990,6 +1107,7 @@ static void release_ctx(Ctx &c)
     c.d_pp.release();
     c.d_ppab.release();
     c.d_items.release();
+    c.md.release();
     for (auto st : c.side) (void)hipStreamDestroy(st);
     for (auto ev : c.side_ev) (void)hipEventDestroy(ev);
     c.side.clear();
@@ -1122,6 +1240,8 @@ struct PatmolHipBatch {
     pm::DevBuf<int4> d_items;
     pm::DevBuf<pm::MolDesc> d_mols;
     pm::DevBuf<pm::FxAcc> d_J, d_K;
+    std::vector<int> shell_l;   // l of each (batch-global) sorted shell
+    pm::MdTables md;            // McMurchie-Davidson tables of every molecule's pairs
     hipStream_t s = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1162,6 +1282,7 @@ static int batch_build(PatmolHipBatch &b, hipStream_t s)
         a.item_mol = b.d_item_mol.p + r.first;
         a.mols = b.d_mols.p;
         a.slot_stride = dtot;
+        b.md.fill(a, g_md_mask & md_class_bit(r.bra_pc, r.ket_pc));
         const int n64 = b.n64[ri];
         if (n64 > 0) {
             a.items = b.d_items.p + r.first;
@@ -1221,6 +1342,7 @@ void patmol_hip_set_precision(int mode, double thr)
     if (thr > 0) g_prec_thr = thr;
 }
 void patmol_hip_set_fp32_policy(int every_class) { g_fp32_policy = every_class ? 1 : 0; }
+void patmol_hip_set_jk_engine(int mask) { g_md_mask = mask < 0 ? MD_DEFAULT_MASK : (mask & 15); }
 void patmol_hip_set_timing(int on) { g_timing = on != 0; }
 
 // erifs.c:32-60: the handle keeps the reference's public fields; nfiles
@@ -1566,6 +1688,7 @@ void patmol_hip_batch_free(PatmolHipBatch *b)
     b->d_mols.release();
     b->d_J.release();
     b->d_K.release();
+    b->md.release();
     delete b;
 }
 
@@ -1579,7 +1702,7 @@ int patmol_hip_batch_add(PatmolHipBatch *b)
         if (ensure_tables()) return g_err;
     }
     Ctx c;
-    if (build_tables(c, 0)) {
+    if (build_tables(c, 0, false)) {
         release_ctx(c);
         return g_err;
     }
@@ -1596,6 +1719,7 @@ int patmol_hip_batch_add(PatmolHipBatch *b)
     m.pstart = b->no2tot;
     const int mi = (int)b->mol.size();
     b->xyz.insert(b->xyz.end(), c.shell_xyz.begin(), c.shell_xyz.end());
+    b->shell_l.insert(b->shell_l.end(), c.shell_l.begin(), c.shell_l.end());
     for (int v : c.shell_soff) b->soff.push_back(v + m.fbase);
     for (int v : c.sorted_orig) b->orig.push_back(v + m.sbase);
     for (int v : c.tile_first) b->tile_first.push_back(v + m.sbase);
@@ -1679,6 +1803,17 @@ int patmol_hip_batch_finalize(PatmolHipBatch *b, int prec_mode, double threshold
         b->d_D.alloc(b->dtot) || b->d_J.alloc((size_t)b->dtot * PM_XCD_SLOTS) ||
         b->d_K.alloc((size_t)b->dtot * PM_XCD_SLOTS))
         return g_err;
+    {
+        // Hermite tables over every molecule's shell pairs (batch-global
+        // pair, shell and primitive-pair indices)
+        std::vector<MdPair> plist;
+        for (const BatchMol &m : b->mol)
+            for (int I = 0; I < m.nshell; ++I)
+                for (int J = 0; J <= I; ++J)
+                    plist.push_back({m.pbase + (int64_t)I * (I + 1) / 2 + J, m.sbase + I, m.sbase + J});
+        if (build_md_tables(b->md, plist, b->shell_l, b->pairs, b->pp.size(), b->d_xyz.p, b->d_pairs.p, b->d_pp.p))
+            return g_err;
+    }
     HIPCHK(hipMemset(b->d_P.p, 0, sizeof(double) * b->no2tot));
     HIPCHK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
     // capture the whole G build once

@@ -499,7 +499,8 @@ def main():
         "roofline": {
             "bound": "mfma",
             "bound_detail": "FP64 arithmetic: MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; "
-                            "the J/K kernels run on the FP64 VALU (no MFMA)",
+                            "the Rys J/K kernels run on the FP64 VALU, the McMurchie-Davidson kernel of "
+                            "(dd|dd) and (dd|dp) on the FP64 matrix cores",
             "kernel": f"J/K {dom['class']} FP64",
             "kernel_symbol": dom["kernel_symbol"],
             "achieved": dom["tflops"],

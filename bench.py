@@ -343,6 +343,39 @@ def cpu_baseline(system, P, kept_total):
             "s_per_build_est": kept_total / rate}
 
 
+def cpu_conventional(head, direct, systems):
+    """The reference's conventional accounting (ERI once, twoints.c:405-408;
+    then G from the stored records every iteration, rhfscf.c:102-104) for
+    the headline workload, on the host threads.  ERI once: the oracle's
+    direct build of every kept quartet (cpu_baseline's estimate; it also
+    digests, so it bounds the ERI pass from above).  G per iteration: the
+    record count of the kept quartets (every Cartesian component, an upper
+    bound on the records the writer keeps above 1e-14) divided by the
+    oracle's threaded record scatter (scfaux.c:428-552 restated), timed on
+    the 10M records of a water dimer."""
+    import oracle
+
+    threads = cpu_threads()
+    nc = {"s": 1, "p": 3, "d": 6}
+    records = 0
+    for cls, v in head["per_class_fp64"].items():
+        a, b, c, d = cls[1], cls[2], cls[4], cls[5]
+        records += v["quartets"] * nc[a] * nc[b] * nc[c] * nc[d]
+    dimer = systems.water_cluster(2, 1, 1, seed=7)
+    ints = oracle.Basis(dimer).two_electron_ints()
+    P = np.eye(dimer.norb)[np.tril_indices(dimer.norb)] * 0.1
+    ints.rhf_gmatrix(P, nthreads=threads)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ints.rhf_gmatrix(P, nthreads=threads)
+    t_g = (time.perf_counter() - t0) / 3
+    rate = ints.n / t_g
+    return {"workload": head["workload"], "threads": threads, "eri_once_s_est": direct["s_per_build_est"],
+            "records_est": int(records), "record_bytes_est": int(16 * records),
+            "scatter_records_per_s": rate, "scatter_sample": f"{ints.n} records of a water dimer in {t_g * 1e3:.1f} ms",
+            "g_per_iteration_s_est": records / rate}
+
+
 def erint_interop(pm, systems, with_cpu):
     """ERINT record interop on a water dimer (s/p/d, 58 shells): the writer
     (every canonical integral above 1e-14 in the reference's order), and the
@@ -525,6 +558,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (oracle direct build, bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline(head_sys, P16, kept)
+        out["cpu_conventional"] = cpu_conventional(head, out["cpu_baseline"], systems)
+        log(f"cpu conventional accounting: {out['cpu_conventional']}")
     out["erint_interop"] = erint_interop(pm, systems, not args.no_cpu)
     log(f"ERINT interop: {out['erint_interop']}")
     print(json.dumps(out))

@@ -433,18 +433,19 @@ def calibrate_copy(nbytes):
     return 0
 
 
-def profile_build(name, fp32=False):
+def profile_build(name, fp32=False, md_mask=-1):
     """One G build of `name` with the class kernels serialised on one stream
     (after one warm-up build), for rocprofv3 kernel-trace and counter runs:
     FP64, or with fp32 every item in FP32 in the classes whose FP32 kernel
-    pays (the tuned policy)."""
+    pays (the tuned policy); md_mask: the classes on the McMurchie-Davidson
+    kernel (-1: the library's default)."""
     import torch
     import patmol_amd as pm
     from patmol_amd import systems
 
     system = getattr(systems, name)()
     pm.load_system(system)
-    pm.set_options(tau=TAU, precision=0, timing=True, strict=False)
+    pm.set_options(tau=TAU, precision=0, timing=True, strict=False, md_engine=md_mask)
     S, T, H = pm.one_electron_ints()
     P = core_guess_density(S, H, system.norb, system.nelec // 2)
     store = pm.EriStore()
@@ -470,6 +471,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--profile-build", default="", metavar="SYSTEM")
     ap.add_argument("--profile-fp32", action="store_true", help="with --profile-build: every item in FP32")
+    ap.add_argument("--md-engine", type=int, default=-1, metavar="MASK",
+                    help="with --profile-build: classes on the McMurchie-Davidson kernel (1 (dd|dd), 2 (dd|dp), "
+                         "4 (dp|dp), 8 (dd|pp); -1 the default)")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-molecule batch block")
     ap.add_argument("--batch-scf", action="store_true",
                     help="also converge every molecule of the batch (FP64 and mixed, about two minutes each)")
@@ -481,7 +485,7 @@ def main():
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.profile_build:
-        return profile_build(args.profile_build, args.profile_fp32)
+        return profile_build(args.profile_build, args.profile_fp32, args.md_engine)
 
     import torch
     import patmol_amd as pm

@@ -417,6 +417,61 @@ def erint_interop(pm, systems, with_cpu):
     return out
 
 
+RING_POINTS = [[0.0, 0.0, 0.0], [0.0, 0.0, 2.0], [0.0, 0.0, -2.0]]
+RING_ORACLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "dpm_ring.json")
+
+
+def positron_ring(pm, systems):
+    """The DPM grid loop (rhf_dpm_calculation, rhfscf.c:217-263, and its UHF
+    form, uhfscf.c:213-262) on the planar C6H6-shaped ring with the positron
+    at the ring centre and 2 bohr above and below it: in FP64, then RHF and
+    UHF with items below MIXED_THR (Schwarz bound x max|P|) in FP32.  Each
+    loop runs twice and the second run is timed.  Energies and vpol are
+    reported against the oracle's FP64 loop on the same ring and points
+    (kept as a fixture, tests/golden/dpm_ring.json: the oracle takes minutes)
+    and the mixed loops also against the FP64 device loop."""
+    system = systems.benzene_ring()
+    gold = None
+    if os.path.exists(RING_ORACLE):
+        with open(RING_ORACLE) as f:
+            gold = json.load(f)
+
+    def run(precision, uhf):
+        for _ in range(2):
+            pm.load_system(system)
+            pm.set_options(tau=TAU, precision=precision, mixed_threshold=MIXED_THR, strict=False, timing=False)
+            pm.add_dpm_center()
+            t0 = time.perf_counter()
+            try:
+                pts, gs = pm.dpm_calculation(RING_POINTS, uhf=uhf)
+            finally:
+                pm.set_options(precision=0)
+            t = time.perf_counter() - t0
+        return pts, gs, t
+
+    out = {"workload": f"C6H6 ring, {system.nshell} electron shells, {system.norb} functions + 13 DPM functions; "
+                       f"positron at {RING_POINTS} bohr",
+           "mixed_threshold": MIXED_THR}
+    ref = None
+    for name, prec, uhf in (("fp64_rhf", 0, False), ("mixed_rhf", 1, False), ("mixed_uhf", 1, True)):
+        got, gs, t = run(prec, uhf)
+        r = {"s": t, "converged": all(p["converged"] == 1 for p in got), "ground_state_energy": gs["energy"],
+             "vpol": [p["vpol"] for p in got], "iterations": [p["iterations"] for p in got]}
+        if gold is not None:
+            r["max_dvpol_vs_oracle"] = max(abs(g["vpol"] - o["vpol"]) for g, o in zip(got, gold["results"]))
+            r["max_dE_positron_vs_oracle"] = max(abs(g["e_conv"] - o["e_conv"]) for g, o in zip(got, gold["results"]))
+            r["dE_ground_state_vs_oracle"] = abs(gs["energy"] - gold["ground_state"]["energy"])
+        if ref is None:
+            ref = got
+        else:
+            r["max_dvpol_vs_fp64"] = max(abs(g["vpol"] - o["vpol"]) for g, o in zip(got, ref))
+            r["max_dE_positron_vs_fp64"] = max(abs(g["e_conv"] - o["e_conv"]) for g, o in zip(got, ref))
+        out[name] = r
+    if gold is not None:
+        out["oracle_vpol"] = [o["vpol"] for o in gold["results"]]
+    return out
+
+
 def calibrate_copy(nbytes):
     """Three launches of a plain device-to-device copy of nbytes: the
     reference reading for the HBM traffic counters."""
@@ -566,6 +621,8 @@ def main():
         log(f"cpu conventional accounting: {out['cpu_conventional']}")
     out["erint_interop"] = erint_interop(pm, systems, not args.no_cpu)
     log(f"ERINT interop: {out['erint_interop']}")
+    out["positron_ring"] = positron_ring(pm, systems)
+    log(f"positron ring: {out['positron_ring']}")
     print(json.dumps(out))
     return 0
 

@@ -69,7 +69,11 @@ void patmol_hip_set_schwarz_threshold(double tau);
  * products, FP64 block sums, fixed-point accumulation), max|P| taken over
  * the six density blocks each item contracts with (RHF builds) or over the
  * whole density (UHF, batches); only in the classes whose FP32 kernel the
- * autotuner measured faster than their FP64 kernel (jk3_tuned.inc). */
+ * autotuner measured faster than their FP64 kernel (jk3_tuned.inc), and not
+ * in (dd|dd) while the McMurchie-Davidson engine runs it (its FP64 items
+ * there are faster).  The device SCF (patmol_hip_rhf_scf / _uhf_scf) builds
+ * its last iterations in FP64: from the iteration whose density change is
+ * within 20x the stop bound, or stops falling below 1e-6. */
 void patmol_hip_set_precision(int mode, double threshold);
 /* Autotuning aid (no counterpart in the reference): 1 sends the items below
  * the threshold to FP32 in every class, 0 (default) follows jk3_tuned.inc. */

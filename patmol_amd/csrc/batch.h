@@ -38,6 +38,12 @@ struct BuildCounts {
 // are skipped (their contributions are below the threshold).  The precision
 // split, when on, uses max|dP|.
 int rhf_gmatrix_delta(const ERIFS *e, const double *dP, double *dG, hipStream_t s, BuildCounts *counts);
+
+// The precision mode of the G builds that follow (patmol_hip_set_precision's
+// mode: 0 FP64, 1 mixed), read and set without touching the threshold: the
+// device SCF runs its last iterations in FP64.
+int precision_mode();
+void set_precision_mode(int mode);
 // the counts of the last build of e (0 if e has no tables)
 int last_build_counts(const ERIFS *e, BuildCounts *counts);
 

@@ -1146,6 +1146,39 @@ int scf_fail(Scf &c, int code, const char *where)
     return aux_fail(code, msg);
 }
 
+// The last iterations of a mixed-precision SCF build G in FP64 (no
+// counterpart in the reference).  The FP32 items' rounding moves the density
+// by a small amount that does not shrink as the SCF converges: on the
+// 300-function C6H6 ring at threshold 1e-4 the density change pdiff stalls
+// at 1-2e-9, while the FP64 SCF falls through the stop bound (2e-10 there)
+// in 32 iterations, so the stop rule pdiff < eps of rhfscf.c:52-59 might
+// never hold.  Once pdiff is within MIXED_TAIL_FACTOR of eps, or stops
+// falling below 1e-6, the precision mode is set to FP64 for the rest of the
+// SCF; the caller's mode is restored when the SCF returns.
+constexpr double MIXED_TAIL_FACTOR = 20.0;
+struct PrecisionTail {
+    const int mode0 = precision_mode();
+    bool on = false;
+    double prev = HUGE_VAL;
+    PrecisionTail() = default;
+    PrecisionTail(const PrecisionTail &) = delete;
+    PrecisionTail &operator=(const PrecisionTail &) = delete;
+    ~PrecisionTail() { set_precision_mode(mode0); }
+    // after iteration `it` (>= 1) with density change pdiff; true when this
+    // call switched the builds to FP64
+    bool step(int it, double pdiff, double eps)
+    {
+        bool now = false;
+        if (mode0 == 1 && !on && it > 1 && (pdiff < MIXED_TAIL_FACTOR * eps || (pdiff < 1e-6 && pdiff >= prev))) {
+            on = true;
+            now = true;
+            set_precision_mode(0);
+        }
+        prev = pdiff;
+        return now;
+    }
+};
+
 }  // namespace
 }  // namespace pm
 
@@ -1211,6 +1244,7 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
         tally.skipped += b.skipped;
     };
     const bool log = scf_log();
+    PrecisionTail tail;
     double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0, dms = 0.0;
     int iteration = 0, done = 0;
     bool first = true;
@@ -1283,6 +1317,7 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
                     enrg + enuc);
         if (iteration == 0) continue;
         if (done || iteration >= maxits) break;
+        if (tail.step(iteration, pdiff, eps) && log) fprintf(stderr, "FP64 G builds from iteration %d\n", iteration + 1);
     }
     if (hipStreamSynchronize(c.s) != hipSuccess) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: device failure");
     const double tot = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -1368,6 +1403,7 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
     // the purification (their launches and trace checks dominate)
     c.purify = g_scf_density >= 1 && n >= 128;
     const bool log = scf_log();
+    PrecisionTail tail;
     double eold = 0.0, enrg = 0.0, ediff = 0.0, pdiff = 0.0, gms = 0.0;
     int iteration = 0, done = 0;
     bool first = true;
@@ -1422,6 +1458,7 @@ extern "C" int patmol_hip_uhf_scf(const ERIFS *mints, const ERIFS *xints, const 
                     enrg + enuc);
         if (iteration == 0) continue;
         if (done || iteration >= maxits) break;
+        if (tail.step(iteration, pdiff, eps) && log) fprintf(stderr, "FP64 G builds from iteration %d\n", iteration + 1);
     }
     if (hipStreamSynchronize(c.s) != hipSuccess) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_uhf_scf: device failure");
     const double tot = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

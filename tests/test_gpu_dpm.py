@@ -10,8 +10,14 @@ density, and vpol = E_conv - E_first.  Compared here: the xints G build
 two_electron_ints(start) records and scatter; mints + xints against the
 oracle's G of the whole augmented basis; the grid loop's energies, vpol and
 iteration counts against the oracle's loop; closed-shell UHF against RHF;
-and the mirror symmetry of a planar ring (z -> -z).
+the mirror symmetry of a planar ring (z -> -z); and the ring's grid loop at
+its centre and 2 bohr above and below it against the oracle's answer kept in
+tests/golden/dpm_ring.json (written by make_dpm_ring.py there), in FP64 and
+in mixed precision (RHF and UHF).
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -160,3 +166,37 @@ def test_dpm_ring_mirror_symmetry():
     assert abs(got[1]["e_conv"] - got[2]["e_conv"]) < E_TOL
     assert abs(got[1]["vpol"] - got[2]["vpol"]) < E_TOL
     assert got[0]["vpol"] < 0.0 and got[1]["vpol"] < 0.0
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed_rhf", "mixed_uhf"])
+def test_dpm_ring_vs_oracle_fixture(golden_dir, mode):
+    """The planar C6H6-shaped ring (300 electron functions): ground state,
+    E_first, E_conv and vpol at the ring centre and 2 bohr above and below it
+    against the oracle's loop, within 1e-8 Eh in FP64 and 1e-6 Eh in mixed
+    precision (items with Schwarz bound x max|P| < 1e-4 in FP32, the RHF and
+    the UHF SCF), iteration counts within 3 of the oracle's (the mixed SCF
+    builds its last iterations in FP64, without which the FP32 rounding kept
+    the density change above the stop bound for up to 200 iterations)."""
+    gold = json.load(open(os.path.join(golden_dir, "dpm_ring.json")))
+    s = systems.benzene_ring()
+    pm.load_system(s)
+    mixed = mode != "fp64"
+    pm.set_options(tau=0.0, precision=1 if mixed else 0, mixed_threshold=1e-4, strict=False, scf_density=1)
+    pm.add_dpm_center()
+    try:
+        got, gs = pm.dpm_calculation(gold["points"], uhf=(mode == "mixed_uhf"))
+    finally:
+        pm.set_options(precision=0)
+    tol = 1e-6 if mixed else E_TOL
+    ogs = gold["ground_state"]
+    note(f"ring {mode}: ground state gpu {gs['energy']:.12f} ({gs['iterations']} it), oracle "
+         f"{ogs['energy']:.12f} ({ogs['iterations']} it)")
+    assert gs["converged"] == 1 and abs(gs["energy"] - ogs["energy"]) < tol
+    assert abs(gs["iterations"] - ogs["iterations"]) <= 3
+    for p, g, r in zip(gold["points"], got, gold["results"]):
+        note(f"  r={p}: vpol gpu {g['vpol']:.10f} oracle {r['vpol']:.10f}; E {g['e_conv']:.12f} / "
+             f"{r['e_conv']:.12f}; it {g['iterations']} / {r['iterations']}")
+        assert g["converged"] == 1 and r["converged"] == 1
+        for k in ("e_first", "e_conv", "vpol"):
+            assert abs(g[k] - r[k]) < tol, k
+        assert abs(g["iterations"] - r["iterations"]) <= 3

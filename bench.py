@@ -185,7 +185,9 @@ def measure(torch, pm, system, key, steps, warmup, scf):
         for key, mode, incr in (("fp64", 0, 0), ("mixed", 1, 0), ("fp64_incremental", 0, INCR_PERIOD),
                                 ("mixed_incremental", 1, INCR_PERIOD)):
             pm.set_options(precision=mode, mixed_threshold=MIXED_THR, scf_incremental=incr)
-            r, _, _ = pm.rhf_scf_device(store=store)
+            r, Pc, _ = pm.rhf_scf_device(store=store)
+            if key == "fp64":
+                Pconv = Pc
             nupd = r["iterations"] + 1
             runs[key] = {"energy": r["energy"], "iterations": r["iterations"], "converged": r["converged"],
                          "ms_per_iteration": r["total_ms"] / nupd, "gbuild_ms_per_iteration": r["gbuild_ms"] / nupd,
@@ -204,6 +206,18 @@ def measure(torch, pm, system, key, steps, warmup, scf):
         out["dE_scf_mixed_vs_fp64"] = runs["mixed"]["dE_vs_fp64"]
         g0 = runs["fp64"]["gbuild_ms_per_iteration"]
         out["scf_g_speedup_vs_fp64"] = {k: g0 / runs[k]["gbuild_ms_per_iteration"] for k in runs}
+        # analytic nuclear gradient at the converged FP64 density: two-electron
+        # part on the device over the same screened work list, one-electron
+        # part and W = P F P (one device G build) on the host
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            g = pm.rhf_gradient(Pconv, store=store)
+            times.append(time.perf_counter() - t0)
+        out["gradient"] = {"s": times[-1], "s_first": times[0], "max_abs_force": float(np.abs(g).max()),
+                           "max_abs_sum_of_forces": float(np.abs(g.sum(axis=0)).max())}
+        log(f"  analytic gradient: {times[-1]:.3f} s (first call {times[0]:.3f} s), max |dE/dR| "
+            f"{out['gradient']['max_abs_force']:.6f}")
     store.close()
     return out, P
 

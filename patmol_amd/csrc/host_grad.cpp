@@ -138,9 +138,14 @@ extern "C" int patmol_host_onee_gradient(const double *P, const double *W, doubl
     double **nf = normalizationFactors();
     if (!sh || !ce || !P || !W || !grad) return 1;
     const double pi32 = 5.5683279968317079;
-    Herm1D d[3];
-    std::vector<double> R;
-    for (int is = 0; is < ns; ++is) {
+    // one row of sums per first shell, filled in parallel and added in
+    // shell order: the result does not depend on the number of threads
+    std::vector<double> rows((size_t)ns * 3 * nc, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int is = ns - 1; is >= 0; --is) {
+        Herm1D d[3];
+        std::vector<double> R;
+        double *row = &rows[(size_t)is * 3 * nc];
         const Shell &a = sh[is];
         const double *A = ce[a.cen].r;
         for (int js = 0; js <= is; ++js) {
@@ -216,13 +221,15 @@ extern "C" int patmol_host_onee_gradient(const double *P, const double *W, doubl
                     }
                 }
             for (int x = 0; x < 3; ++x) {
-                grad[3 * a.cen + x] += gA[x];
-                grad[3 * b.cen + x] += gB[x];
+                row[3 * a.cen + x] += gA[x];
+                row[3 * b.cen + x] += gB[x];
             }
             for (int cc = 0; cc < nc; ++cc)
-                for (int x = 0; x < 3; ++x) grad[3 * cc + x] += gC[3 * cc + x];
+                for (int x = 0; x < 3; ++x) row[3 * cc + x] += gC[3 * cc + x];
         }
     }
+    for (int is = 0; is < ns; ++is)
+        for (int k = 0; k < 3 * nc; ++k) grad[k] += rows[(size_t)is * 3 * nc + k];
     return 0;
 }
 

@@ -1317,7 +1317,12 @@ extern "C" int patmol_hip_rhf_scf(const ERIFS *mints, const ERIFS *xints, const 
                     enrg + enuc);
         if (iteration == 0) continue;
         if (done || iteration >= maxits) break;
-        if (tail.step(iteration, pdiff, eps) && log) fprintf(stderr, "FP64 G builds from iteration %d\n", iteration + 1);
+        if (tail.step(iteration, pdiff, eps)) {
+            // an incremental SCF also rebuilds its reference G in FP64: the
+            // mixed one would carry its FP32 rounding to the end
+            if (incr > 0) since_full = 0;
+            if (log) fprintf(stderr, "FP64 G builds from iteration %d\n", iteration + 1);
+        }
     }
     if (hipStreamSynchronize(c.s) != hipSuccess) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_rhf_scf: device failure");
     const double tot = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

@@ -135,7 +135,7 @@ def measure(torch, pm, system, key, steps, warmup, scf):
     k = int(np.argmax(cls_ms))
     achieved = f36[k] / (cls_ms[k] * 1e-3) / 1e12 if cls_ms[k] > 0 else 0.0
     dom = f"({CLASS_NAMES[k // 6]}|{CLASS_NAMES[k % 6]})"
-    dom_kernel = store.class_kernel_name(k // 6, k % 6)
+    dom_kernel = store.class_kernel_name(k // 6, k % 6, md_mask=store.engine_mask())
     traffic = traffic_of(key, dom_kernel)
     per_class = {f"({CLASS_NAMES[i // 6]}|{CLASS_NAMES[i % 6]})":
                  {"ms": round(float(cls_ms[i]), 4), "quartets": int(q36[i]),

@@ -83,7 +83,10 @@ void patmol_hip_set_fp32_policy(int every_class);
  * McMurchie-Davidson kernel (Hermite expansion, the two contractions on the
  * FP64 matrix cores) instead of their Rys kernels, a mask: bit 0 (dd|dd),
  * bit 1 (dd|dp), bit 2 (dp|dp), bit 3 (dd|pp); -1 restores the default,
- * (dd|dd) and (dd|dp).  For comparisons and timing: both engines give the
+ * (dd|dd) and (dd|dp), where (dd|dp) stays on its Rys kernel for a basis
+ * whose (dd|dp) work items are mostly kept (patmol_hip_jk_engine_mask);
+ * bit 4 keeps every class of the mask on the McMurchie-Davidson kernel
+ * regardless.  For comparisons and timing: both engines give the
  * reference's integrals. */
 void patmol_hip_set_jk_engine(int mask);
 
@@ -282,6 +285,13 @@ int patmol_hip_class_stats(const ERIFS *tfiles, int64_t *quartets36, double *flo
 
 /* Schwarz bound of every shell pair, indexed by original shell indices
  * (I >= J) at I*(I+1)/2 + J. */
+/* No counterpart in the reference.  The classes whose FP64 items run on
+ * the McMurchie-Davidson kernel for this handle (patmol_hip_set_jk_engine's
+ * bits): the engine mask in force, less (dd|dp) when more than 40 % of the
+ * shell-quartet slots of its work items survive the Schwarz screen (there
+ * the Rys kernel measured faster), 0 when no Hermite tables were built;
+ * negative on error. */
+int patmol_hip_jk_engine_mask(const ERIFS *tfiles);
 int patmol_hip_pair_schwarz(const ERIFS *tfiles, double *q_out);
 
 /* Work items as built by two_electron_ints: per item the four tile indices

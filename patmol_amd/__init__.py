@@ -129,6 +129,8 @@ def lib():
         G.patmol_hip_class_stats.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.int64), _dp]
         G.patmol_hip_pair_schwarz.restype = C.c_int
         G.patmol_hip_pair_schwarz.argtypes = [C.c_void_p, _dp]
+        G.patmol_hip_jk_engine_mask.restype = C.c_int
+        G.patmol_hip_jk_engine_mask.argtypes = [C.c_void_p]
         G.patmol_hip_work_items.restype = C.c_int64
         G.patmol_hip_work_items.argtypes = [C.c_void_p] + [C.c_void_p] * 7
         G.patmol_hip_eri_blocks.restype = C.c_int
@@ -219,7 +221,7 @@ def exported_symbols():
             "patmol_hip_set_schwarz_threshold", "patmol_hip_set_precision", "patmol_hip_set_fp32_policy",
             "patmol_hip_set_jk_engine",
             "patmol_hip_rhf_gmatrix_device", "patmol_hip_rhf_gmatrix_device_noalloc", "patmol_hip_stats", "patmol_hip_class_stats",
-            "patmol_hip_pair_schwarz",
+            "patmol_hip_pair_schwarz", "patmol_hip_jk_engine_mask",
             "patmol_hip_work_items", "patmol_hip_eri_blocks", "patmol_hip_eri_blocks_prec", "patmol_hip_last_timing",
             "patmol_hip_set_timing", "patmol_hip_rhf_scf", "patmol_hip_uhf_scf", "patmol_hip_diag_fock",
             "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "patmol_hip_set_scf_incremental", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
@@ -247,7 +249,9 @@ def set_options(tau: Optional[float] = None, precision: Optional[int] = None,
                 md_engine: Optional[int] = None):
     """md_engine: classes whose FP64 items run on the McMurchie-Davidson
     kernel, a mask (1 (dd|dd), 2 (dd|dp), 4 (dp|dp), 8 (dd|pp); 0 none, -1 the
-    library's default (dd|dd) and (dd|dp))."""
+    library's default (dd|dd) and (dd|dp)); (dd|dp) stays on its Rys kernel
+    for a basis whose (dd|dp) work items are mostly kept, unless 16 is added
+    (EriStore.engine_mask tells what runs)."""
     _, G = lib()
     if md_engine is not None:
         G.patmol_hip_set_jk_engine(int(md_engine))
@@ -373,6 +377,17 @@ class EriStore:
         if (la, lb, lc, ld) == (2, 2, 2, 2):
             return f"void pm::jk_kernel<{la}, {lb}, {lc}, {ld}, {r}>"
         return f"void pm::jk_kernel3<{la}, {lb}, {lc}, {ld}, {r}>"
+
+    def engine_mask(self) -> int:
+        """Classes whose FP64 items run on the McMurchie-Davidson kernel for
+        this store (patmol_hip_jk_engine_mask; bits as set_options'
+        md_engine), for class_kernel_name's md_mask."""
+        _, L = lib()
+        m = L.patmol_hip_jk_engine_mask(self.h)
+        if m < 0:
+            _check("patmol_hip_jk_engine_mask")
+            raise PatmolError("patmol_hip_jk_engine_mask failed")
+        return m
 
     def pair_schwarz(self) -> np.ndarray:
         _, L = lib()

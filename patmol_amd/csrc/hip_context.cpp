@@ -102,6 +102,18 @@ static int md_class_bit(int bra_pc, int ket_pc)
 // largest Hermite-coefficient table set built per basis; above it the
 // classes stay on their Rys kernels
 static const size_t MD_TABLE_MAX_BYTES = (size_t)8 << 30;
+// (dd|dp) runs on the McMurchie-Davidson kernel only when at most this share
+// of the shell-quartet slots of its work items survives the Schwarz screen:
+// that kernel packs the kept ket pairs of a bra pair into its lanes, the Rys
+// kernel keeps a lane per slot.  Measured (kept share; MD against Rys time of
+// the class): (H2O)16 0.25, 8.9 against 9.7 ms; (H2O)8 0.53, 1.24 against
+// 1.05 ms; the 64-molecule batch 0.93, 124 ms slower in all.
+static const double MD_DDDP_MAX_KEPT = 0.4;
+// engine-mask bit that keeps every class of the mask on the McMurchie-Davidson
+// kernel whatever its kept share (tests of that kernel)
+static const int MD_FORCE = 16;
+// the mask bits in force for a basis whose rule switched off `off`
+static int md_effective(int off) { return g_md_mask & 15 & ((g_md_mask & MD_FORCE) ? 15 : ~off); }
 
 // classes whose FP64 items on the McMurchie-Davidson kernel took less time
 // than their FP32 Rys kernel on (H2O)16 ((dd|dd): 2.7 against 5.3 ms for
@@ -249,6 +261,10 @@ struct MdTables {
 struct Ctx {
     bool built = false;
     int N = 0, nshell = 0, start = 0;
+    // engine-mask bits switched off for this basis (MD_DDDP_MAX_KEPT)
+    int md_off = 0;
+    // kept shell quartets and slots of a sample of the (dd|dp) items
+    int64_t ddp_kept = 0, ddp_slots = 0;
     double tau = 1e-12;
     // host tables
     std::vector<int> sorted_orig, orig_to_sorted, shell_l, shell_nst, shell_off;
@@ -383,6 +399,40 @@ static int build_md_tables(Ctx &c)
 // build: shells, pairs, Schwarz, tiles, work items
 // ---------------------------------------------------------------------------
 // md: also the McMurchie-Davidson tables (a batch builds its own)
+// Kept shell quartets and shell-quartet slots (canonical order inside equal
+// tiles) of up to 512 evenly spaced work items of range r: the kept share of
+// a class's slots, the same rule as the kernels' validity test
+static void sample_kept(const Ctx &c, const ClassRange &r, int64_t &kept, int64_t &slots)
+{
+    const int stride = std::max(1, r.count / 512);
+    for (int it = r.first; it < r.first + r.count; it += stride) {
+        const int4 t = c.items[it];
+        const bool sameBra = t.x == t.y, sameKet = t.z == t.w, samePair = t.x == t.z && t.y == t.w;
+        for (int i = 0; i < c.tile_size[t.x]; ++i) {
+            const int I = c.tile_first[t.x] + i;
+            for (int j = 0; j < c.tile_size[t.y]; ++j) {
+                const int J = c.tile_first[t.y] + j;
+                if (sameBra && J > I) continue;
+                const double qb = c.pair_q[(int64_t)I * (I + 1) / 2 + J];
+                for (int kq = 0; kq < c.tile_size[t.z]; ++kq) {
+                    const int K = c.tile_first[t.z] + kq;
+                    for (int lq = 0; lq < c.tile_size[t.w]; ++lq) {
+                        const int L = c.tile_first[t.w] + lq;
+                        if (sameKet && L > K) continue;
+                        if (samePair && !(I > K || (I == K && J >= L))) continue;
+                        ++slots;
+                        if (qb * c.pair_q[(int64_t)K * (K + 1) / 2 + L] < c.tau) continue;
+                        if (c.start > 0 && std::max(std::max(c.sorted_orig[I], c.sorted_orig[J]),
+                                                    std::max(c.sorted_orig[K], c.sorted_orig[L])) < c.start)
+                            continue;
+                        ++kept;
+                    }
+                }
+            }
+        }
+    }
+}
+
 static int build_tables(Ctx &c, int start, bool md = true)
 {
     const Shell *sh = shells();
@@ -648,6 +698,11 @@ static int build_tables(Ctx &c, int start, bool md = true)
     }
     if (c.d_items.upload(c.items) || c.d_bound.upload(c.item_bound)) return g_err;
     if (md && build_md_tables(c)) return g_err;
+    c.md_off = 0;
+    c.ddp_kept = c.ddp_slots = 0;
+    for (const ClassRange &r : c.ranges)
+        if (md_class_bit(r.bra_pc, r.ket_pc) == 2) sample_kept(c, r, c.ddp_kept, c.ddp_slots);
+    if (c.ddp_slots > 0 && (double)c.ddp_kept > MD_DDDP_MAX_KEPT * (double)c.ddp_slots) c.md_off |= 2;
     c.cls_ready = false;
     c.built = true;
     return 0;
@@ -930,7 +985,7 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         a.item_mol = nullptr;
         a.mols = nullptr;
         a.slot_stride = (long long)c.N * c.N;
-        c.md.fill(a, g_md_mask & md_class_bit(r.bra_pc, r.ket_pc));
+        c.md.fill(a, md_effective(c.md_off) & md_class_bit(r.bra_pc, r.ket_pc));
         if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
         if (n64 > 0) {
             a.items = list64;
@@ -1255,6 +1310,9 @@ struct PatmolHipBatch {
     hipGraphExec_t exec = nullptr;
     int64_t items_fp32 = 0;
     double tau = 1e-12;
+    // (dd|dp) kept share over every molecule's sample (MD_DDDP_MAX_KEPT)
+    int64_t ddp_kept = 0, ddp_slots = 0;
+    int md_off = 0;
 };
 
 namespace pm {
@@ -1290,7 +1348,7 @@ static int batch_build(PatmolHipBatch &b, hipStream_t s)
         a.item_mol = b.d_item_mol.p + r.first;
         a.mols = b.d_mols.p;
         a.slot_stride = dtot;
-        b.md.fill(a, g_md_mask & md_class_bit(r.bra_pc, r.ket_pc));
+        b.md.fill(a, md_effective(b.md_off) & md_class_bit(r.bra_pc, r.ket_pc));
         const int n64 = b.n64[ri];
         if (n64 > 0) {
             a.items = b.d_items.p + r.first;
@@ -1350,7 +1408,7 @@ void patmol_hip_set_precision(int mode, double thr)
     if (thr > 0) g_prec_thr = thr;
 }
 void patmol_hip_set_fp32_policy(int every_class) { g_fp32_policy = every_class ? 1 : 0; }
-void patmol_hip_set_jk_engine(int mask) { g_md_mask = mask < 0 ? MD_DEFAULT_MASK : (mask & 15); }
+void patmol_hip_set_jk_engine(int mask) { g_md_mask = mask < 0 ? MD_DEFAULT_MASK : (mask & 31); }
 void patmol_hip_set_timing(int on) { g_timing = on != 0; }
 
 // erifs.c:32-60: the handle keeps the reference's public fields; nfiles
@@ -1514,6 +1572,13 @@ int patmol_hip_last_timing(const ERIFS *e, double *total_ms, double *class_ms)
     if (total_ms) *total_ms = c->last_ms;
     if (class_ms) memcpy(class_ms, c->class_ms, sizeof(c->class_ms));
     return 0;
+}
+
+int patmol_hip_jk_engine_mask(const ERIFS *e)
+{
+    Ctx *c = ctx_of(e);
+    if (!c || !c->built) return -fail(PATMOL_HIP_ESTATE, "no tables");
+    return c->md.ready ? md_effective(c->md_off) : 0;
 }
 
 int patmol_hip_pair_schwarz(const ERIFS *e, double *q)
@@ -1743,6 +1808,8 @@ int patmol_hip_batch_add(PatmolHipBatch *b)
             b->its.push_back({make_int4(t.x + m.tbase, t.y + m.tbase, t.z + m.tbase, t.w + m.tbase), mi, r.bra_pc,
                               r.ket_pc, c.item_bound[i]});
         }
+    b->ddp_kept += c.ddp_kept;
+    b->ddp_slots += c.ddp_slots;
     b->dtot += (long long)c.N * c.N;
     b->no2tot += (long long)c.N * (c.N + 1) / 2;
     b->nshell += c.nshell;
@@ -1759,6 +1826,7 @@ int patmol_hip_batch_finalize(PatmolHipBatch *b, int prec_mode, double threshold
     g_err = PATMOL_HIP_OK;
     if (!b || b->mol.empty()) return fail(PATMOL_HIP_EBADARG, "patmol_hip_batch_finalize: empty batch");
     if (b->finalized) return fail(PATMOL_HIP_ESTATE, "patmol_hip_batch_finalize: already finalised");
+    b->md_off = b->ddp_slots > 0 && (double)b->ddp_kept > MD_DDDP_MAX_KEPT * (double)b->ddp_slots ? 2 : 0;
     // per class: FP64 items first, each part by bound descending (the work
     // list of every molecule merged into one queue per class)
     auto fp32 = [&](const PatmolHipBatch::It &x) {

@@ -3,7 +3,9 @@ the FP64 work items of (dd|dd), (dd|dp), (dp|dp) and (dd|pp), against the
 oracle's restatement of the reference's Rys path (twoints.c:23-186,
 scfaux.c:428-714) and against the library's own Rys kernels for the same
 classes (patmol_hip_set_jk_engine mask 0); the tests put all four classes
-on the MD kernel (mask 15), not only the default (dd|dd) and (dd|dp).
+on the MD kernel (mask 15, with bit 4 so that the kept-share rule does not
+send (dd|dp) back to its Rys kernel), not only the default (dd|dd) and
+(dd|dp).
 
 Tolerances as in test_gpu_parity.py: 1e-10 absolute on G elements against
 the oracle; the two device engines agree to 1e-12 (different algorithms,
@@ -22,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 G_TOL = 1e-10
 ENGINE_TOL = 1e-12
-ALL = 15   # (dd|dd), (dd|dp), (dp|dp), (dd|pp) on the MD kernel
+ALL = 31   # (dd|dd), (dd|dp), (dp|dp), (dd|pp) on the MD kernel, whatever their kept share (bit 4)
 
 
 def _store(sys_, tau=0.0, start=0):
@@ -121,3 +123,23 @@ def test_md_water8_vs_rys_screened():
     note(f"MD (H2O)8: max |G_md - G_rys| = {e:.3e}")
     assert e < ENGINE_TOL
     st.close()
+
+
+def test_engine_mask_kept_share_rule():
+    """(dd|dp) leaves the MD kernel by default where most of its work items'
+    shell-quartet slots are kept (the compact water dimer), and stays on it
+    when bit 4 is set; (dd|dd) stays in both cases."""
+    s = systems.water_cluster(2, 1, 1, seed=1001)
+    pm.load_system(s)
+    pm.set_options(tau=1e-12, precision=0, strict=False, md_engine=-1)
+    st = pm.EriStore()
+    try:
+        m_default = st.engine_mask()
+        pm.set_options(md_engine=3 | 16)
+        m_forced = st.engine_mask()
+    finally:
+        pm.set_options(md_engine=-1)
+    st.close()
+    note(f"engine mask: default {m_default}, forced {m_forced}")
+    assert m_default == 1 and m_forced == 3
+

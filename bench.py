@@ -19,6 +19,7 @@ functions) the same way.
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
        python bench.py --calibrate-copy BYTES   (plain device copy of BYTES,
        three launches, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)
+       python bench.py --profile-gradient SYSTEM   (the analytic gradient, twice)
        python bench.py --profile-build SYSTEM   (one FP64 build with classes
        serialised, for rocprofv3 runs; SYSTEM water8 or water16)
 """
@@ -531,6 +532,34 @@ def profile_build(name, fp32=False, md_mask=-1):
     return 0
 
 
+def profile_gradient(name):
+    """The analytic gradient of `name` at the core-Hamiltonian guess density,
+    twice (the two-electron part alone, then the whole gradient), for
+    rocprofv3 kernel traces of the derivative kernels; prints the host-clock
+    time of each call."""
+    import patmol_amd as pm
+    from patmol_amd import systems
+
+    system = getattr(systems, name)()
+    pm.load_system(system)
+    pm.set_options(tau=TAU, precision=0, timing=False, strict=False)
+    S, T, H = pm.one_electron_ints()
+    P = core_guess_density(S, H, system.norb, system.nelec // 2)
+    store = pm.EriStore()
+    out = {"profile_gradient": name}
+    for k in range(2):
+        t0 = time.perf_counter()
+        pm.eri_gradient(P, store)
+        t1 = time.perf_counter()
+        pm.rhf_gradient(P, store=store)
+        t2 = time.perf_counter()
+        out[f"eri_gradient_s_{k}"] = t1 - t0
+        out[f"rhf_gradient_s_{k}"] = t2 - t1
+    store.close()
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -543,6 +572,8 @@ def main():
     ap.add_argument("--md-engine", type=int, default=-1, metavar="MASK",
                     help="with --profile-build: classes on the McMurchie-Davidson kernel (1 (dd|dd), 2 (dd|dp), "
                          "4 (dp|dp), 8 (dd|pp); -1 the default)")
+    ap.add_argument("--profile-gradient", default="", metavar="SYSTEM",
+                    help="the analytic gradient of SYSTEM twice (water8, water16), for rocprofv3 runs")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-molecule batch block")
     ap.add_argument("--batch-scf", action="store_true",
                     help="also converge every molecule of the batch (FP64 and mixed, about two minutes each)")
@@ -555,6 +586,8 @@ def main():
         return calibrate_copy(args.calibrate_copy)
     if args.profile_build:
         return profile_build(args.profile_build, args.profile_fp32, args.md_engine)
+    if args.profile_gradient:
+        return profile_gradient(args.profile_gradient)
 
     import torch
     import patmol_amd as pm

@@ -251,11 +251,238 @@ __global__ __launch_bounds__(256) void grad2e_kernel(GradArgs a)
     }
 }
 
+// The same sum with one wave per shell quartet, for the classes with at
+// least GRAD_WAVE_MIN Cartesian products: lane l owns the products
+// c = l + 64 k (a-major), with Gamma_c in registers (its six density values
+// read once per quartet); per primitive quartet and root, three lanes run the
+// 2-D recursion of one axis each into LDS, all lanes fill the shifted 1-D
+// table (transfer) in LDS, and each lane adds its products' derivative terms;
+// at the end of the quartet the wave sums its lanes in a fixed butterfly and
+// one lane adds the centre sums to the item's fixed-point shell sums.
+constexpr int GRAD_WAVE_MIN = 108;
+
+template <int LA, int LB, int LC, int LD>
+struct GradWave {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int NCOMP = NA * NB * NC * ND, CPL = (NCOMP + 63) / 64;
+    static constexpr int L12 = LA + LB + 1, L34 = LC + LD + 1;
+    static constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
+    static constexpr int NG = (L12 + 1) * (L34 + 1);
+    static constexpr int SA = (LB + 2) * (LC + 2) * (LD + 1), SB = (LC + 2) * (LD + 1), SC = LD + 1;
+    static constexpr int NX = (LA + 2) * SA;
+    static_assert(NX < 256, "8-bit table offsets");
+};
+
+// C(n, k) for n <= 3
+__device__ __forceinline__ double binom3(int n, int k)
+{
+    return (n == 3 && (k == 1 || k == 2)) ? 3.0 : (n == 2 && k == 1) ? 2.0 : 1.0;
+}
+
+__device__ __forceinline__ double pick3(const double (&v)[3], int i) { return i == 0 ? v[0] : i == 1 ? v[1] : v[2]; }
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void grad2e_wave_kernel(GradArgs a)
+{
+    using W = GradWave<LA, LB, LC, LD>;
+    constexpr int TI = tile_of(LA), TJ = tile_of(LB), TK = tile_of(LC), TL = tile_of(LD);
+    constexpr int NB = W::NB, NC = W::NC, ND = W::ND, NCOMP = W::NCOMP;
+    constexpr int L12 = W::L12, L34 = W::L34, NR = W::NR, NG = W::NG, NX = W::NX;
+    constexpr int SA = W::SA, SB = W::SB, SC = W::SC;
+    __shared__ FxAcc sg[(TI + TJ + TK + TL) * 3];
+    __shared__ double sG[4][3 * NG];
+    __shared__ double sT[4][3 * NX];
+    // per product: the three axes' table offsets (8 bits each) and the
+    // exponents a, b, c of each axis (2 bits each, 6 bits per axis); the
+    // normalisation product
+    __shared__ int sCo[NCOMP], sCe[NCOMP];
+    __shared__ double sNrm[NCOMP];
+    // Gamma of the current quartet's products, per wave
+    __shared__ double sGam[4][NCOMP];
+
+    const int4 it = a.items[blockIdx.x];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int fI = a.tile_first[it.x], nI = a.tile_size[it.x];
+    const int fJ = a.tile_first[it.y], nJ = a.tile_size[it.y];
+    const int fK = a.tile_first[it.z], nK = a.tile_size[it.z];
+    const int fL = a.tile_first[it.w], nL = a.tile_size[it.w];
+    const bool sameBra = it.x == it.y, sameKet = it.z == it.w, samePair = it.x == it.z && it.y == it.w;
+    const int N = a.N;
+    for (int e = tid; e < (TI + TJ + TK + TL) * 3; e += 256) sg[e] = FxAcc{0, 0};
+    for (int c = tid; c < NCOMP; c += 256) {
+        const int ia = c / (NB * NC * ND), ib = (c / (NC * ND)) % NB, ic = (c / ND) % NC, id = c % ND;
+        const int ea = cart_exps<LA>(ia), eb = cart_exps<LB>(ib), ec = cart_exps<LC>(ic), ed = cart_exps<LD>(id);
+        int co = 0, ce = 0;
+        for (int x = 0; x < 3; ++x) {
+            const int ax = (ea >> (2 * x)) & 3, bx = (eb >> (2 * x)) & 3, cx = (ec >> (2 * x)) & 3, dx = (ed >> (2 * x)) & 3;
+            co |= (ax * SA + bx * SB + cx * SC + dx) << (8 * x);
+            ce |= (ax | (bx << 2) | (cx << 4)) << (6 * x);
+        }
+        sCo[c] = co;
+        sCe[c] = ce;
+        sNrm[c] = cart_norm_rt<LA>(ia) * cart_norm_rt<LB>(ib) * cart_norm_rt<LC>(ic) * cart_norm_rt<LD>(id);
+    }
+    __syncthreads();
+    double *const G0 = sG[wave];
+    double *const T0 = sT[wave];
+
+    for (int q = wave; q < TI * TJ * TK * TL; q += 4) {
+        const int lq = q % TL, kq = (q / TL) % TK, jq = (q / (TL * TK)) % TJ, iq = q / (TL * TK * TJ);
+        if (iq >= nI || jq >= nJ || kq >= nK || lq >= nL) continue;
+        const int I = fI + iq, J = fJ + jq, K = fK + kq, L = fL + lq;
+        if (sameBra && J > I) continue;
+        if (sameKet && L > K) continue;
+        if (samePair && !(I > K || (I == K && J >= L))) continue;
+        const int pb = I * (I + 1) / 2 + J, pk = K * (K + 1) / 2 + L;
+        if (a.pair_q[pb] * a.pair_q[pk] < a.tau) continue;
+        double w = 8.0;
+        if (I == J) w *= 0.5;
+        if (K == L) w *= 0.5;
+        if (I == K && J == L) w *= 0.5;
+        const PairRef bref = a.pairs[pb], kref = a.pairs[pk];
+        double A[3], B[3], Cc[3], D[3];
+        for (int x = 0; x < 3; ++x) {
+            A[x] = a.shell_xyz[3 * I + x];
+            B[x] = a.shell_xyz[3 * J + x];
+            Cc[x] = a.shell_xyz[3 * K + x];
+            D[x] = a.shell_xyz[3 * L + x];
+        }
+        const int oI = a.shell_off[I], oJ = a.shell_off[J], oK = a.shell_off[K], oL = a.shell_off[L];
+        // Gamma of this lane's products (this wave's LDS row)
+        double *const gam = sGam[wave];
+        for (int c = lane; c < NCOMP; c += 64) {
+            const int ia = c / (NB * NC * ND), ib = (c / (NC * ND)) % NB, ic = (c / ND) % NC, id = c % ND;
+            const int ra = oI + ia, rb = oJ + ib, rc = oK + ic, rd = oL + id;
+            const double Pab = a.D[(size_t)ra * N + rb], Pcd = a.D[(size_t)rc * N + rd];
+            const double Pac = a.D[(size_t)ra * N + rc], Pbd = a.D[(size_t)rb * N + rd];
+            const double Pad = a.D[(size_t)ra * N + rd], Pbc = a.D[(size_t)rb * N + rc];
+            gam[c] = (2.0 * Pab * Pcd - 0.5 * (Pac * Pbd + Pad * Pbc)) * sNrm[c];
+        }
+        double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
+        double AB[3], CD[3];
+        for (int x = 0; x < 3; ++x) {
+            AB[x] = A[x] - B[x];
+            CD[x] = Cc[x] - D[x];
+        }
+        for (int ib = 0; ib < bref.npp; ++ib) {
+            const PrimPair bp = a.pp[bref.pp0 + ib];
+            const double2 bab = a.ppab[bref.pp0 + ib];
+            for (int ik = 0; ik < kref.npp; ++ik) {
+                const PrimPair kp = a.pp[kref.pp0 + ik];
+                const double2 kab = a.ppab[kref.pp0 + ik];
+                const double p = bp.p, qq = kp.p, pq = p + qq;
+                const double rpq = rsqrt_pos(pq), ipq = rpq * rpq;
+                const double sr0 = SRTERM * (bp.s * bp.ip) * (kp.s * kp.ip) * rpq;
+                if (sr0 < PRIM_SCREEN) continue;   // twoints.c:110-111
+                const double sr = sr0 * bp.c * kp.c * w;
+                double PQ[3], PA[3], QC[3];
+                for (int x = 0; x < 3; ++x) {
+                    PQ[x] = bp.P[x] - kp.P[x];
+                    PA[x] = bp.P[x] - A[x];
+                    QC[x] = kp.P[x] - Cc[x];
+                }
+                const double T = (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]) * p * qq * ipq;
+                const double al2 = 2.0 * bab.x, be2 = 2.0 * bab.y, ga2 = 2.0 * kab.x;
+#pragma unroll 1
+                for (int r = 0; r < NR; ++r) {
+                    double xr, wr;
+                    rys_eval_one<NR>(T, r, xr, wr, a.rys);
+                    const double fff = xr * ipq, B00 = 0.5 * fff;
+                    const double B1 = (0.5 - B00 * qq) * bp.ip, B1p = (0.5 - B00 * p) * kp.ip;
+                    if (lane < 3) {
+                        const int x = lane;
+                        const double pqx = pick3(PQ, x);
+                        double G[L12 + 1][L34 + 1];
+                        rys_vrr<double, L12, L34>(G, B00, B1, B1p, pick3(PA, x) - qq * pqx * fff, pick3(QC, x) + p * pqx * fff,
+                                                  x == 2 ? wr * sr : 1.0);
+#pragma unroll
+                        for (int i = 0; i <= L12; ++i)
+#pragma unroll
+                            for (int j = 0; j <= L34; ++j) G0[x * NG + i * (L34 + 1) + j] = G[i][j];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    // transfer: entry (axis, a, b, c, d) of the shifted 1-D table
+                    for (int e = lane; e < 3 * NX; e += 64) {
+                        const int x = e / NX, rem = e % NX;
+                        const int A_ = rem / SA, B_ = (rem / SB) % (LB + 2), C_ = (rem / SC) % (LC + 2), D_ = rem % SC;
+                        if (A_ + B_ > L12 || C_ + D_ > L34) continue;   // never read
+                        const double abx = pick3(AB, x), cdx = pick3(CD, x);
+                        const double *g = G0 + x * NG;
+                        double sum = 0.0, abp = 1.0;
+                        for (int i = 0; i <= B_; ++i) {
+                            double t = 0.0, cdp = 1.0;
+                            for (int j = 0; j <= D_; ++j) {
+                                t += binom3(D_, j) * cdp * g[(A_ + B_ - i) * (L34 + 1) + C_ + D_ - j];
+                                cdp *= cdx;
+                            }
+                            sum += binom3(B_, i) * abp * t;
+                            abp *= abx;
+                        }
+                        T0[e] = sum;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    // this lane's products
+#pragma unroll 1
+                    for (int c = lane; c < NCOMP; c += 64) {
+                        const int co = sCo[c], ce = sCe[c];
+                        const double gc = gam[c];
+                        const int o[3] = {co & 255, (co >> 8) & 255, (co >> 16) & 255};
+                        const double v[3] = {T0[o[0]], T0[NX + o[1]], T0[2 * NX + o[2]]};
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) {
+                            const double ox = gc * v[x == 0 ? 1 : 0] * v[x == 2 ? 1 : 2];
+                            const double *tx = T0 + x * NX;
+                            const int ex = (ce >> (6 * x)) & 63;
+                            const int ax = ex & 3, bx = (ex >> 2) & 3, cx = ex >> 4;
+                            const int ox_ = o[x];
+                            const double da = al2 * tx[ox_ + SA] - ax * tx[max(ox_ - SA, 0)];
+                            const double db = be2 * tx[ox_ + SB] - bx * tx[max(ox_ - SB, 0)];
+                            const double dc = ga2 * tx[ox_ + SC] - cx * tx[max(ox_ - SC, 0)];
+                            gA[x] += ox * da;
+                            gB[x] += ox * db;
+                            gC[x] += ox * dc;
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+        // the wave's sums (fixed butterfly), then lane 0 into the shell sums
+        double red[9] = {gA[0], gA[1], gA[2], gB[0], gB[1], gB[2], gC[0], gC[1], gC[2]};
+#pragma unroll
+        for (int v = 0; v < 9; ++v)
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) red[v] += __shfl_xor(red[v], m, 64);
+        if (lane == 0) {
+            for (int x = 0; x < 3; ++x) {
+                fx_lds_add(&sg[3 * iq + x], red[x]);
+                fx_lds_add(&sg[3 * (TI + jq) + x], red[3 + x]);
+                fx_lds_add(&sg[3 * (TI + TJ + kq) + x], red[6 + x]);
+                fx_lds_add(&sg[3 * (TI + TJ + TK + lq) + x], -(red[x] + red[3 + x] + red[6 + x]));
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < (TI + TJ + TK + TL) * 3; e += 256) {
+        const int slot = e / 3, x = e % 3;
+        int sh;
+        if (slot < TI) sh = slot < nI ? fI + slot : -1;
+        else if (slot < TI + TJ) sh = slot - TI < nJ ? fJ + slot - TI : -1;
+        else if (slot < TI + TJ + TK) sh = slot - TI - TJ < nK ? fK + slot - TI - TJ : -1;
+        else sh = slot - TI - TJ - TK < nL ? fL + slot - TI - TJ - TK : -1;
+        const FxAcc v = sg[e];
+        if (sh >= 0 && (v.hi != 0 || v.lo != 0)) fx_global_add(&a.gshell[3 * sh + x], v);
+    }
+}
+
 template <int LA, int LB, int LC, int LD>
 static void launch_grad_t(const GradArgs &args, hipStream_t s)
 {
     if (args.nitems <= 0) return;
-    hipLaunchKernelGGL((grad2e_kernel<LA, LB, LC, LD, double>), dim3(args.nitems), dim3(256), 0, s, args);
+    if constexpr (GradWave<LA, LB, LC, LD>::NCOMP >= GRAD_WAVE_MIN)
+        hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD>), dim3(args.nitems), dim3(256), 0, s, args);
+    else
+        hipLaunchKernelGGL((grad2e_kernel<LA, LB, LC, LD, double>), dim3(args.nitems), dim3(256), 0, s, args);
 }
 
 int launch_grad(int bra_pc, int ket_pc, const GradArgs &args, hipStream_t s)

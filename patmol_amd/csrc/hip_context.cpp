@@ -987,13 +987,17 @@ static int run_jk(Ctx &c, const double *DJ, const double *DK, FxAcc *J, FxAcc *K
         a.slot_stride = (long long)c.N * c.N;
         c.md.fill(a, md_effective(c.md_off) & md_class_bit(r.bra_pc, r.ket_pc));
         if (timed && ci < 64) HIPCHK(hipEventRecord(c.cls_ev[2 * ci], s));
+        // a class whose items below the threshold stay FP64 runs both lists
+        // in one launch where they are adjacent (every build but a delta
+        // build, which drops items between them)
+        const int p32 = fp32_prec(r.bra_pc, r.ket_pc);
+        if (p32 == 0 && nkeep > n64 && list32 == list64 + n64) n64 = nkeep;
         if (n64 > 0) {
             a.items = list64;
             a.nitems = n64;
             if (launch_jk(r.bra_pc, r.ket_pc, a, 0, ls)) return fail(PATMOL_HIP_EUNSUPPORTED, "no J/K kernel for class %d/%d", r.bra_pc, r.ket_pc);
         }
         if (nkeep > n64) {
-            const int p32 = fp32_prec(r.bra_pc, r.ket_pc);
             a.items = list32;
             a.nitems = nkeep - n64;
             if (p32) c.items_fp32 += nkeep - n64;
@@ -1349,7 +1353,8 @@ static int batch_build(PatmolHipBatch &b, hipStream_t s)
         a.mols = b.d_mols.p;
         a.slot_stride = dtot;
         b.md.fill(a, md_effective(b.md_off) & md_class_bit(r.bra_pc, r.ket_pc));
-        const int n64 = b.n64[ri];
+        // a class whose items below the threshold stay FP64: one launch
+        const int n64 = fp32_prec(r.bra_pc, r.ket_pc) ? b.n64[ri] : r.count;
         if (n64 > 0) {
             a.items = b.d_items.p + r.first;
             a.nitems = n64;

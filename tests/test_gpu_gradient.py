@@ -62,6 +62,47 @@ def test_eri_gradient_vs_oracle_finite_differences():
     assert np.abs(g.sum(axis=0)).max() < 1e-9
 
 
+def _contracted_water():
+    """Water with contracted shells on O (s of 3, p of 2, two d of 2
+    primitives each) and on H (s of 2): the primitive loops of both
+    derivative kernels, (dd|dd) through (ss|ss)."""
+    from patmol_amd.systems import Shell
+    s = systems.water()
+    sh = [Shell(0, 0, [5.2, 1.1, 0.3], [0.25, 0.55, 0.4]), Shell(0, 0, [0.12], [1.0]),
+          Shell(1, 0, [2.4, 0.5], [0.45, 0.7]), Shell(1, 0, [0.15], [1.0]),
+          Shell(2, 0, [1.6, 0.45], [0.5, 0.6]), Shell(2, 0, [0.25, 0.9], [0.7, 0.3])]
+    for c in (1, 2):
+        sh += [Shell(0, c, [1.9, 0.4], [0.35, 0.75]), Shell(0, c, [0.11], [1.0]), Shell(1, c, [0.8], [1.0])]
+    return dataclasses.replace(s, shells=sh)
+
+
+def test_eri_gradient_contracted_vs_oracle_finite_differences():
+    """The two-electron gradient with contracted s, p and d shells (both
+    derivative kernels' primitive loops) against central differences of
+    sum w P G(P) from the oracle's records, 1e-7."""
+    s = _contracted_water()
+    n = s.norb
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((n, n)) * 0.2
+    P = (A @ A.T / n)[np.tril_indices(n)].copy()
+    pm.load_system(s)
+    pm.set_options(tau=0.0, precision=0, strict=False)
+    st = pm.EriStore()
+    g = pm.eri_gradient(P, st)
+    st.close()
+    w = _w(n)
+    fd = np.zeros_like(g)
+    for c in range(s.ncen):
+        for x in range(3):
+            e = [float(np.sum(w * P * oracle.Basis(_moved(s, c, x, sg * H_STEP)).two_electron_ints().rhf_gmatrix(P)))
+                 for sg in (1, -1)]
+            fd[c, x] = (e[0] - e[1]) / (2 * H_STEP)
+    err = float(np.abs(g - fd).max())
+    note(f"contracted water ({n} functions): ERI gradient vs oracle differences {err:.3e} (|g| max {np.abs(g).max():.3f})")
+    assert err < 1e-7
+    assert np.abs(g.sum(axis=0)).max() < 1e-9
+
+
 def _device_energy(s, guess=None, precision=0, tau=1e-12):
     pm.load_system(s)
     pm.set_options(tau=tau, precision=precision, strict=False, scf_density=1)

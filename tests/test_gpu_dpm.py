@@ -172,9 +172,10 @@ def test_dpm_ring_mirror_symmetry():
 def test_dpm_ring_vs_oracle_fixture(golden_dir, mode):
     """The planar C6H6-shaped ring (300 electron functions): ground state,
     E_first, E_conv and vpol at the ring centre and 2 bohr above and below it
-    against the oracle's loop, within 1e-8 Eh in FP64 and 1e-6 Eh in mixed
-    precision (items with Schwarz bound x max|P| < 1e-4 in FP32, the RHF and
-    the UHF SCF), iteration counts within 3 of the oracle's (the mixed SCF
+    against the oracle's loop, within 1e-8 Eh in FP64 (E_first and vpol
+    1e-7, see below) and 1e-6 Eh in mixed precision (items with Schwarz
+    bound x max|P| < 1e-4 in FP32, the RHF and the UHF SCF), iteration
+    counts within 3 of the oracle's (the mixed SCF
     builds its last iterations in FP64, without which the FP32 rounding kept
     the density change above the stop bound for up to 200 iterations)."""
     gold = json.load(open(os.path.join(golden_dir, "dpm_ring.json")))
@@ -188,6 +189,12 @@ def test_dpm_ring_vs_oracle_fixture(golden_dir, mode):
     finally:
         pm.set_options(precision=0)
     tol = 1e-6 if mixed else E_TOL
+    # E_first is the energy of the ground-state density in the augmented
+    # basis: first order in that density's residual error, which the stop
+    # rule (density change < 2e-10 here) leaves at a level that moves E_first
+    # by up to ~5e-8 Eh between two correct SCFs (measured: 2e-9 at tau 0,
+    # 4.6e-8 at tau 1e-12); vpol = E_conv - E_first inherits it
+    tol_first = max(tol, 1e-7)
     ogs = gold["ground_state"]
     note(f"ring {mode}: ground state gpu {gs['energy']:.12f} ({gs['iterations']} it), oracle "
          f"{ogs['energy']:.12f} ({ogs['iterations']} it)")
@@ -197,6 +204,7 @@ def test_dpm_ring_vs_oracle_fixture(golden_dir, mode):
         note(f"  r={p}: vpol gpu {g['vpol']:.10f} oracle {r['vpol']:.10f}; E {g['e_conv']:.12f} / "
              f"{r['e_conv']:.12f}; it {g['iterations']} / {r['iterations']}")
         assert g["converged"] == 1 and r["converged"] == 1
-        for k in ("e_first", "e_conv", "vpol"):
-            assert abs(g[k] - r[k]) < tol, k
+        assert abs(g["e_conv"] - r["e_conv"]) < tol
+        for k in ("e_first", "vpol"):
+            assert abs(g[k] - r[k]) < tol_first, k
         assert abs(g["iterations"] - r["iterations"]) <= 3

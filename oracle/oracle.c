@@ -25,8 +25,10 @@
  *                                                         s-type closed forms
  *   or_dpm_ints         dpmints.c:18-199                  reference dpm_ints (_ref); s-type closed form
  *   or_rhf_scf          rhfscf.c:21-174,432-471           SF6 golden: E, 43 iterations, E_nuc
- *   or_uhf_scf          uhfscf.c:22-168,485-527           closed-shell UHF == RHF (parity unpinned
- *                                                         for open shells: no reference answer)
+ *   or_uhf_scf          uhfscf.c:22-168,485-527           closed-shell UHF == RHF; open shells (H2O+
+ *                                                         doublet): aufbau fixed point of the
+ *                                                         textbook Pople-Nesbet Fock matrices and
+ *                                                         the UHF energy functional (numpy einsum)
  *   or_check_phase      spack_math.c:274-297              reference spack_diag eigenvectors (_ref),
  *                                                         incl. a dominant negative first component
  *   or_diag_fock        rhfscf.c:432-471, scfaux.c:5-41   reference spack_trans/spack_diag/

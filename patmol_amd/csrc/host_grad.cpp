@@ -289,14 +289,14 @@ static int rhf_gradient(const ERIFS *mints, const double *P, double *grad, bool 
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             double s = 0.0;
-            for (int k = 0; k < n; ++k) s += Pf[(size_t)i * n + k] * Ff[(size_t)k * n + j];
+            for (int k = 0; k < n; ++k) s += Pf[(size_t)i * n + k] * Ff[(size_t)j * n + k];   // F symmetric
             PF[(size_t)i * n + j] = s;
         }
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j <= i; ++j) {
             double s = 0.0;
-            for (int k = 0; k < n; ++k) s += PF[(size_t)i * n + k] * Pf[(size_t)k * n + j];
+            for (int k = 0; k < n; ++k) s += PF[(size_t)i * n + k] * Pf[(size_t)j * n + k];   // P symmetric
             W[(size_t)i * (i + 1) / 2 + j] = s;
         }
     for (int k = 0; k < 3 * nc; ++k) grad[k] = 0.0;

@@ -198,7 +198,8 @@ def _measure_one(args):
 
     s = getattr(systems, args.system)()
     pm.load_system(s)
-    pm.set_options(tau=1e-12, precision=0, strict=False, timing=True)
+    # every class on its Rys kernel: the shapes tuned here are jk_kernel3's
+    pm.set_options(tau=1e-12, precision=0, strict=False, timing=True, md_engine=0)
     S, _, H = pm.one_electron_ints()
     n, nocc = s.norb, s.nelec // 2
     il = np.tril_indices(n)

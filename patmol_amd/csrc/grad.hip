@@ -251,15 +251,28 @@ __global__ __launch_bounds__(256) void grad2e_kernel(GradArgs a)
     }
 }
 
-// The same sum with one wave per shell quartet, for the classes with at
-// least GRAD_WAVE_MIN Cartesian products: lane l owns the products
-// c = l + 64 k (a-major), with Gamma_c in registers (its six density values
-// read once per quartet); per primitive quartet and root, three lanes run the
-// 2-D recursion of one axis each into LDS, all lanes fill the shifted 1-D
-// table (transfer) in LDS, and each lane adds its products' derivative terms;
-// at the end of the quartet the wave sums its lanes in a fixed butterfly and
-// one lane adds the centre sums to the item's fixed-point shell sums.
-constexpr int GRAD_WAVE_MIN = 108;
+// The same sum with one group of GL lanes (64 or 32: one or two groups per
+// wave) per shell quartet, for the classes with at least
+// GRAD_GROUP_MIN Cartesian products (below it the per-thread kernel):
+// lane g of a group owns the products c = g + GL k (a-major), whose Gamma_c
+// it writes into the group's LDS row once per quartet (six density values
+// each); per primitive quartet and root, three lanes run the 2-D recursion
+// of one axis each into LDS, the group's lanes fill the shifted 1-D table
+// (transfer) in LDS, and each lane adds its products' derivative terms; at
+// the end of the quartet the group sums its lanes in a fixed butterfly and
+// its first lane adds the centre sums to the item's fixed-point shell sums.
+constexpr int GRAD_GROUP_MIN = 108;
+// lanes per group, measured per class at (H2O)16: 32 (two quartets per
+// wave) for (dp|ds) (102 -> 94 ms) and (dp|pp) (43 -> 38 ms), a whole wave
+// for the others ((dd|ps) 59 ms with 64 lanes, 65 with 32); 16-lane groups
+// for the classes of 27-81 products measured slower than the per-thread
+// kernel ((ds|ds) 71 against 16 ms, (dd|ss) 35, (pp|ps) 49; (ds|pp) 62
+// against 70)
+template <int LA, int LB, int LC, int LD>
+constexpr int grad_group_lanes()
+{
+    return (LA == 2 && LB == 1 && ((LC == 2 && LD == 0) || (LC == 1 && LD == 1))) ? 32 : 64;
+}
 
 template <int LA, int LB, int LC, int LD>
 struct GradWave {
@@ -287,21 +300,22 @@ __global__ __launch_bounds__(256) void grad2e_wave_kernel(GradArgs a)
     using W = GradWave<LA, LB, LC, LD>;
     constexpr int TI = tile_of(LA), TJ = tile_of(LB), TK = tile_of(LC), TL = tile_of(LD);
     constexpr int NB = W::NB, NC = W::NC, ND = W::ND, NCOMP = W::NCOMP;
+    constexpr int GL = grad_group_lanes<LA, LB, LC, LD>(), NGRP = 4 * (64 / GL);
     constexpr int L12 = W::L12, L34 = W::L34, NR = W::NR, NG = W::NG, NX = W::NX;
     constexpr int SA = W::SA, SB = W::SB, SC = W::SC;
     __shared__ FxAcc sg[(TI + TJ + TK + TL) * 3];
-    __shared__ double sG[4][3 * NG];
-    __shared__ double sT[4][3 * NX];
+    __shared__ double sG[NGRP][3 * NG];
+    __shared__ double sT[NGRP][3 * NX];
     // per product: the three axes' table offsets (8 bits each) and the
     // exponents a, b, c of each axis (2 bits each, 6 bits per axis); the
     // normalisation product
     __shared__ int sCo[NCOMP], sCe[NCOMP];
     __shared__ double sNrm[NCOMP];
-    // Gamma of the current quartet's products, per wave
-    __shared__ double sGam[4][NCOMP];
+    // Gamma of the current quartet's products, per group
+    __shared__ double sGam[NGRP][NCOMP];
 
     const int4 it = a.items[blockIdx.x];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, grp = tid / GL, lane = tid % GL;
     const int fI = a.tile_first[it.x], nI = a.tile_size[it.x];
     const int fJ = a.tile_first[it.y], nJ = a.tile_size[it.y];
     const int fK = a.tile_first[it.z], nK = a.tile_size[it.z];
@@ -323,10 +337,10 @@ __global__ __launch_bounds__(256) void grad2e_wave_kernel(GradArgs a)
         sNrm[c] = cart_norm_rt<LA>(ia) * cart_norm_rt<LB>(ib) * cart_norm_rt<LC>(ic) * cart_norm_rt<LD>(id);
     }
     __syncthreads();
-    double *const G0 = sG[wave];
-    double *const T0 = sT[wave];
+    double *const G0 = sG[grp];
+    double *const T0 = sT[grp];
 
-    for (int q = wave; q < TI * TJ * TK * TL; q += 4) {
+    for (int q = grp; q < TI * TJ * TK * TL; q += NGRP) {
         const int lq = q % TL, kq = (q / TL) % TK, jq = (q / (TL * TK)) % TJ, iq = q / (TL * TK * TJ);
         if (iq >= nI || jq >= nJ || kq >= nK || lq >= nL) continue;
         const int I = fI + iq, J = fJ + jq, K = fK + kq, L = fL + lq;
@@ -348,9 +362,9 @@ __global__ __launch_bounds__(256) void grad2e_wave_kernel(GradArgs a)
             D[x] = a.shell_xyz[3 * L + x];
         }
         const int oI = a.shell_off[I], oJ = a.shell_off[J], oK = a.shell_off[K], oL = a.shell_off[L];
-        // Gamma of this lane's products (this wave's LDS row)
-        double *const gam = sGam[wave];
-        for (int c = lane; c < NCOMP; c += 64) {
+        // Gamma of this lane's products (this group's LDS row)
+        double *const gam = sGam[grp];
+        for (int c = lane; c < NCOMP; c += GL) {
             const int ia = c / (NB * NC * ND), ib = (c / (NC * ND)) % NB, ic = (c / ND) % NC, id = c % ND;
             const int ra = oI + ia, rb = oJ + ib, rc = oK + ic, rd = oL + id;
             const double Pab = a.D[(size_t)ra * N + rb], Pcd = a.D[(size_t)rc * N + rd];
@@ -402,7 +416,7 @@ __global__ __launch_bounds__(256) void grad2e_wave_kernel(GradArgs a)
                     }
                     __builtin_amdgcn_wave_barrier();
                     // transfer: entry (axis, a, b, c, d) of the shifted 1-D table
-                    for (int e = lane; e < 3 * NX; e += 64) {
+                    for (int e = lane; e < 3 * NX; e += GL) {
                         const int x = e / NX, rem = e % NX;
                         const int A_ = rem / SA, B_ = (rem / SB) % (LB + 2), C_ = (rem / SC) % (LC + 2), D_ = rem % SC;
                         if (A_ + B_ > L12 || C_ + D_ > L34) continue;   // never read
@@ -423,7 +437,7 @@ __global__ __launch_bounds__(256) void grad2e_wave_kernel(GradArgs a)
                     __builtin_amdgcn_wave_barrier();
                     // this lane's products
 #pragma unroll 1
-                    for (int c = lane; c < NCOMP; c += 64) {
+                    for (int c = lane; c < NCOMP; c += GL) {
                         const int co = sCo[c], ce = sCe[c];
                         const double gc = gam[c];
                         const int o[3] = {co & 255, (co >> 8) & 255, (co >> 16) & 255};
@@ -447,12 +461,13 @@ __global__ __launch_bounds__(256) void grad2e_wave_kernel(GradArgs a)
                 }
             }
         }
-        // the wave's sums (fixed butterfly), then lane 0 into the shell sums
+        // the group's sums (fixed butterfly inside the group), then its
+        // first lane into the shell sums
         double red[9] = {gA[0], gA[1], gA[2], gB[0], gB[1], gB[2], gC[0], gC[1], gC[2]};
 #pragma unroll
         for (int v = 0; v < 9; ++v)
 #pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) red[v] += __shfl_xor(red[v], m, 64);
+            for (int m = GL / 2; m >= 1; m >>= 1) red[v] += __shfl_xor(red[v], m, 64);
         if (lane == 0) {
             for (int x = 0; x < 3; ++x) {
                 fx_lds_add(&sg[3 * iq + x], red[x]);
@@ -479,7 +494,7 @@ template <int LA, int LB, int LC, int LD>
 static void launch_grad_t(const GradArgs &args, hipStream_t s)
 {
     if (args.nitems <= 0) return;
-    if constexpr (GradWave<LA, LB, LC, LD>::NCOMP >= GRAD_WAVE_MIN)
+    if constexpr (GradWave<LA, LB, LC, LD>::NCOMP >= GRAD_GROUP_MIN)
         hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD>), dim3(args.nitems), dim3(256), 0, s, args);
     else
         hipLaunchKernelGGL((grad2e_kernel<LA, LB, LC, LD, double>), dim3(args.nitems), dim3(256), 0, s, args);

@@ -127,19 +127,25 @@ def test_md_water8_vs_rys_screened():
 
 def test_engine_mask_kept_share_rule():
     """(dd|dp) leaves the MD kernel by default where most of its work items'
-    shell-quartet slots are kept (the compact water dimer), and stays on it
-    when bit 4 is set; (dd|dd) stays in both cases."""
-    s = systems.water_cluster(2, 1, 1, seed=1001)
-    pm.load_system(s)
-    pm.set_options(tau=1e-12, precision=0, strict=False, md_engine=-1)
-    st = pm.EriStore()
+    shell-quartet slots are kept (the water dimer at 2.9 A: 0.86 of them,
+    counted on the oracle's work-list mirror), and stays on it when bit 4 is
+    set, or when the screen empties most slots (the dimer at 8 A: 0.32);
+    (dd|dd) stays in every case."""
+    masks = {}
     try:
-        m_default = st.engine_mask()
-        pm.set_options(md_engine=3 | 16)
-        m_forced = st.engine_mask()
+        for spacing in (2.9, 8.0):
+            s = systems.water_cluster(2, 1, 1, seed=1001, spacing=spacing)
+            pm.load_system(s)
+            pm.set_options(tau=1e-12, precision=0, strict=False, md_engine=-1)
+            st = pm.EriStore()
+            masks[spacing] = st.engine_mask()
+            if spacing == 2.9:
+                pm.set_options(md_engine=3 | 16)
+                masks["forced"] = st.engine_mask()
+                pm.set_options(md_engine=-1)
+            st.close()
     finally:
         pm.set_options(md_engine=-1)
-    st.close()
-    note(f"engine mask: default {m_default}, forced {m_forced}")
-    assert m_default == 1 and m_forced == 3
+    note(f"engine masks: {masks}")
+    assert masks[2.9] == 1 and masks["forced"] == 3 and masks[8.0] == 3
 

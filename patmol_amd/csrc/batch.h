@@ -24,6 +24,10 @@ bool batch_view(PatmolHipBatch *b, BatchView &v);
 // G = 2J[P] - K[P] for every molecule, P and G the batch's own buffers:
 // one replay of the captured graph on `s`
 int batch_launch(PatmolHipBatch *b, hipStream_t s);
+// which molecules the following replays build (host array, one flag per
+// molecule; the rest keep their G of the last replay that built them);
+// all 1 after finalisation
+int batch_set_active(PatmolHipBatch *b, const int *active, hipStream_t s);
 
 // Work-item counts of one G build
 struct BuildCounts {

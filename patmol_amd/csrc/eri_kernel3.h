@@ -194,6 +194,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     constexpr int KS = jk3_wg_per_item<LA, LB, LC, LD, R>();
     const int item = (int)(blockIdx.x / (unsigned)KS), ksub = (int)(blockIdx.x % (unsigned)KS);
     if (item >= a.nitems) return;
+    if (a.mol_active && !a.mol_active[a.item_mol[item]]) return;   // whole workgroup
     const int4 it = a.items[item];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;

@@ -100,6 +100,9 @@ struct JKArgs {
     // batches (null item_mol: one molecule, N x N blocks at offset 0)
     const int *item_mol;       // molecule of each item
     const MolDesc *mols;
+    // batches: nonzero for the molecules whose items run (null: all); the
+    // batched SCF stops building the molecules that have converged
+    const int *mol_active = nullptr;
     long long slot_stride;     // elements of one per-XCD copy of J' (and of K')
     // McMurchie-Davidson tables (md_kernel.h): slot of each primitive pair
     // and the bra / ket Hermite-coefficient tables per pair class; with

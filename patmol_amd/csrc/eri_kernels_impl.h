@@ -165,6 +165,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
 
     const int item = blockIdx.x;
     if (item >= a.nitems) return;
+    if (a.mol_active && !a.mol_active[a.item_mol[item]]) return;   // whole workgroup
     const int4 it = a.items[item];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ll = lane & (TL - 1), kk = (lane >> LBITS) & (TK - 1), js = lane >> (LBITS + KBITS);

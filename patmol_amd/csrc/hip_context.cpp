@@ -1306,6 +1306,7 @@ struct PatmolHipBatch {
     pm::DevBuf<pm::PrimPair> d_pp;
     pm::DevBuf<int4> d_items;
     pm::DevBuf<pm::MolDesc> d_mols;
+    pm::DevBuf<int> d_active;   // per molecule: its items run (all 1 outside the batched SCF)
     pm::DevBuf<pm::FxAcc> d_J, d_K;
     std::vector<int> shell_l;   // l of each (batch-global) sorted shell
     pm::MdTables md;            // McMurchie-Davidson tables of every molecule's pairs
@@ -1351,6 +1352,7 @@ static int batch_build(PatmolHipBatch &b, hipStream_t s)
         a.rys = g_rys;
         a.item_mol = b.d_item_mol.p + r.first;
         a.mols = b.d_mols.p;
+        a.mol_active = b.d_active.p;
         a.slot_stride = dtot;
         b.md.fill(a, md_effective(b.md_off) & md_class_bit(r.bra_pc, r.ket_pc));
         // a class whose items below the threshold stay FP64: one launch
@@ -1385,6 +1387,14 @@ bool batch_view(PatmolHipBatch *b, BatchView &v)
     v.d_P = b->d_P.p;
     v.d_G = b->d_G.p;
     return true;
+}
+
+int batch_set_active(PatmolHipBatch *b, const int *active, hipStream_t s)
+{
+    if (!b || !b->finalized) return fail(PATMOL_HIP_ESTATE, "batch not finalised");
+    HIPCHK(hipMemcpyAsync(b->d_active.p, active, sizeof(int) * b->mol.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
 }
 
 int batch_launch(PatmolHipBatch *b, hipStream_t s)
@@ -1756,7 +1766,7 @@ void patmol_hip_batch_free(PatmolHipBatch *b)
     if (b->s) (void)hipStreamDestroy(b->s);
     for (auto *x : {&b->d_xyz, &b->d_q, &b->d_P, &b->d_G, &b->d_D}) x->release();
     for (auto *x : {&b->d_soff, &b->d_orig, &b->d_tile_first, &b->d_tile_size, &b->d_fperm, &b->d_finv, &b->d_item_mol, &b->d_N,
-                    &b->d_fbase})
+                    &b->d_fbase, &b->d_active})
         x->release();
     b->d_dstart.release();
     b->d_pstart.release();
@@ -1880,7 +1890,8 @@ int patmol_hip_batch_finalize(PatmolHipBatch *b, int prec_mode, double threshold
         b->d_tile_first.upload(b->tile_first) || b->d_tile_size.upload(b->tile_size) || b->d_fperm.upload(b->fperm) ||
         b->d_finv.upload(b->finv) || b->d_item_mol.upload(imol) || b->d_N.upload(Nv) || b->d_fbase.upload(fb) ||
         b->d_dstart.upload(ds) || b->d_pstart.upload(ps) || b->d_pairs.upload(b->pairs) || b->d_pp.upload(b->pp) ||
-        b->d_items.upload(items) || b->d_mols.upload(md) || b->d_P.alloc(b->no2tot) || b->d_G.alloc(b->no2tot) ||
+        b->d_items.upload(items) || b->d_mols.upload(md) || b->d_active.upload(std::vector<int>(nm, 1)) ||
+        b->d_P.alloc(b->no2tot) || b->d_G.alloc(b->no2tot) ||
         b->d_D.alloc(b->dtot) || b->d_J.alloc((size_t)b->dtot * PM_XCD_SLOTS) ||
         b->d_K.alloc((size_t)b->dtot * PM_XCD_SLOTS))
         return g_err;

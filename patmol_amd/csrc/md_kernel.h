@@ -207,6 +207,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
 
     const int item = blockIdx.x;
     if (item >= a.nitems) return;
+    if (a.mol_active && !a.mol_active[a.item_mol[item]]) return;   // whole workgroup
     const int4 it = a.items[item];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int fI = a.tile_first[it.x], nI = a.tile_size[it.x];

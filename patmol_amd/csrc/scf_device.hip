@@ -1591,14 +1591,35 @@ extern "C" int patmol_hip_batch_rhf_scf(PatmolHipBatch *b, const double *S, cons
     }
     auto t0 = std::chrono::steady_clock::now();
     double gms = 0.0;
+    // the molecules the next replay builds: those still iterating (a
+    // converged molecule's G is not read again)
+    std::vector<int> active(nm, 1), built(nm, 1);
+    auto restore_all = [&]() {
+        std::fill(active.begin(), active.end(), 1);
+        return batch_set_active(b, active.data(), st);
+    };
     for (int sweep = 0;; ++sweep) {
         bool any = false;
-        for (int m = 0; m < nm; ++m) any = any || (!mols[m].done && mols[m].iteration < maxits);
+        for (int m = 0; m < nm; ++m) {
+            active[m] = (!mols[m].done && mols[m].iteration < maxits) ? 1 : 0;
+            any = any || active[m];
+        }
         if (!any) break;
+        if (active != built) {
+            if (batch_set_active(b, active.data(), st)) {
+                const int e = patmol_hip_last_error();
+                (void)restore_all();
+                cleanup();
+                return e;
+            }
+            built = active;
+        }
         (void)hipEventRecord(g0, st);
         if (batch_launch(b, st)) {
+            const int e = patmol_hip_last_error();
+            (void)restore_all();
             cleanup();
-            return patmol_hip_last_error();
+            return e;
         }
         (void)hipEventRecord(g1, st);
         for (int m = 0; m < nm; ++m) {
@@ -1641,6 +1662,10 @@ extern "C" int patmol_hip_batch_rhf_scf(PatmolHipBatch *b, const double *S, cons
         if (hipEventSynchronize(g1) == hipSuccess && hipEventElapsedTime(&ms, g0, g1) == hipSuccess) gms += ms;
     }
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_batch_rhf_scf: device failure");
+    // later G builds of the batch cover every molecule again
+    if (active != std::vector<int>(nm, 1) || built != std::vector<int>(nm, 1)) {
+        if (restore_all() && !rc) rc = patmol_hip_last_error();
+    }
     if (!rc) {
         for (int m = 0; m < nm; ++m) {
             energy[m] = mols[m].enrg + enuc[m];

@@ -251,9 +251,10 @@ __global__ __launch_bounds__(256) void grad2e_kernel(GradArgs a)
     }
 }
 
-// The same sum with one group of GL lanes (64 or 32: one or two groups per
-// wave) per shell quartet, for the classes with at least
-// GRAD_GROUP_MIN Cartesian products (below it the per-thread kernel):
+// The same sum with one group of GL lanes (64, 32 or 16: one, two or four
+// groups per wave) per shell quartet, for the classes with at least
+// GRAD_GROUP_MIN Cartesian products and (ds|pp) (the others run the
+// per-thread kernel):
 // lane g of a group owns the products c = g + GL k (a-major), whose Gamma_c
 // it writes into the group's LDS row once per quartet (six density values
 // each); per primitive quartet and root, three lanes run the 2-D recursion
@@ -266,12 +267,19 @@ constexpr int GRAD_GROUP_MIN = 108;
 // wave) for (dp|ds) (102 -> 94 ms) and (dp|pp) (43 -> 38 ms), a whole wave
 // for the others ((dd|ps) 59 ms with 64 lanes, 65 with 32); 16-lane groups
 // for the classes of 27-81 products measured slower than the per-thread
-// kernel ((ds|ds) 71 against 16 ms, (dd|ss) 35, (pp|ps) 49; (ds|pp) 62
-// against 70)
+// kernel ((ds|ds) 71 against 16 ms, (dd|ss) 35, (pp|ps) 49) except for
+// (ds|pp)
 template <int LA, int LB, int LC, int LD>
 constexpr int grad_group_lanes()
 {
+    if (LA == 2 && LB == 0 && LC == 1 && LD == 1) return 16;   // (ds|pp): 62 against 70 ms per-thread
     return (LA == 2 && LB == 1 && ((LC == 2 && LD == 0) || (LC == 1 && LD == 1))) ? 32 : 64;
+}
+// the classes that run the group kernel
+template <int LA, int LB, int LC, int LD>
+constexpr bool grad_grouped()
+{
+    return ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD) >= GRAD_GROUP_MIN || (LA == 2 && LB == 0 && LC == 1 && LD == 1);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -494,7 +502,7 @@ template <int LA, int LB, int LC, int LD>
 static void launch_grad_t(const GradArgs &args, hipStream_t s)
 {
     if (args.nitems <= 0) return;
-    if constexpr (GradWave<LA, LB, LC, LD>::NCOMP >= GRAD_GROUP_MIN)
+    if constexpr (grad_grouped<LA, LB, LC, LD>())
         hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD>), dim3(args.nitems), dim3(256), 0, s, args);
     else
         hipLaunchKernelGGL((grad2e_kernel<LA, LB, LC, LD, double>), dim3(args.nitems), dim3(256), 0, s, args);

@@ -6,8 +6,9 @@ build captured as a HIP graph.  Checked: each molecule's G against its own
 single-molecule build (bitwise: the J/K sums are fixed point) and against
 the oracle's scatter; graph replays, with the program's own work between
 them, against the eager launches; the batched SCF's per-molecule energies
-against the oracle's SCF (1e-8 Eh); the mixed-precision batch against FP64
-(|dE| <= 1e-6 Eh).
+against the oracle's SCF (1e-8 Eh), with the molecules leaving the replays
+as they converge and every molecule built again afterwards; the
+mixed-precision batch against FP64 (|dE| <= 1e-6 Eh).
 """
 import numpy as np
 import pytest
@@ -65,10 +66,15 @@ def test_batch_rhf_scf_vs_oracle_and_mixed():
     for m in mols:
         b.add(m)
     b.finalize()
+    Pr = _densities(mols, 300)
+    G_before = b.gmatrix(Pr, use_graph=True)
     e, it, cv, P, tot, gms = b.rhf_scf(accel=3)
+    # the SCF drops converged molecules from its replays and restores the
+    # full build when it returns
+    assert np.array_equal(b.gmatrix(Pr, use_graph=True), G_before)
     b.close()
     note(f"batched SCF of {len(mols)} molecules: {tot:.0f} ms ({gms:.0f} ms in G graphs); iterations {list(it)}")
-    assert cv.all()
+    assert cv.all() and len(set(int(x) for x in it)) > 1
     for i, m in enumerate(mols):
         ob = oracle.Basis(m)
         ref, _, _, _ = oracle.rhf_scf_ex(ob, ob.two_electron_ints(), m.nelec, maxits=300, eps=1e-10, scf_a=2)

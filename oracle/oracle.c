@@ -790,7 +790,7 @@ void or_uhf_form_gmatrix(const OERINT *ints, int64_t n, const double *PA, const 
  * after that many shell quartets (a bounded timing sample). */
 /* Schwarz bound Q_IJ = sqrt(max |(ab|ab)|) of every shell pair, without the
  * primitive screen, as a full ns x ns matrix (no reference counterpart: the
- * reference does not screen, K4 of its survey; used to time the CPU build on
+ * reference does not screen by Schwarz bounds; used to time the CPU build on
  * the same screened quartet set as the device build). */
 void or_pair_schwarz(const OBasis *bs, double *q, int nthreads)
 {

@@ -147,6 +147,43 @@ def _even(a0: float, ratio: float, n: int) -> List[float]:
     return [a0 * ratio ** k for k in range(n)]
 
 
+def read_posgrid(path) -> np.ndarray:
+    """Positron positions (bohr) from a posgrid.dat as the reference's DPM
+    loop reads it (rhfscf.c:230-235): the point count, then x y z per
+    point, whitespace separated.  Returns an (npts, 3) array."""
+    with open(path) as f:
+        tok = f.read().split()
+    n = int(tok[0])
+    v = np.array([float(x) for x in tok[1:1 + 3 * n]], dtype=np.float64)
+    if v.size != 3 * n:
+        raise ValueError(f"{path}: {n} points announced, {v.size // 3} found")
+    return v.reshape(n, 3)
+
+
+def write_posgrid(points, path) -> None:
+    """The inverse of read_posgrid."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    with open(path, "w") as f:
+        f.write(f"{len(pts)}\n")
+        for x, y, z in pts:
+            f.write(f"{x:.10f} {y:.10f} {z:.10f}\n")
+
+
+def write_dpmvpol(path, points, results, ground_energy: float) -> None:
+    """One line per point in the reference's dpmvpol.dat layout
+    (rhfscf.c:257-258: "%15.6le %15.6le %15.6le %20.10le %15.6le %20.10le":
+    x y z vpol alfa vstat).  results: the per-point dicts of
+    patmol_amd.dpm_calculation.  The reference's vstat reads a variable it
+    never sets (rhfscf.c:254); written here is E_first - E_ground, the
+    energy of the ground-state density in the field of the positron less the
+    ground-state energy."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    with open(path, "w") as f:
+        for (x, y, z), r in zip(pts, results):
+            f.write("%15.6e %15.6e %15.6e %20.10e %15.6e %20.10e\n"
+                    % (x, y, z, r["vpol"], r["alfa"], r["e_first"] - ground_energy))
+
+
 def heavy_atom_shells(cen: int, zscale: float = 1.0, d: bool = True) -> List[Shell]:
     """O-type recipe: s 0.1*3^k (k<8), p 0.15*3^k (k<6), d 0.3*3^k (k<3),
     all exponents multiplied by zscale ((Z/8)^2 for C and N)."""

@@ -2,6 +2,7 @@
 import os
 
 import numpy as np
+import pytest
 
 from patmol_amd import systems
 
@@ -54,3 +55,28 @@ def test_patin_round_trip(tmp_path, golden_dir):
     assert np.array_equal(s.coords, t.coords) and np.array_equal(s.charges, t.charges)
     assert [(a.l, a.cen, a.al, a.co) for a in s.shells] == [(a.l, a.cen, a.al, a.co) for a in t.shells]
     assert (t.maxits, t.scf_eps, t.scf_a) == (s.maxits, s.scf_eps, s.scf_a)
+
+
+def test_posgrid_and_dpmvpol_files(tmp_path):
+    """posgrid.dat as rhfscf.c:230-235 reads it (count, then x y z per point,
+    any whitespace) and dpmvpol.dat in the layout of rhfscf.c:257-258."""
+    p = tmp_path / "posgrid.dat"
+    p.write_text("3\n0 0 0\n0.0 0.0\n2.0   0.5 -0.4 2.5\n")
+    pts = systems.read_posgrid(p)
+    assert pts.shape == (3, 3) and np.array_equal(pts[2], [0.5, -0.4, 2.5]) and pts[1, 2] == 2.0
+    q = tmp_path / "again.dat"
+    systems.write_posgrid(pts, q)
+    assert np.array_equal(systems.read_posgrid(q), pts)
+    bad = tmp_path / "short.dat"
+    bad.write_text("2\n0 0 0\n")
+    with pytest.raises(ValueError):
+        systems.read_posgrid(bad)
+    out = tmp_path / "dpmvpol.dat"
+    res = [{"vpol": -0.125, "alfa": 0.0, "e_first": -10.5}, {"vpol": -0.0625, "alfa": 2.0, "e_first": -10.25},
+           {"vpol": -0.03125, "alfa": 3.5, "e_first": -10.125}]
+    systems.write_dpmvpol(out, pts, res, -11.0)
+    lines = out.read_text().splitlines()
+    assert len(lines) == 3
+    assert lines[1] == "%15.6e %15.6e %15.6e %20.10e %15.6e %20.10e" % (0.0, 0.0, 2.0, -0.0625, 2.0, 0.75)
+    cols = [float(x) for x in lines[2].split()]
+    assert cols == [0.5, -0.4, 2.5, -0.03125, 3.5, 0.875]

@@ -259,6 +259,13 @@ int patmol_hip_batch_rhf_scf(PatmolHipBatch *b, const double *S, const double *H
  * the basis (grad has 3 * number_of_centers() doubles, accumulated).  FP64,
  * reproducible bit for bit. */
 int patmol_hip_eri_gradient(const ERIFS *tfiles, const double *Pmat, double *grad);
+/* The one-electron part of the same gradient (no counterpart in the
+ * reference): grad[3 c + x] += 2 sum P dH/dR - 2 sum W dS/dR for the packed
+ * half density P and energy-weighted density W of the current basis, H = T
+ * + V without skip_center()'s attraction (V's derivative includes the
+ * nuclei's own Hellmann-Feynman term).  One GPU thread per shell pair;
+ * fixed-point sums, reproducible. */
+int patmol_hip_onee_gradient(const double *Pmat, const double *Wmat, double *grad);
 /* The DPM potential's part of the gradient (no counterpart in the
  * reference): for the packed half density P of the current basis,
  * grad[3 c + x] += d/dR_{c,x} of -2 sum_ab P_ab V_ab, V_ab the dpm_ints

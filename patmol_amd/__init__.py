@@ -181,6 +181,8 @@ def lib():
         H.patmol_nuclear_repulsion_gradient.argtypes = [_dp]
         G.patmol_hip_eri_gradient.restype = C.c_int
         G.patmol_hip_eri_gradient.argtypes = [C.c_void_p, _dp, _dp]
+        G.patmol_hip_onee_gradient.restype = C.c_int
+        G.patmol_hip_onee_gradient.argtypes = [_dp, _dp, _dp]
         G.patmol_hip_dpm_gradient.restype = C.c_int
         G.patmol_hip_dpm_gradient.argtypes = [_dp, _dp]
         G.patmol_hip_batch_new.restype = C.c_void_p
@@ -227,6 +229,7 @@ def exported_symbols():
             "patmol_hip_uhf_gmatrix_device", "patmol_hip_set_scf_density", "patmol_hip_set_scf_incremental", "momints", "patmol_hip_moment_ints", "patmol_hip_write_momints",
             "patmol_hip_write_erints", "patmol_hip_rhf_gmatrix_erints", "patmol_hip_rhf_gmatrix_erint_file",
             "patmol_hip_uhf_gmatrix_erints", "patmol_hip_uhf_gmatrix_erint_file", "patmol_hip_eri_gradient",
+            "patmol_hip_onee_gradient",
             "patmol_hip_dpm_gradient",
             "patmol_hip_batch_new", "patmol_hip_batch_add", "patmol_hip_batch_finalize", "patmol_hip_batch_gmatrix",
             "patmol_hip_batch_buffers", "patmol_hip_batch_launch", "patmol_hip_batch_info", "patmol_hip_batch_free",
@@ -733,6 +736,17 @@ def eri_gradient(P: np.ndarray, store: "EriStore") -> np.ndarray:
     g = np.zeros(3 * nc)
     if G.patmol_hip_eri_gradient(store.h, np.ascontiguousarray(P, dtype=np.float64), g):
         _check("patmol_hip_eri_gradient")
+    return g.reshape(nc, 3)
+
+
+def onee_gradient(P: np.ndarray, W: np.ndarray) -> np.ndarray:
+    """The one-electron part 2 sum P dH/dR - 2 sum W dS/dR on the GPU
+    (patmol_hip_onee_gradient), ncen x 3."""
+    H, G = lib()
+    nc = H.number_of_centers()
+    g = np.zeros(3 * nc)
+    if G.patmol_hip_onee_gradient(np.ascontiguousarray(P, dtype=np.float64), np.ascontiguousarray(W, dtype=np.float64), g):
+        _check("patmol_hip_onee_gradient")
     return g.reshape(nc, 3)
 
 

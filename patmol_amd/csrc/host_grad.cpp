@@ -301,7 +301,7 @@ static int rhf_gradient(const ERIFS *mints, const double *P, double *grad, bool 
         }
     for (int k = 0; k < 3 * nc; ++k) grad[k] = 0.0;
     if (!rc && patmol_hip_eri_gradient(mints, P, grad)) rc = 2;
-    if (!rc && patmol_host_onee_gradient(P, W.data(), grad)) rc = 3;
+    if (!rc && patmol_hip_onee_gradient(P, W.data(), grad)) rc = 2;
     if (!rc) patmol_nuclear_repulsion_gradient(grad);
     if (!rc && dpm && patmol_hip_dpm_gradient(P, grad)) rc = 2;
     if (own) clean_ERIFS(own);

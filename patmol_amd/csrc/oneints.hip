@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "eri_engine.h"
+#include "eri_kernels.h"
 #include "patmol_hip.h"
 
 extern "C" {
@@ -317,6 +318,205 @@ int launch_onee(int la, int lb, const OneArgs &args, hipStream_t s)
     }
 }
 
+// The one-electron part of the RHF nuclear gradient (no counterpart in the
+// reference): for the packed half density P and energy-weighted density W,
+// grad += 2 sum P dH/dR - 2 sum W dS/dR with H = T + V (V without
+// skip_center()'s attraction).  One thread per shell pair (both orders of an
+// off-diagonal pair: weight 2).  Per axis, with the Obara-Saika overlaps
+// S(i,j) and kinetic T(i,j) of the forward kernel, d/dA_x of a primitive
+// Cartesian Gaussian is 2 alpha G(a+1_x) - a_x G(a-1_x), d/dB = -d/dA for S
+// and T; V's derivatives come from the Rys point-charge tables one order
+// higher (d/dA and d/dB from the shifted exponents, the nucleus C taking
+// -(d/dA + d/dB), Hellmann-Feynman).  The per-nucleus sums are summed over
+// the wave in a fixed butterfly; every sum lands in fixed point (FxAcc), so
+// the gradient is reproducible.
+struct OneGradArgs {
+    OneArgs o;
+    const double *P, *W;       // packed lower triangles
+    const int *cen;            // centre of each shell
+    const int *nuc_cen;        // centre of each point nucleus
+    FxAcc *g;                  // 3 per centre
+};
+
+__device__ __forceinline__ void fx_add_dev(FxAcc *p, double v)
+{
+    if (v == 0.0) return;
+    const FxAcc s = fx_split(v);
+    if (s.hi != 0) __hip_atomic_fetch_add(&p->hi, s.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s.lo != 0) __hip_atomic_fetch_add(&p->lo, s.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void onee_grad_kernel(OneGradArgs g)
+{
+    const OneArgs &a = g.o;
+    constexpr int NA = ncart(LA), NB = ncart(LB), L1 = LA + LB + 1, NR = L1 / 2 + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool here = t < a.npairs;
+    const int2 sh = here ? a.pairs[t] : make_int2(0, 0);
+    const PairRef ref = here ? a.refs[t] : PairRef{0, 0};
+    double A[3], AB[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        A[x] = a.xyz[3 * sh.x + x];
+        AB[x] = A[x] - a.xyz[3 * sh.y + x];
+    }
+    const double pairw = sh.x == sh.y ? 1.0 : 2.0;
+    const int oI = a.off[sh.x], oJ = a.off[sh.y];
+    // 2 P and -2 W of this pair's components, with the pair weight and the
+    // component normalisation
+    double pw[NA * NB], ww[NA * NB];
+    static_for<NA>([&](auto ia) {
+        static_for<NB>([&](auto ib) {
+            constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+            constexpr double nrm = 1.0 / csqrt(cart_dfprod(LA, IA) * cart_dfprod(LB, IB));
+            const int r = oI + IA, c = oJ + IB, hi = r > c ? r : c, lo = r > c ? c : r;
+            const size_t idx = (size_t)hi * (hi + 1) / 2 + lo;
+            pw[IA * NB + IB] = here ? 2.0 * pairw * nrm * g.P[idx] : 0.0;
+            ww[IA * NB + IB] = here ? -2.0 * pairw * nrm * g.W[idx] : 0.0;
+        });
+    });
+    double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
+
+    // ---- overlap and kinetic ----
+    for (int k = 0; k < ref.npp; ++k) {
+        const PrimPair q = a.pp[ref.pp0 + k];
+        const double2 ex = a.ab[ref.pp0 + k];
+        const double hip = 0.5 * q.ip, al = ex.x, be = ex.y;
+        double o[3][LA + 3][LB + 2];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) overlap_1d<LA + 2, LB + 1>(o[x], q.P[x] - A[x], AB[x], hip);
+        const double s12 = PI32 * q.s * q.ip * sqrt(q.ip) * q.c;
+        auto kin = [&](const double(&ox)[LA + 3][LB + 2], int i, int j) {
+            double v = 2.0 * al * be * ox[i + 1][j + 1];
+            if (i) v -= double(i) * be * ox[i - 1][j + 1];
+            if (j) v -= double(j) * al * ox[i + 1][j - 1];
+            if (i && j) v += 0.5 * double(i * j) * ox[i - 1][j - 1];
+            return v;
+        };
+        static_for<NA>([&](auto ia) {
+            static_for<NB>([&](auto ib) {
+                constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+                constexpr int ea[3] = {cart_x(LA, IA), cart_y(LA, IA), cart_z(LA, IA)};
+                constexpr int eb[3] = {cart_x(LB, IB), cart_y(LB, IB), cart_z(LB, IB)};
+                double E[3], K[3], dE[3], dK[3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const int i = ea[x], j = eb[x];
+                    E[x] = o[x][i][j];
+                    K[x] = kin(o[x], i, j);
+                    dE[x] = 2.0 * al * o[x][i + 1][j] - (i ? double(i) * o[x][i - 1][j] : 0.0);
+                    dK[x] = 2.0 * al * kin(o[x], i + 1, j) - (i ? double(i) * kin(o[x], i - 1, j) : 0.0);
+                }
+                const double wS = ww[IA * NB + IB] * s12, wT = pw[IA * NB + IB] * s12;
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const int y = (x + 1) % 3, z = (x + 2) % 3;
+                    const double dS = dE[x] * E[y] * E[z];
+                    const double dT = dK[x] * E[y] * E[z] + dE[x] * (K[y] * E[z] + E[y] * K[z]);
+                    const double v = wS * dS + wT * dT;   // d/dA; d/dB = -d/dA
+                    gA[x] += v;
+                    gB[x] -= v;
+                }
+            });
+        });
+    }
+
+    // ---- nuclear attraction, one nucleus at a time ----
+    for (int c = 0; c < a.nnuc; ++c) {
+        const double4 n = a.nuc[c];
+        double gC[3] = {0, 0, 0};
+        for (int k = 0; k < ref.npp; ++k) {
+            const PrimPair q = a.pp[ref.pp0 + k];
+            const double2 ex = a.ab[ref.pp0 + k];
+            const double hip = 0.5 * q.ip, al = ex.x, be = ex.y;
+            const double PC[3] = {q.P[0] - n.y, q.P[1] - n.z, q.P[2] - n.w};
+            const double Tv = q.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+            double rx[NR], rw[NR];
+            rys_eval<NR>(Tv, rx, rw, a.rys);
+            const double qv = -n.x * TWOPI * q.ip * q.s * q.c;
+            double abp[3][LB + 2];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                abp[x][0] = 1.0;
+#pragma unroll
+                for (int i = 1; i <= LB + 1; ++i) abp[x][i] = abp[x][i - 1] * AB[x];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double b1 = (1.0 - rx[r]) * hip;
+                double G[3][L1 + 1];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const double C = q.P[x] - A[x] - rx[r] * PC[x];
+                    G[x][0] = x == 2 ? rw[r] * qv : 1.0;
+                    G[x][1] = C * G[x][0];
+#pragma unroll
+                    for (int i = 2; i <= L1; ++i) G[x][i] = C * G[x][i - 1] + double(i - 1) * b1 * G[x][i - 2];
+                }
+                // transfer: I_x(i, j) = sum_k C(j, k) AB^k G[i + j - k]
+                auto tr = [&](int x, int i, int j) {
+                    double v = 0.0;
+                    for (int kk = 0; kk <= j; ++kk) v += binom(j, kk) * abp[x][kk] * G[x][i + j - kk];
+                    return v;
+                };
+                static_for<NA>([&](auto ia) {
+                    static_for<NB>([&](auto ib) {
+                        constexpr int IA = decltype(ia)::value, IB = decltype(ib)::value;
+                        constexpr int ea[3] = {cart_x(LA, IA), cart_y(LA, IA), cart_z(LA, IA)};
+                        constexpr int eb[3] = {cart_x(LB, IB), cart_y(LB, IB), cart_z(LB, IB)};
+                        double I0[3];
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) I0[x] = tr(x, ea[x], eb[x]);
+                        const double f = pw[IA * NB + IB];
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) {
+                            const int y = (x + 1) % 3, z = (x + 2) % 3;
+                            const int i = ea[x], j = eb[x];
+                            double dA = 2.0 * al * tr(x, i + 1, j);
+                            if (i) dA -= double(i) * tr(x, i - 1, j);
+                            double dB = 2.0 * be * tr(x, i, j + 1);
+                            if (j) dB -= double(j) * tr(x, i, j - 1);
+                            const double o = f * I0[y] * I0[z];
+                            gA[x] += o * dA;
+                            gB[x] += o * dB;
+                            gC[x] -= o * (dA + dB);
+                        }
+                    });
+                });
+            }
+        }
+        // this nucleus's share: the wave's sum, then one lane into its centre
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) gC[x] += __shfl_xor(gC[x], m, 64);
+        if ((threadIdx.x & 63) == 0)
+            for (int x = 0; x < 3; ++x) fx_add_dev(&g.g[3 * g.nuc_cen[c] + x], gC[x]);
+    }
+    if (here) {
+        const int cA = g.cen[sh.x], cB = g.cen[sh.y];
+        for (int x = 0; x < 3; ++x) {
+            fx_add_dev(&g.g[3 * cA + x], gA[x]);
+            fx_add_dev(&g.g[3 * cB + x], gB[x]);
+        }
+    }
+}
+
+int launch_onee_grad(int la, int lb, const OneGradArgs &args, hipStream_t s)
+{
+    dim3 gr((args.o.npairs + 63) / 64), b(64);
+    switch (la * 3 + lb) {
+    case 0: hipLaunchKernelGGL((onee_grad_kernel<0, 0>), gr, b, 0, s, args); return 0;
+    case 3: hipLaunchKernelGGL((onee_grad_kernel<1, 0>), gr, b, 0, s, args); return 0;
+    case 4: hipLaunchKernelGGL((onee_grad_kernel<1, 1>), gr, b, 0, s, args); return 0;
+    case 6: hipLaunchKernelGGL((onee_grad_kernel<2, 0>), gr, b, 0, s, args); return 0;
+    case 7: hipLaunchKernelGGL((onee_grad_kernel<2, 1>), gr, b, 0, s, args); return 0;
+    case 8: hipLaunchKernelGGL((onee_grad_kernel<2, 2>), gr, b, 0, s, args); return 0;
+    default: return -1;
+    }
+}
+
 // defined in hip_context.cpp
 int aux_fail(int code, const char *msg);
 RysTables aux_tables();
@@ -495,6 +695,53 @@ extern "C" void one_electron_ints(double *Smat, double *Tmat, double *Hmat)
          hipMemcpy(Tmat, dS + no2, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess &&
          hipMemcpy(Hmat, dS + 2 * no2, sizeof(double) * no2, hipMemcpyDeviceToHost) == hipSuccess;
     if (!ok) aux_fail(PATMOL_HIP_EDEVICE, "one_electron_ints: device failure");
+}
+
+// grad[3 c + x] += 2 sum P dH/dR - 2 sum W dS/dR (packed half density P,
+// packed energy-weighted density W, H = T + V without skip_center()'s
+// attraction) on the GPU: onee_grad_kernel per pair class.
+extern "C" int patmol_hip_onee_gradient(const double *P, const double *W, double *grad)
+{
+    using namespace pm;
+    if (!P || !W || !grad) return aux_fail(PATMOL_HIP_EBADARG, "patmol_hip_onee_gradient: null argument");
+    RysTables rys = aux_tables();
+    if (!rys.cheb) return patmol_hip_last_error();
+    PairStage st;
+    if (stage_pairs(st, 0, "patmol_hip_onee_gradient")) return PATMOL_HIP_EDEVICE;
+    const Shell *sh = shells();
+    const int ns = number_of_shells(), nc = number_of_centers(), skip = skip_center();
+    std::vector<int> cen(ns), ncen;
+    for (int i = 0; i < ns; ++i) cen[i] = sh[i].cen;
+    for (int c = 0; c < nc; ++c)
+        if (c != skip) ncen.push_back(c);
+    const size_t no2 = st.no2;
+    char *buf = nullptr;
+    const size_t oP = 0, oW = align256(sizeof(double) * no2), oC = oW + align256(sizeof(double) * no2),
+                 oN = oC + align256(sizeof(int) * ns), oG = oN + align256(sizeof(int) * (ncen.size() + 1)),
+                 bytes = oG + sizeof(FxAcc) * 3 * (size_t)nc;
+    bool ok = hipMalloc(&buf, bytes) == hipSuccess;
+    ok = ok && hipMemcpy(buf + oP, P, sizeof(double) * no2, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(buf + oW, W, sizeof(double) * no2, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(buf + oC, cen.data(), sizeof(int) * ns, hipMemcpyHostToDevice) == hipSuccess &&
+         (ncen.empty() || hipMemcpy(buf + oN, ncen.data(), sizeof(int) * ncen.size(), hipMemcpyHostToDevice) == hipSuccess) &&
+         hipMemset(buf + oG, 0, sizeof(FxAcc) * 3 * (size_t)nc) == hipSuccess;
+    for (int c = 0; ok && c < 6; ++c) {
+        OneGradArgs ga;
+        ga.o = st.args(c, rys);
+        if (!ga.o.npairs) continue;
+        ga.P = (const double *)(buf + oP);
+        ga.W = (const double *)(buf + oW);
+        ga.cen = (const int *)(buf + oC);
+        ga.nuc_cen = (const int *)(buf + oN);
+        ga.g = (FxAcc *)(buf + oG);
+        ok = launch_onee_grad(kClassL[c][0], kClassL[c][1], ga, 0) == 0 && hipGetLastError() == hipSuccess;
+    }
+    std::vector<FxAcc> g(3 * (size_t)nc);
+    ok = ok && hipMemcpy(g.data(), buf + oG, sizeof(FxAcc) * g.size(), hipMemcpyDeviceToHost) == hipSuccess;
+    if (buf) (void)hipFree(buf);
+    if (!ok) return aux_fail(PATMOL_HIP_EDEVICE, "patmol_hip_onee_gradient: device failure");
+    for (size_t k = 0; k < g.size(); ++k) grad[k] += fx_value(g[k].hi, g[k].lo);
+    return 0;
 }
 
 // Dipole (3 * no2) and quadrupole (6 * no2: xx, xy, xz, yy, yz, zz) integrals

@@ -103,6 +103,31 @@ def test_eri_gradient_contracted_vs_oracle_finite_differences():
     assert np.abs(g.sum(axis=0)).max() < 1e-9
 
 
+@pytest.mark.parametrize("name", ["h2o_d", "contracted", "water16"])
+def test_onee_gradient_device_vs_host(name):
+    """patmol_hip_onee_gradient (one thread per shell pair, the attraction's
+    Rys roots per nucleus) against the host's patmol_host_onee_gradient,
+    which the CPU suite pins by finite differences of S and H, for random
+    symmetric P and W; the two sum in different orders, so the bar is
+    relative 1e-12."""
+    s = {"h2o_d": lambda: systems.water(d_shells=True), "contracted": _contracted_water,
+         "water16": systems.water16}[name]()
+    pm.load_system(s)
+    n = s.norb
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((n, n))
+    P = (A @ A.T / n)[np.tril_indices(n)].copy()
+    B = rng.standard_normal((n, n))
+    W = ((B + B.T) / n)[np.tril_indices(n)].copy()
+    gd = pm.onee_gradient(P, W)
+    gh = pm.host_onee_gradient(P, W)
+    err = float(np.abs(gd - gh).max())
+    note(f"{name} ({n} functions): one-electron gradient device vs host {err:.3e} (|g| max {np.abs(gh).max():.3f})")
+    assert err < 1e-12 * max(1.0, np.abs(gh).max())
+    # a second call on the same inputs gives the same bits (fixed-point sums)
+    assert np.array_equal(pm.onee_gradient(P, W), gd)
+
+
 def _device_energy(s, guess=None, precision=0, tau=1e-12):
     pm.load_system(s)
     pm.set_options(tau=tau, precision=precision, strict=False, scf_density=1)

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void grad2e_kernel(GradArgs a)
 
 // The same sum with one group of GL lanes (64, 32 or 16: one, two or four
 // groups per wave) per shell quartet, for the classes with at least
-// GRAD_GROUP_MIN Cartesian products and (ds|pp) (the others run the
+// GRAD_GROUP_MIN Cartesian products, (ds|pp) and (pp|pp) (the others run the
 // per-thread kernel):
 // lane g of a group owns the products c = g + GL k (a-major), whose Gamma_c
 // it writes into the group's LDS row once per quartet (six density values
@@ -267,19 +267,21 @@ constexpr int GRAD_GROUP_MIN = 108;
 // wave) for (dp|ds) (102 -> 94 ms) and (dp|pp) (43 -> 38 ms), a whole wave
 // for the others ((dd|ps) 59 ms with 64 lanes, 65 with 32); 16-lane groups
 // for the classes of 27-81 products measured slower than the per-thread
-// kernel ((ds|ds) 71 against 16 ms, (dd|ss) 35, (pp|ps) 49) except for
-// (ds|pp)
+// kernel ((ds|ds) 71 against 16 ms, (dd|ss) 35, (pp|ps) 49; (dp|ps) 72 with
+// 16 lanes and 87 with 32 against 70) except for (ds|pp) and (pp|pp)
 template <int LA, int LB, int LC, int LD>
 constexpr int grad_group_lanes()
 {
     if (LA == 2 && LB == 0 && LC == 1 && LD == 1) return 16;   // (ds|pp): 62 against 70 ms per-thread
+    if (LA == 1 && LB == 1 && LC == 1 && LD == 1) return 32;   // (pp|pp): 10.3 against 13.1 ms per-thread, 15.4 with 64
     return (LA == 2 && LB == 1 && ((LC == 2 && LD == 0) || (LC == 1 && LD == 1))) ? 32 : 64;
 }
 // the classes that run the group kernel
 template <int LA, int LB, int LC, int LD>
 constexpr bool grad_grouped()
 {
-    return ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD) >= GRAD_GROUP_MIN || (LA == 2 && LB == 0 && LC == 1 && LD == 1);
+    return ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD) >= GRAD_GROUP_MIN || (LA == 2 && LB == 0 && LC == 1 && LD == 1) ||
+           (LA == 1 && LB == 1 && LC == 1 && LD == 1);
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -33,6 +33,26 @@
  *                                                         incl. a dominant negative first component
  *   or_diag_fock        rhfscf.c:432-471, scfaux.c:5-41   reference spack_trans/spack_diag/
  *                                                         form_cmatrix/form_pmatrix (_ref)
+ * Helpers, each pinned against a pinned function above:
+ *   or_eigh             spack_math.c (Jacobi)             reference spack_diag eigenvalues and
+ *                                                         vectors (_ref)
+ *   or_nuclear_repulsion rhfscf.c:120-174 (E_nuc)         SF6 closed form from patin.dat; reference
+ *                                                         nuclear repulsion (_ref)
+ *   or_eri_count        twoints.c:228-337                 integer recount of the canonical filter
+ *   or_eri_block_noscreen                                 the same block as or_eri_block at tau 0
+ *   or_pair_schwarz     (bound of twoints.c:316-337)      sqrt of the largest diagonal element of
+ *                                                         or_eri_block_noscreen(I, J, I, J)
+ *   or_rhf_direct_gmatrix(_sampled), or_g_rows, or_rhf_form_gmatrix_mt
+ *                                                         equal to or_rhf_form_gmatrix's scatter of
+ *                                                         the same records (1e-12); the sampled form
+ *                                                         at tau 0, stride 1 equal to the full build
+ *   or_rhf_scf_ex       rhfscf.c:21-174,217-263           or_rhf_scf with a caller's H (DPM potential,
+ *                                                         finite field), a guess density and the first
+ *                                                         iteration's energy (vpol); finite-field
+ *                                                         -dE/dF against the dipole integrals
+ *   or_basis_set_center, or_basis_set_skipcen             the centre moves and skip_center of the
+ *                                                         DPM loop, checked through or_one_electron_ints
+ *                                                         and or_dpm_ints against the reference (_ref)
  */
 #include "oracle.h"
 #include <math.h>

@@ -22,6 +22,8 @@ Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
        python bench.py --profile-gradient SYSTEM   (the analytic gradient, twice)
        python bench.py --profile-build SYSTEM   (one FP64 build with classes
        serialised, for rocprofv3 runs; SYSTEM water8 or water16)
+       python bench.py --profile-build SYSTEM --profile-mixed   (the same,
+       one mixed-precision build at the benchmark's threshold)
 """
 import argparse
 import json
@@ -503,12 +505,13 @@ def calibrate_copy(nbytes):
     return 0
 
 
-def profile_build(name, fp32=False, md_mask=-1):
+def profile_build(name, fp32=False, md_mask=-1, mixed=False):
     """One G build of `name` with the class kernels serialised on one stream
     (after one warm-up build), for rocprofv3 kernel-trace and counter runs:
     FP64, or with fp32 every item in FP32 in the classes whose FP32 kernel
-    pays (the tuned policy); md_mask: the classes on the McMurchie-Davidson
-    kernel (-1: the library's default)."""
+    pays (the tuned policy), or with mixed the benchmark's mixed precision
+    (items below MIXED_THR in FP32); md_mask: the classes on the
+    McMurchie-Davidson kernel (-1: the library's default)."""
     import torch
     import patmol_amd as pm
     from patmol_amd import systems
@@ -521,6 +524,8 @@ def profile_build(name, fp32=False, md_mask=-1):
     store = pm.EriStore()
     if fp32:
         pm.set_options(precision=1, mixed_threshold=1e300)
+    elif mixed:
+        pm.set_options(precision=1, mixed_threshold=MIXED_THR)
     stream = torch.cuda.current_stream()
     dP = torch.tensor(P, dtype=torch.float64, device="cuda")
     dG = torch.zeros_like(dP)
@@ -528,7 +533,7 @@ def profile_build(name, fp32=False, md_mask=-1):
     st = store.stats()
     store.close()
     pm.set_options(precision=0)
-    print(json.dumps({"profile_build": name, "fp32": fp32, "ms": ms, "items_fp32": st["work_items_fp32"]}))
+    print(json.dumps({"profile_build": name, "fp32": fp32, "mixed": mixed, "ms": ms, "items_fp32": st["work_items_fp32"]}))
     return 0
 
 
@@ -569,6 +574,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--profile-build", default="", metavar="SYSTEM")
     ap.add_argument("--profile-fp32", action="store_true", help="with --profile-build: every item in FP32")
+    ap.add_argument("--profile-mixed", action="store_true",
+                    help="with --profile-build: the benchmark's mixed precision (items below MIXED_THR in FP32)")
     ap.add_argument("--md-engine", type=int, default=-1, metavar="MASK",
                     help="with --profile-build: classes on the McMurchie-Davidson kernel (1 (dd|dd), 2 (dd|dp), "
                          "4 (dp|dp), 8 (dd|pp); -1 the default)")
@@ -585,7 +592,7 @@ def main():
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.profile_build:
-        return profile_build(args.profile_build, args.profile_fp32, args.md_engine)
+        return profile_build(args.profile_build, args.profile_fp32, args.md_engine, args.profile_mixed)
     if args.profile_gradient:
         return profile_gradient(args.profile_gradient)
 
